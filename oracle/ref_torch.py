@@ -1,0 +1,158 @@
+"""FP32 op-for-op torch restatement of the reference acoustic PML path.
+
+TEST INFRASTRUCTURE: imported only by tests/, smoke() and bench.py's
+cpu_baseline leg; never by the product.
+
+It uses the same torch ops in the same order as the reference so that its
+FP32 rounding behaves like the reference's CPU path:
+* Laplacian  conv2d(u[:, None], h**-2 * K, padding=1)       acoustic.py:60-71
+* step       (dt**-2 + b*dt**-1).pow(-1) * (2/dt**2*h1
+               - (dt**-2 - b*dt**-1)*h2 + c.pow(2)*lap)       acoustic.py:81-84
+* source     Y = Y.clone(); Y += smask * w[n]                 source.py:49-52
+* receivers  u[bidx, z, x]                                    probe.py:42-44
+* loop       step -> source -> record                        rnn.py:178-212
+* misfit     sum of MSELoss(reduction='sum') over shots       loss.py:417-421
+
+``gradient_ad`` backpropagates through the whole loop (the reference's
+``boundary_saving: false`` path, cell.py:74-75).  ``gradient_bs`` trades
+memory for recompute the way the reference's boundary-saving path does
+(checkpoint_new.py:114-217): the forward keeps only the 1-cell ring at
+index 49/-50 per step, the backward rebuilds the interior in reverse time and
+differentiates one recomputed step at a time.  It follows the exact
+time pairing (not the reference's two-step shift, which is a defect of that
+path) so its interior gradient equals ``gradient_ad``'s.
+"""
+import torch
+import torch.nn.functional as F
+
+_K = torch.tensor([[0.0, 1.0, 0.0], [1.0, -4.0, 1.0], [0.0, 1.0, 0.0]])
+
+
+def laplacian(y, h):
+    op = (h ** (-2) * _K.to(y.dtype)).view(1, 1, 3, 3)
+    return F.conv2d(y.unsqueeze(1), op, padding=1).squeeze(1)
+
+
+def time_step(c, h1, h2, dt, h, b):
+    y = torch.mul((dt ** -2 + b * dt ** -1).pow(-1),
+                  (2 / dt ** 2 * h1 - torch.mul((dt ** -2 - b * dt ** -1), h2)
+                   + torch.mul(c.pow(2), laplacian(h1, h))))
+    return y, h1
+
+
+def _setup(c, d, dt, h, wavelet, src, rec, dtype):
+    c = torch.as_tensor(c).to(dtype)
+    d = torch.as_tensor(d).to(dtype)
+    dt_t = torch.tensor(dt, dtype=dtype)
+    h_t = torch.tensor(h, dtype=dtype)
+    w = torch.as_tensor(wavelet).to(dtype)
+    src = torch.as_tensor(src, dtype=torch.int64).reshape(-1, 2)
+    recs = [torch.as_tensor(r, dtype=torch.int64).reshape(-1, 2) for r in rec]
+    B = src.shape[0]
+    nz, nx = c.shape
+    smask = torch.zeros((B, nz, nx), dtype=dtype)
+    smask[torch.arange(B), src[:, 0], src[:, 1]] = 1.0
+    bidx = torch.cat([torch.full((r.shape[0],), b, dtype=torch.int64) for b, r in enumerate(recs)])
+    rz = torch.cat([r[:, 0] for r in recs])
+    rx = torch.cat([r[:, 1] for r in recs])
+    counts = [r.shape[0] for r in recs]
+    return c, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts
+
+
+def forward(c, d, dt, h, wavelet, src, rec, dtype=torch.float32, c_param=None):
+    """Records as a list of (nt, nrec_b, 1) tensors (rnn.py:207-212)."""
+    c0, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(c, d, dt, h, wavelet, src, rec, dtype)
+    c = c0 if c_param is None else c_param
+    B = smask.shape[0]
+    h1 = torch.zeros_like(smask)
+    h2 = torch.zeros_like(smask)
+    recs = []
+    for n in range(w.shape[0]):
+        h1, h2 = time_step(c, h1, h2, dt_t, h_t, d)
+        y = h1.clone()
+        y += smask * w[n]
+        h1 = y
+        recs.append(h1[bidx, rz, rx])
+    stacked = torch.stack(recs, 0).unsqueeze(2)
+    return list(torch.split(stacked, counts, dim=1))
+
+
+def l2(syn, obs):
+    loss = 0.0
+    for s, o in zip(syn, obs):
+        loss = loss + torch.nn.MSELoss(reduction='sum')(s, torch.as_tensor(o).to(s.dtype))
+    return loss
+
+
+def gradient_ad(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32):
+    """Autograd through the whole loop: returns (J, dJ/dc, records)."""
+    cp = torch.as_tensor(c).to(dtype).clone().requires_grad_(True)
+    syn = forward(c, d, dt, h, wavelet, src, rec, dtype=dtype, c_param=cp)
+    J = l2(syn, obs)
+    J.backward()
+    return J.detach(), cp.grad.detach(), [s.detach() for s in syn]
+
+
+def _ring_idx(nz, nx, npml=49):
+    z0, z1, x0, x1 = npml, nz - 1 - npml, npml, nx - 1 - npml
+    xs = torch.arange(x0 + 1, x1)
+    zs = torch.arange(z0 + 1, z1)
+    z = torch.cat([torch.full_like(xs, z0), torch.full_like(xs, z1), zs, zs])
+    x = torch.cat([xs, xs, torch.full_like(zs, x0), torch.full_like(zs, x1)])
+    return z, x
+
+
+def gradient_bs(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32, npml=49):
+    """Boundary-saving gradient by per-step recompute (CPU baseline shape).
+
+    Returns (J, dJ/dc on the interior [npml+1, -npml-1), records).
+    """
+    c, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(c, d, dt, h, wavelet, src, rec, dtype)
+    nz, nx = c.shape
+    nt = w.shape[0]
+    rzi, rxi = _ring_idx(nz, nx, npml)
+    rings = []
+    h1 = torch.zeros_like(smask)
+    h2 = torch.zeros_like(smask)
+    recs = []
+    with torch.no_grad():
+        for n in range(nt):
+            h1, h2 = time_step(c, h1, h2, dt_t, h_t, d)
+            h1 = h1 + smask * w[n]
+            recs.append(h1[bidx, rz, rx])
+            rings.append(h1[:, rzi, rxi].clone())
+    stacked = torch.stack(recs, 0).unsqueeze(2)
+    syn = list(torch.split(stacked, counts, dim=1))
+    J = l2(syn, obs)
+    res = 2.0 * (stacked - torch.cat([torch.as_tensor(o).to(dtype) for o in obs], 1))[..., 0]
+    grad = torch.zeros_like(c)
+    # state: u_cur = u^{n+1}, u_prev = u^n ; adjoint grads wrt them
+    u_next, u_cur = h1, h2
+    g_next = torch.zeros_like(smask)
+    g_cur = torch.zeros_like(smask)
+    lo, hi_z, hi_x = npml + 1, nz - npml - 1, nx - npml - 1
+    for n in range(nt - 1, -1, -1):
+        # adjoint of the record of u^{n+1}
+        g_next = g_next.index_put((bidx, rz, rx), res[n], accumulate=True)
+        # rebuild u^{n-1} on the interior from u^{n+1}, u^n (source removed)
+        with torch.no_grad():
+            u_src = u_next - smask * w[n]
+            prev = torch.zeros_like(u_cur)
+            if n > 1:  # u^0 = u^-1 = 0; rings[m] holds the ring of u^{m+1}
+                lap = laplacian(u_cur, h_t)
+                prev = 2 * u_cur - u_src + (c.pow(2) * dt_t ** 2) * lap
+                keep = torch.zeros_like(prev)
+                keep[:, lo:hi_z, lo:hi_x] = prev[:, lo:hi_z, lo:hi_x]
+                keep[:, rzi, rxi] = rings[n - 2]
+                prev = keep
+        cp = c.clone().requires_grad_(True)
+        ucur = u_cur.clone().requires_grad_(True)
+        uprev = prev.clone().requires_grad_(True)
+        y, _ = time_step(cp, ucur, uprev, dt_t, h_t, d)
+        y.backward(g_next)
+        grad += cp.grad
+        g_next, g_cur = g_cur + ucur.grad, uprev.grad
+        u_next, u_cur = u_cur, prev
+    mask = torch.zeros_like(grad)
+    mask[lo:hi_z, lo:hi_x] = 1.0
+    return J.detach(), grad * mask, syn
