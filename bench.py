@@ -1,0 +1,167 @@
+"""Benchmark: acoustic FWI gradient (forward + hand-written adjoint) on one MI355X.
+
+Workload (one "step"): the full L2-misfit gradient of 64 shots on an
+800 x 2400 model at dx = 10 m (900 x 2500 with the 50-cell PML), 8 Hz Ricker,
+dt = 0.8 ms, nt = 5000, 2400 surface receivers: forward loop with boundary
+saving, misfit, adjoint loop with reconstruction and gradient accumulation.
+Inputs (model, observed records) are resident in HBM before timing starts.
+
+Metric: Gcell-updates/s = shots * nz_pad * nx_pad * nt / wall-time of the step
+(each cell, step and shot counted once for forward+adjoint together).
+
+Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build measures one
+GPU only.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "Gcell-updates/s fwd+adjoint, % HBM roofline; wall-clock to target misfit"
+HBM_PEAK = 8.0e12  # B/s, MI355X
+BYTES_PER_CELL = {"k_fwd": 16.0, "k_bwd": 36.0}  # algorithmic bytes per cell-update
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-steps", type=int, default=60,
+                   help="time steps of the bounded CPU sample (1 shot, full grid)")
+    return p.parse_args()
+
+
+def build_problem(dev, seed=20230503):
+    import torch
+    from seistorch_amd import synthetic
+    from seistorch_amd.pml import generate_pml_coefficients_2d
+    from seistorch_amd.utils import pad_model, ricker_wave
+    nz_i, nx_i, nshots, nt, dt, h, fm = 800, 2400, 64, 5000, 0.8e-3, 10.0, 8.0
+    true, init = synthetic.anomaly_model(nz_i, nx_i, seed=seed)
+    vt = pad_model(true)
+    vi = pad_model(init)
+    nz, nx = vi.shape
+    src, rec = synthetic.surface_acquisition(nx_i, nshots, src_depth=2)
+    P = dict(nz=nz, nx=nx, B=nshots, nt=nt, dt=dt, h=h, fm=fm, nz_i=nz_i, nx_i=nx_i)
+    P["vp_true"] = torch.from_numpy(vt).to(dev)
+    P["vp_init"] = torch.from_numpy(vi).to(dev)
+    P["d"] = generate_pml_coefficients_2d((nz, nx), 50).to(dev)
+    P["w"] = ricker_wave(fm, dt, nt, delay=256).to(dev)
+    P["src"] = torch.from_numpy(src.astype(np.int32)).to(dev)
+    P["rec"] = torch.from_numpy(synthetic.rec_flat_index(rec, nz, nx)).to(dev)
+    P["src_np"], P["rec_np"] = src, rec
+    P["vp_init_np"], P["d_np"], P["w_np"] = vi, P["d"].cpu().numpy(), P["w"].cpu().numpy()
+    return P
+
+
+def gradient_step(P):
+    import torch
+    from seistorch_amd.equations2d import acoustic as ac
+    vp = P["vp_init"].detach().requires_grad_(True)
+    rec = ac.propagate(vp, P["d"], P["w"], P["src"], P["rec"], P["B"], P["dt"], P["h"])
+    loss = torch.sum((rec - P["obs"]) ** 2)
+    loss.backward()
+    return loss.detach(), vp.grad
+
+
+def cpu_baseline(P, steps):
+    """The FP32 torch restatement of the reference path on the host cores:
+    1 shot on the full padded grid for `steps` time steps (forward with ring
+    saving + per-step recompute backward), scaled per cell-update."""
+    import torch
+    from oracle import ref_torch
+    threads = torch.get_num_threads()  # honours OMP_NUM_THREADS (the host's CPU share)
+    w = P["w_np"][:steps]
+    src = P["src_np"][:1]
+    rec = P["rec_np"][:1]
+    obs = [torch.zeros((steps, len(rec[0]), 1))]
+    t0 = time.perf_counter()
+    ref_torch.gradient_bs(P["vp_init_np"], P["d_np"], P["dt"], P["h"], w, src, rec, obs)
+    el = time.perf_counter() - t0
+    cells = P["nz"] * P["nx"] * steps
+    return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 shot, {P['nz']}x{P['nx']} padded grid, {steps} of {P['nt']} steps, "
+                      f"fwd+adjoint (oracle/ref_torch.gradient_bs, torch {torch.__version__}), "
+                      f"{el:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    from seistorch_amd import _lib
+    dev = torch.device("cuda:0")
+    P = build_problem(dev)
+    with torch.no_grad():
+        from seistorch_amd.equations2d import acoustic as ac
+        P["obs"] = ac.propagate(P["vp_true"], P["d"], P["w"], P["src"], P["rec"], P["B"],
+                                P["dt"], P["h"])
+    torch.cuda.synchronize()
+    print(f"[bench] problem ready: {P['B']} shots, {P['nz']}x{P['nx']}, nt={P['nt']}",
+          file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        gradient_step(P)
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    lib = _lib.load()
+    lib.seis_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        loss, grad = gradient_step(P)
+        print(f"[bench] step {i + 1}/{args.steps} enqueued", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    lib.seis_profile_enable(0)
+    kern = {}
+    import ctypes
+    for name, kid in (("k_fwd", 0), ("k_bwd", 1)):
+        ms = ctypes.c_double(0.0)
+        cnt = ctypes.c_int64(0)
+        _lib.call("seis_profile_collect", kid, ctypes.byref(ms), ctypes.byref(cnt))
+        kern[name] = (ms.value, cnt.value)
+    cells = P["B"] * P["nz"] * P["nx"] * P["nt"]
+    value = cells * args.steps / el / 1e9
+    dom = max(kern, key=lambda k: kern[k][0])
+    tot_ms, n_launch = kern[dom]
+    avg_s = tot_ms / max(n_launch, 1) / 1e3
+    alg_bytes = BYTES_PER_CELL[dom] * P["B"] * P["nz"] * P["nx"]
+    achieved = alg_bytes / avg_s if avg_s > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(dom)
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded anomaly model, records from the true model)",
+        "config": {"workload": "acoustic FWI gradient, PML 50, fwd + boundary-saving adjoint",
+                   "grid": [P["nz_i"], P["nx_i"]], "padded_grid": [P["nz"], P["nx"]],
+                   "shots": P["B"], "nt": P["nt"], "dt": P["dt"], "dx": P["h"],
+                   "receivers_per_shot": P["nx_i"], "f_peak_hz": P["fm"]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch,
+                     "bytes_per_cell_update": BYTES_PER_CELL[dom]},
+        "kernels_ms_total": {k: round(v[0], 2) for k, v in kern.items()},
+        "loss": float(loss),
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(P, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

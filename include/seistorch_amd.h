@@ -1,0 +1,97 @@
+/* C-ABI of the seistorch_amd HIP library (gfx950 / MI355X).
+ *
+ * Plain pointers and sizes only.  Every pointer argument is device memory
+ * unless stated otherwise; `stream` is a hipStream_t passed as void*, and all
+ * work is enqueued on it (nothing here synchronises the device).  Wavefields
+ * are FP32 (B, nz, nx) with x contiguous, models and damping arrays (nz, nx),
+ * exactly the tensors the reference hands to an equation backend.
+ *
+ * Every `int` function returns 0 on success and a nonzero code otherwise
+ * (1 invalid argument, 2 HIP runtime error); seis_last_error() then returns a
+ * message for the calling thread.  The Python layer turns a nonzero code into
+ * RuntimeError, the reference's error type for bad tensors.
+ */
+#ifndef SEISTORCH_AMD_H
+#define SEISTORCH_AMD_H
+
+#include <stdint.h>
+
+#define SEIS_ABI_VERSION 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message for the last failing call on this thread. */
+const char* seis_last_error(void);
+int32_t seis_abi_version(void);
+
+/* Launch timing (no reference counterpart; used by the benchmark).  When
+ * enabled, the loop entry points record a HIP event pair on their stream
+ * around every launch of the kernels below; seis_profile_collect waits for
+ * the last event and returns the summed duration and the number of launches
+ * of one kernel since the previous collect. */
+#define SEIS_KERNEL_ACOUSTIC_FWD 0
+#define SEIS_KERNEL_ACOUSTIC_BWD 1
+#define SEIS_PROFILE_KERNELS 8
+int seis_profile_enable(int32_t on);
+int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
+
+/* Number of cells in the boundary-saving ring of an (nz, nx) grid: rows
+ * npml and nz-1-npml, columns npml and nx-1-npml, without the corners
+ * (the 5-point neighbours of the undamped interior).  -1 if the grid has no
+ * interior.  Replaces the ring shape of seistorch/equations2d/utils.py:50-69. */
+int64_t seis_ring_len(int64_t nz, int64_t nx, int32_t npml);
+
+/* ---------------- acoustic, constant density, PML ------------------------ */
+
+/* One time step y = A*(2/dt^2*h1 - B*h2 + vp^2*L h1), A = 1/(dt^-2 + d/dt),
+ * B = dt^-2 - d/dt.  Replaces seistorch/equations2d/acoustic.py:73-86
+ * (_time_step, whose conv2d Laplacian is acoustic.py:60-71).  y must not
+ * alias h1. */
+int seis_acoustic_step(const float* vp, const float* d, const float* h1, const float* h2,
+                       float* y, int64_t B, int64_t nz, int64_t nx, float dt, float h,
+                       void* stream);
+
+/* Vector-Jacobian product of seis_acoustic_step for cotangent gy of y (and
+ * gh1_pass of the pass-through output h1, may be NULL): writes gh1, gh2 and,
+ * if gvp != NULL, the shot-summed gvp (nz, nx).  Replaces torch.autograd
+ * through acoustic.py:81-84 (the reference's boundary_saving: false path,
+ * seistorch/cell.py:74-75). */
+int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, const float* gy,
+                           const float* gh1_pass, float* gh1, float* gh2, float* gvp, int64_t B,
+                           int64_t nz, int64_t nx, float dt, float h, void* stream);
+
+/* Whole forward loop (seistorch/rnn.py:178-212 with acoustic.py:73-86,
+ * source.py:47-57 and probe.py:42-44): nt steps from zero fields, the
+ * wavelet sample wavelet[n] (device, nt) added at src_zx[b] = (z, x) of each
+ * shot after step n, receivers rec_flat[r] (flat indices into (B, nz, nx))
+ * recorded into rec_out[n*R + r].  If ring != NULL the ring of u^{n+1} is
+ * stored in slot n of ring ((nt-1) x B x seis_ring_len) for n < nt-1 (the
+ * boundary saving of seistorch/checkpoint_new.py:114-143).  On return u_a
+ * holds u^{nt} and u_b holds u^{nt-1}.  shots_per_group shots share one
+ * thread (model reads amortised over them). */
+int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
+                          const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
+                          float* rec_out, float* ring, float* u_a, float* u_b, int64_t B,
+                          int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                          int32_t shots_per_group, void* stream);
+
+/* Adjoint of seis_acoustic_forward (replaces the boundary-saving backward of
+ * seistorch/checkpoint_new.py:146-217 and acoustic.py:88-137, with the exact
+ * time pairing): rec_grad[n*R + r] = dJ/d rec_out[n*R + r]; u_a/u_b hold the
+ * final forward state (consumed); mu_a, mu_b are (B, nz, nx) work buffers;
+ * grad_part is ceil(B/shots_per_group) x (nz, nx) work; grad (nz, nx) gets
+ * dJ/dvp on the interior [npml+1, n-npml-1) and zero elsewhere. */
+int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet, int64_t nt,
+                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
+                           const float* rec_grad, const float* ring, float* u_a, float* u_b,
+                           float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
+                           int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                           int32_t shots_per_group, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEISTORCH_AMD_H */

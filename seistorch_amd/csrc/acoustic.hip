@@ -1,0 +1,447 @@
+// Constant-density acoustic PML kernels for gfx950.
+//
+// Forward step (seistorch/equations2d/acoustic.py:73-86):
+//   y = A*(2/dt^2*u - B*u_prev + c^2*L u),  A = 1/(dt^-2 + d/dt),
+//   B = dt^-2 - d/dt, L = zero-padded 5-point Laplacian / h^2.
+// Loop order per step (seistorch/rnn.py:178-202): step, raw add of the
+// wavelet sample at each shot's source cell, record the receivers.
+//
+// Adjoint (exact transpose of that loop, mu = A*lambda):
+//   mu^k  = A*(s^k + 2/dt^2*mu^{k+1} - B*mu^{k+2} + L(c^2*mu^{k+1}))
+//   dJ/dc = sum_n 2c*(L u^n)*mu^{n+1}
+// The forward field is rebuilt backwards in time on the undamped interior
+// [npml+1, n-npml-1) from the 1-cell ring stored at index npml during the
+// forward (seistorch/equations2d/utils.py:50-90 stores the same ring), so the
+// gradient is exact on that interior and left at zero in the damped strips.
+//
+// Work decomposition: a 64x4 thread tile covers 64 contiguous x by 4 z rows;
+// each thread advances its cell for a group of S shots so that the model,
+// the damping and (in the adjoint) the per-group gradient are read/written
+// once per S shot-updates.  Neighbour reads hit L1/L2.
+#include "common.h"
+#include "seistorch_amd.h"
+
+namespace {
+
+constexpr int kTX = 64;
+constexpr int kTZ = 4;
+
+struct Grid {
+  int nz, nx;
+  int npml;
+  int64_t plane;  // nz*nx
+};
+
+__device__ __forceinline__ float lap5(const float* __restrict__ f, int z, int x, int i,
+                                      const Grid g, float ih2) {
+  const float c = f[i];
+  const float zm = z > 0 ? f[i - g.nx] : 0.0f;
+  const float zp = z < g.nz - 1 ? f[i + g.nx] : 0.0f;
+  const float xm = x > 0 ? f[i - 1] : 0.0f;
+  const float xp = x < g.nx - 1 ? f[i + 1] : 0.0f;
+  return ((zm + zp) + (xm + xp) - 4.0f * c) * ih2;
+}
+
+__device__ __forceinline__ bool interior(int z, int x, const Grid g) {
+  return z > g.npml && z < g.nz - 1 - g.npml && x > g.npml && x < g.nx - 1 - g.npml;
+}
+
+// j-th cell of the boundary-saving ring: top row, bottom row, left column,
+// right column, each restricted to the 5-point neighbours of the interior.
+__device__ __forceinline__ int64_t ring_cell(int64_t j, const Grid g) {
+  const int nxi = g.nx - 2 * g.npml - 2;
+  const int nzi = g.nz - 2 * g.npml - 2;
+  int z, x;
+  if (j < nxi) {
+    z = g.npml; x = g.npml + 1 + (int)j;
+  } else if (j < 2 * nxi) {
+    z = g.nz - 1 - g.npml; x = g.npml + 1 + (int)(j - nxi);
+  } else if (j < 2 * nxi + nzi) {
+    z = g.npml + 1 + (int)(j - 2 * nxi); x = g.npml;
+  } else {
+    z = g.npml + 1 + (int)(j - 2 * nxi - nzi); x = g.nx - 1 - g.npml;
+  }
+  return (int64_t)z * g.nx + x;
+}
+
+// ---------------------------------------------------------------- per step
+
+__global__ __launch_bounds__(256) void k_step(const float* __restrict__ vp,
+                                              const float* __restrict__ d,
+                                              const float* __restrict__ h1,
+                                              const float* __restrict__ h2,
+                                              float* __restrict__ y, Grid g,
+                                              seis::TimeConsts k) {
+  const int x = blockIdx.x * kTX + threadIdx.x;
+  const int z = blockIdx.y * kTZ + threadIdx.y;
+  if (x >= g.nx || z >= g.nz) return;
+  const int64_t off = (int64_t)blockIdx.z * g.plane;
+  const int i = z * g.nx + x;
+  const float c = vp[i];
+  const float dd = d[i];
+  const float A = 1.0f / (k.idt2 + dd * k.idt);
+  const float Bc = k.idt2 - dd * k.idt;
+  const float L = lap5(h1 + off, z, x, i, g, k.ih2);
+  y[off + i] = A * (k.two_idt2 * h1[off + i] - Bc * h2[off + i] + c * c * L);
+}
+
+// d/d(h1), d/d(h2) of one step: gh1 = 2/dt^2*A*gy + L(c^2*A*gy) + pass,
+// gh2 = -B*A*gy.
+__global__ __launch_bounds__(256) void k_step_vjp_fields(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ gy,
+    const float* __restrict__ pass, float* __restrict__ gh1, float* __restrict__ gh2, Grid g,
+    seis::TimeConsts k) {
+  const int x = blockIdx.x * kTX + threadIdx.x;
+  const int z = blockIdx.y * kTZ + threadIdx.y;
+  if (x >= g.nx || z >= g.nz) return;
+  const int64_t off = (int64_t)blockIdx.z * g.plane;
+  const float* gb = gy + off;
+  const int i = z * g.nx + x;
+  auto q = [&](int j) {
+    const float c = vp[j];
+    const float A = 1.0f / (k.idt2 + d[j] * k.idt);
+    return c * c * A * gb[j];
+  };
+  const float dd = d[i];
+  const float A = 1.0f / (k.idt2 + dd * k.idt);
+  const float Bc = k.idt2 - dd * k.idt;
+  const float qc = q(i);
+  const float qzm = z > 0 ? q(i - g.nx) : 0.0f;
+  const float qzp = z < g.nz - 1 ? q(i + g.nx) : 0.0f;
+  const float qxm = x > 0 ? q(i - 1) : 0.0f;
+  const float qxp = x < g.nx - 1 ? q(i + 1) : 0.0f;
+  const float Lq = ((qzm + qzp) + (qxm + qxp) - 4.0f * qc) * k.ih2;
+  const float ag = A * gb[i];
+  float v = k.two_idt2 * ag + Lq;
+  if (pass) v += pass[off + i];
+  gh1[off + i] = v;
+  gh2[off + i] = -Bc * ag;
+}
+
+// d/dc of one step summed over shots: 2c*A*(L h1)*gy.
+__global__ __launch_bounds__(256) void k_step_vjp_model(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ h1,
+    const float* __restrict__ gy, float* __restrict__ gvp, int B, Grid g, seis::TimeConsts k) {
+  const int x = blockIdx.x * kTX + threadIdx.x;
+  const int z = blockIdx.y * kTZ + threadIdx.y;
+  if (x >= g.nx || z >= g.nz) return;
+  const int i = z * g.nx + x;
+  const float c = vp[i];
+  const float A = 1.0f / (k.idt2 + d[i] * k.idt);
+  float acc = 0.0f;
+  for (int b = 0; b < B; ++b) {
+    const int64_t off = (int64_t)b * g.plane;
+    acc += lap5(h1 + off, z, x, i, g, k.ih2) * gy[off + i];
+  }
+  gvp[i] = 2.0f * c * A * acc;
+}
+
+// ------------------------------------------------------------ fused loop
+
+// u^{n+1} = step(u^n, u^{n-1}) + w[n] at the source cell, written in place
+// over u^{n-1} (each cell of u^{n-1} is read only by the thread that
+// overwrites it).
+__global__ __launch_bounds__(256) void k_fwd(const float* __restrict__ vp,
+                                             const float* __restrict__ d,
+                                             const float* __restrict__ u, float* __restrict__ up,
+                                             const int32_t* __restrict__ src,
+                                             const float* __restrict__ wn, int B, int S, Grid g,
+                                             seis::TimeConsts k) {
+  const int x = blockIdx.x * kTX + threadIdx.x;
+  const int z = blockIdx.y * kTZ + threadIdx.y;
+  if (x >= g.nx || z >= g.nz) return;
+  const int i = z * g.nx + x;
+  const float c = vp[i];
+  const float dd = d[i];
+  const float A = 1.0f / (k.idt2 + dd * k.idt);
+  const float Bc = k.idt2 - dd * k.idt;
+  const float c2 = c * c;
+  const int b0 = blockIdx.z * S;
+  const int b1 = min(B, b0 + S);
+  const float w = *wn;
+  for (int b = b0; b < b1; ++b) {
+    const int64_t off = (int64_t)b * g.plane;
+    const float L = lap5(u + off, z, x, i, g, k.ih2);
+    float v = A * (k.two_idt2 * u[off + i] - Bc * up[off + i] + c2 * L);
+    if (z == src[2 * b] && x == src[2 * b + 1]) v += w;
+    up[off + i] = v;
+  }
+}
+
+// Receivers of the new field into one record row, and its ring into one slot.
+__global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
+                                                const int64_t* __restrict__ rec, int64_t R,
+                                                float* __restrict__ rec_row,
+                                                float* __restrict__ ring_slot, int64_t ring_len,
+                                                int B, Grid g) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R) {
+    rec_row[t] = u[rec[t]];
+    return;
+  }
+  t -= R;
+  if (ring_slot && t < (int64_t)B * ring_len) {
+    const int64_t b = t / ring_len;
+    const int64_t j = t - b * ring_len;
+    ring_slot[t] = u[b * g.plane + ring_cell(j, g)];
+  }
+}
+
+// One reverse iteration k (see file header): mu^k over the whole grid into
+// m2 (was mu^{k+2}); on the interior the gradient term 2c*L(u^{k-1})*mu^k and
+// the rebuilt u^{k-2} = 2u^{k-1} - u^k + w[k-1]*src + c^2 dt^2 L u^{k-1} into
+// uc (was u^k).  recon: 0 none, 1 rebuild, 2 write zeros (u^0).
+__global__ __launch_bounds__(256) void k_bwd(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ um,
+    float* __restrict__ uc, const float* __restrict__ m1, float* __restrict__ m2,
+    float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
+    int recon, int B, int S, Grid g, seis::TimeConsts k) {
+  const int x = blockIdx.x * kTX + threadIdx.x;
+  const int z = blockIdx.y * kTZ + threadIdx.y;
+  if (x >= g.nx || z >= g.nz) return;
+  const int i = z * g.nx + x;
+  const float c = vp[i];
+  const float dd = d[i];
+  const float A = 1.0f / (k.idt2 + dd * k.idt);
+  const float Bc = k.idt2 - dd * k.idt;
+  const float c2 = c * c;
+  const float czm = z > 0 ? vp[i - g.nx] : 0.0f;
+  const float czp = z < g.nz - 1 ? vp[i + g.nx] : 0.0f;
+  const float cxm = x > 0 ? vp[i - 1] : 0.0f;
+  const float cxp = x < g.nx - 1 ? vp[i + 1] : 0.0f;
+  const float c2zm = czm * czm, c2zp = czp * czp, c2xm = cxm * cxm, c2xp = cxp * cxp;
+  const bool in = interior(z, x, g);
+  const int b0 = blockIdx.z * S;
+  const int b1 = min(B, b0 + S);
+  float gacc = 0.0f;
+  for (int b = b0; b < b1; ++b) {
+    const int64_t off = (int64_t)b * g.plane;
+    const float* m = m1 + off;
+    const float mzm = z > 0 ? m[i - g.nx] : 0.0f;
+    const float mzp = z < g.nz - 1 ? m[i + g.nx] : 0.0f;
+    const float mxm = x > 0 ? m[i - 1] : 0.0f;
+    const float mxp = x < g.nx - 1 ? m[i + 1] : 0.0f;
+    const float mc = m[i];
+    const float Lm =
+        ((c2zm * mzm + c2zp * mzp) + (c2xm * mxm + c2xp * mxp) - 4.0f * (c2 * mc)) * k.ih2;
+    const float mu = A * (k.two_idt2 * mc - Bc * m2[off + i] + Lm);
+    m2[off + i] = mu;
+    if (in) {
+      const float Lu = lap5(um + off, z, x, i, g, k.ih2);
+      gacc += Lu * mu;
+      if (recon == 1) {
+        float v = 2.0f * um[off + i] - uc[off + i] + c2 * k.dt2 * Lu;
+        if (z == src[2 * b] && x == src[2 * b + 1]) v += *wn;
+        uc[off + i] = v;
+      } else if (recon == 2) {
+        uc[off + i] = 0.0f;
+      }
+    }
+  }
+  if (in) gpart[(int64_t)blockIdx.z * g.plane + i] += 2.0f * c * gacc;
+}
+
+// After k_bwd: inject the adjoint source A*s^k at the receivers of mu^k (and
+// its gradient share 2c*L(u^{k-1})*A*s^k), restore the ring of u^{k-2}.
+__global__ __launch_bounds__(256) void k_bwd_post(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ m2, float* __restrict__ gpart,
+    const int64_t* __restrict__ rec, const float* __restrict__ rgrad, int64_t R,
+    const float* __restrict__ ring_slot, int64_t ring_len, int recon, int B, int S, Grid g,
+    seis::TimeConsts k) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R) {
+    const int64_t f = rec[t];
+    const int64_t b = f / g.plane;
+    const int i = (int)(f - b * g.plane);
+    const int z = i / g.nx;
+    const int x = i - z * g.nx;
+    const float A = 1.0f / (k.idt2 + d[i] * k.idt);
+    const float as = A * rgrad[t];
+    atomicAdd(m2 + f, as);
+    if (interior(z, x, g)) {
+      const float Lu = lap5(um + b * g.plane, z, x, i, g, k.ih2);
+      atomicAdd(gpart + (b / S) * g.plane + i, 2.0f * vp[i] * Lu * as);
+    }
+    return;
+  }
+  t -= R;
+  if (recon != 0 && t < (int64_t)B * ring_len) {
+    const int64_t b = t / ring_len;
+    const int64_t j = t - b * ring_len;
+    uc[b * g.plane + ring_cell(j, g)] = recon == 1 ? ring_slot[t] : 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill(float* __restrict__ p, int64_t n, float v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ __launch_bounds__(256) void k_sum_groups(const float* __restrict__ gpart, int G,
+                                                    float* __restrict__ out, int64_t plane) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= plane) return;
+  float s = 0.0f;
+  for (int q = 0; q < G; ++q) s += gpart[(int64_t)q * plane + t];
+  out[t] = s;
+}
+
+inline dim3 tile_grid(const Grid& g, int zdim) {
+  return dim3((g.nx + kTX - 1) / kTX, (g.nz + kTZ - 1) / kTZ, zdim);
+}
+
+inline int fill(float* p, int64_t n, float v, hipStream_t s) {
+  if (n <= 0) return 0;
+  k_fill<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+int check_grid(int64_t B, int64_t nz, int64_t nx, float dt, float h) {
+  SEIS_CHECK_ARG(B >= 1 && B <= 65535, "batch size %lld outside [1, 65535]", (long long)B);
+  SEIS_CHECK_ARG(nz >= 1 && nx >= 1, "empty grid %lld x %lld", (long long)nz, (long long)nx);
+  SEIS_CHECK_ARG(nz * nx < (int64_t)1 << 31, "grid %lld x %lld too large", (long long)nz,
+                 (long long)nx);
+  SEIS_CHECK_ARG(dt > 0.0f && h > 0.0f, "dt and h must be positive (dt=%g, h=%g)", dt, h);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t seis_ring_len(int64_t nz, int64_t nx, int32_t npml) {
+  const int64_t nxi = nx - 2 * (int64_t)npml - 2;
+  const int64_t nzi = nz - 2 * (int64_t)npml - 2;
+  if (npml < 0 || nxi < 1 || nzi < 1) return -1;
+  return 2 * nxi + 2 * nzi;
+}
+
+int seis_acoustic_step(const float* vp, const float* d, const float* h1, const float* h2,
+                       float* y, int64_t B, int64_t nz, int64_t nx, float dt, float h,
+                       void* stream) {
+  if (int e = check_grid(B, nz, nx, dt, h)) return e;
+  SEIS_CHECK_ARG(vp && d && h1 && h2 && y, "null pointer argument");
+  const Grid g{(int)nz, (int)nx, 0, nz * nx};
+  hipStream_t s = (hipStream_t)stream;
+  k_step<<<tile_grid(g, (int)B), dim3(kTX, kTZ), 0, s>>>(vp, d, h1, h2, y, g,
+                                                         seis::make_consts(dt, h));
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, const float* gy,
+                           const float* gh1_pass, float* gh1, float* gh2, float* gvp, int64_t B,
+                           int64_t nz, int64_t nx, float dt, float h, void* stream) {
+  if (int e = check_grid(B, nz, nx, dt, h)) return e;
+  SEIS_CHECK_ARG(vp && d && h1 && gy && gh1 && gh2, "null pointer argument");
+  const Grid g{(int)nz, (int)nx, 0, nz * nx};
+  const seis::TimeConsts k = seis::make_consts(dt, h);
+  hipStream_t s = (hipStream_t)stream;
+  k_step_vjp_fields<<<tile_grid(g, (int)B), dim3(kTX, kTZ), 0, s>>>(vp, d, gy, gh1_pass, gh1,
+                                                                    gh2, g, k);
+  SEIS_CHECK_LAUNCH();
+  if (gvp) {
+    k_step_vjp_model<<<tile_grid(g, 1), dim3(kTX, kTZ), 0, s>>>(vp, d, h1, gy, gvp, (int)B, g,
+                                                                k);
+    SEIS_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
+                          const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
+                          float* rec_out, float* ring, float* u_a, float* u_b, int64_t B,
+                          int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                          int32_t shots_per_group, void* stream) {
+  if (int e = check_grid(B, nz, nx, dt, h)) return e;
+  SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b, "null pointer argument");
+  SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
+  SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
+  SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1");
+  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml) : 0;
+  SEIS_CHECK_ARG(!ring || ring_len > 0, "grid %lld x %lld has no interior inside npml=%d",
+                 (long long)nz, (long long)nx, npml);
+  const Grid g{(int)nz, (int)nx, npml, nz * nx};
+  const seis::TimeConsts k = seis::make_consts(dt, h);
+  hipStream_t s = (hipStream_t)stream;
+  const int S = shots_per_group;
+  const int G = (int)((B + S - 1) / S);
+  // After nt steps u^{nt} must sit in u_a: step n writes into the buffer that
+  // holds u^{n-1}, so u^{n} lives in the step-0 "current" buffer iff n is even.
+  float* cur = (nt % 2 == 0) ? u_a : u_b;
+  float* prev = (nt % 2 == 0) ? u_b : u_a;
+  if (int e = fill(u_a, B * g.plane, 0.0f, s)) return e;
+  if (int e = fill(u_b, B * g.plane, 0.0f, s)) return e;
+  const int64_t aux = R + ((ring && nt > 1) ? B * ring_len : 0);
+  for (int64_t n = 0; n < nt; ++n) {
+    SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
+    k_fwd<<<tile_grid(g, G), dim3(kTX, kTZ), 0, s>>>(vp, d, cur, prev, src_zx, wavelet + n, (int)B, S,
+                                                    g, k);
+    SEIS_CHECK_LAUNCH();
+    SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
+    float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
+    const int64_t work = R + (slot ? B * ring_len : 0);
+    if (work > 0 && aux > 0) {
+      k_record<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(
+          prev, rec_flat, R, R ? rec_out + n * R : nullptr, slot, ring_len, (int)B, g);
+      SEIS_CHECK_LAUNCH();
+    }
+    float* t = cur; cur = prev; prev = t;
+  }
+  return 0;
+}
+
+int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet, int64_t nt,
+                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
+                           const float* rec_grad, const float* ring, float* u_a, float* u_b,
+                           float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
+                           int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                           int32_t shots_per_group, void* stream) {
+  if (int e = check_grid(B, nz, nx, dt, h)) return e;
+  SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b && mu_a && mu_b && grad_part && grad,
+                 "null pointer argument");
+  SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
+  SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_grad)), "bad receiver arguments");
+  SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1");
+  const int64_t ring_len = seis_ring_len(nz, nx, npml);
+  SEIS_CHECK_ARG(ring_len > 0, "grid %lld x %lld has no interior inside npml=%d", (long long)nz,
+                 (long long)nx, npml);
+  SEIS_CHECK_ARG(ring || nt <= 2, "ring storage is required for nt > 2");
+  const Grid g{(int)nz, (int)nx, npml, nz * nx};
+  const seis::TimeConsts k = seis::make_consts(dt, h);
+  hipStream_t s = (hipStream_t)stream;
+  const int S = shots_per_group;
+  const int G = (int)((B + S - 1) / S);
+  if (int e = fill(mu_a, B * g.plane, 0.0f, s)) return e;
+  if (int e = fill(mu_b, B * g.plane, 0.0f, s)) return e;
+  if (int e = fill(grad_part, (int64_t)G * g.plane, 0.0f, s)) return e;
+  float* X = u_b;   // u^{k-1}
+  float* Y = u_a;   // u^k  -> u^{k-2}
+  float* M1 = mu_a; // mu^{k+1}
+  float* M2 = mu_b; // mu^{k+2} -> mu^k
+  for (int64_t kk = nt; kk >= 1; --kk) {
+    const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
+    SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
+    k_bwd<<<tile_grid(g, G), dim3(kTX, kTZ), 0, s>>>(vp, d, X, Y, M1, M2, grad_part, src_zx,
+                                                    wavelet + (kk - 1), recon, (int)B, S, g, k);
+    SEIS_CHECK_LAUNCH();
+    SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
+    const int64_t work = R + (recon ? B * ring_len : 0);
+    if (work > 0) {
+      const float* slot = recon == 1 ? ring + (kk - 3) * B * ring_len : nullptr;
+      k_bwd_post<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(
+          vp, d, X, Y, M2, grad_part, rec_flat, R ? rec_grad + (kk - 1) * R : nullptr, R, slot,
+          ring_len, recon, (int)B, S, g, k);
+      SEIS_CHECK_LAUNCH();
+    }
+    float* t = X; X = Y; Y = t;
+    t = M1; M1 = M2; M2 = t;
+  }
+  k_sum_groups<<<(unsigned)((g.plane + 255) / 256), 256, 0, s>>>(grad_part, G, grad, g.plane);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // extern "C"

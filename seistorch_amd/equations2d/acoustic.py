@@ -1,0 +1,200 @@
+"""Constant-density acoustic equation with PML, backed by HIP kernels.
+
+Drop-in for ``seistorch.equations2d.acoustic`` (seistorch/equations2d/acoustic.py):
+the same three functions with the same positional arguments,
+
+    _time_step(vp, h1, h2, dt, h, d, habcs=None)              -> (y, h1)
+    _time_step_backward(vp, h1, h2, dt, h, d, rings, source)   -> (y, h1)
+    _time_step_backward_multiple(...)                          -> (y, h1)
+
+so ``equation: acoustic`` in a YAML config selects it.  The per-step
+functions are differentiable (the reference's ``boundary_saving: false`` path
+backpropagates through them, seistorch/cell.py:74-75); their gradient is the
+hand-written VJP kernel, not autograd of elementwise ops.
+
+``AcousticLoop`` is the fused alternative used by ``WaveRNN`` when the
+configuration allows it: the whole time loop (step, source, receivers, ring
+saving) in one library call, and its backward the hand-written adjoint with
+boundary-saving reconstruction, replacing autograd-through-the-loop
+(seistorch/checkpoint_new.py:109-217).
+"""
+import torch
+
+from .. import _lib
+from .utils import restore_boundaries
+
+NPML = 49  # ring index of the boundary saving (seistorch/equations2d/utils.py:50)
+
+
+def _f(x):
+    return float(x.item()) if torch.is_tensor(x) else float(x)
+
+
+class _StepFn(torch.autograd.Function):
+    """y = A*(2/dt^2*h1 - B*h2 + vp^2*L h1) (acoustic.py:81-84) and its VJP."""
+
+    @staticmethod
+    def forward(ctx, vp, h1, h2, d, dt, h):
+        _lib.require_device(vp, h1, h2, d)
+        if h1.shape != h2.shape or h1.dim() != 3 or vp.shape != h1.shape[1:] or d.shape != vp.shape:
+            raise RuntimeError(f"acoustic step: shape mismatch vp{tuple(vp.shape)} "
+                               f"h1{tuple(h1.shape)} h2{tuple(h2.shape)} d{tuple(d.shape)}")
+        B, nz, nx = h1.shape
+        y = torch.empty_like(h1)
+        _lib.call("seis_acoustic_step", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(h1), _lib.ptr(h2),
+                  _lib.ptr(y), B, nz, nx, dt, h, _lib.stream_of(h1))
+        ctx.save_for_backward(vp, d, h1)
+        ctx.dt, ctx.h = dt, h
+        return y
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, gy):
+        vp, d, h1 = ctx.saved_tensors
+        gy = gy.contiguous()
+        B, nz, nx = h1.shape
+        gh1 = torch.empty_like(h1)
+        gh2 = torch.empty_like(h1)
+        gvp = torch.empty_like(vp) if ctx.needs_input_grad[0] else None
+        _lib.call("seis_acoustic_step_vjp", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(h1),
+                  _lib.ptr(gy), None, _lib.ptr(gh1), _lib.ptr(gh2), _lib.ptr(gvp), B, nz, nx,
+                  ctx.dt, ctx.h, _lib.stream_of(h1))
+        return gvp, gh1, gh2, None, None, None
+
+
+def step(vp, h1, h2, d, dt, h):
+    return _StepFn.apply(vp, h1.contiguous(), h2.contiguous(), d, _f(dt), _f(h))
+
+
+def _time_step(*args, **kwargs):
+    """acoustic.py:73-86: returns (y, h1)."""
+    c = args[0]
+    h1, h2 = args[1:3]
+    dt, h, b = args[3:6]
+    return step(c, h1, h2, b, dt, h), h1
+
+
+def _backward_common(args, multiple):
+    vp = args[0]
+    h1, h2 = args[1:3]
+    dt, h, b = args[3:6]
+    h_bd, _ = args[-2]
+    src_type, src_func, src_values = args[-1]
+    y = step(vp, h1, h2, b, dt, h)
+    with torch.no_grad():
+        y = restore_boundaries(y, h_bd, multiple=multiple)
+    y = src_func(y, src_values, 1)
+    return y, h1
+
+
+def _time_step_backward(*args, **kwargs):
+    """acoustic.py:88-112: reverse step, ring restore, source re-add."""
+    return _backward_common(args, multiple=False)
+
+
+def _time_step_backward_multiple(*args, **kwargs):
+    """acoustic.py:114-137: as above with the top ring left alone."""
+    return _backward_common(args, multiple=True)
+
+
+# ----------------------------------------------------------------- fused loop
+
+def ring_len(nz, nx, npml=NPML):
+    return int(_lib.load().seis_ring_len(nz, nx, npml))
+
+
+def default_shots_per_group(B):
+    """Shots advanced by one thread: amortises model and gradient traffic."""
+    for s in (4, 2):
+        if B % s == 0:
+            return s
+    return 1
+
+
+def _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad):
+    """Host-side bounds checks: the kernels index with these values unchecked."""
+    nz, nx = vp.shape
+    if vp.dim() != 2 or d.shape != vp.shape:
+        raise RuntimeError(f"vp {tuple(vp.shape)} and d {tuple(d.shape)} must be equal (nz, nx)")
+    if wavelet.dim() != 1 or wavelet.shape[0] < 1 or wavelet.dtype != torch.float32:
+        raise RuntimeError("wavelet must be a non-empty float32 vector (nt,)")
+    if src_zx.dtype != torch.int32 or tuple(src_zx.shape) != (B, 2):
+        raise RuntimeError(f"src_zx must be int32 ({B}, 2), got {src_zx.dtype} {tuple(src_zx.shape)}")
+    if rec_flat.dtype != torch.int64 or rec_flat.dim() != 1:
+        raise RuntimeError("rec_flat must be an int64 vector")
+    lo = torch.stack([src_zx.min(), src_zx[:, 0].max() - nz, src_zx[:, 1].max() - nx])
+    bad_src = bool((lo[0] < 0) | (lo[1] >= 0) | (lo[2] >= 0))
+    bad_rec = rec_flat.numel() > 0 and bool((rec_flat.min() < 0) | (rec_flat.max() >= B * nz * nx))
+    if bad_src:
+        raise RuntimeError(f"a source lies outside the {nz}x{nx} grid")
+    if bad_rec:
+        raise RuntimeError(f"a receiver index lies outside the ({B}, {nz}, {nx}) fields")
+    if need_grad:
+        m = NPML + 1
+        if nz <= 2 * m or nx <= 2 * m:
+            raise RuntimeError(f"grid {nz}x{nx} has no interior inside the 50-cell boundary")
+        if bool(d[m:nz - m, m:nx - m].abs().max() != 0):
+            raise RuntimeError("boundary-saving adjoint needs d == 0 inside the 50-cell boundary")
+
+
+class AcousticLoop(torch.autograd.Function):
+    """Whole forward loop -> records (nt, R); backward = hand-written adjoint.
+
+    Inputs: vp, d (nz, nx); wavelet (nt,); src_zx (B, 2) int32 padded (z, x);
+    rec_flat (R,) int64 flat indices into (B, nz, nx); B; dt; h.
+    """
+
+    @staticmethod
+    def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group):
+        _lib.require_device(vp, d, wavelet, src_zx, rec_flat)
+        nz, nx = vp.shape
+        nt = wavelet.shape[0]
+        R = rec_flat.shape[0]
+        S = shots_per_group or default_shots_per_group(B)
+        need_grad = ctx.needs_input_grad[0]
+        _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad)
+        u_a = torch.empty((B, nz, nx), device=vp.device, dtype=torch.float32)
+        u_b = torch.empty_like(u_a)
+        rec = torch.empty((nt, R), device=vp.device, dtype=torch.float32)
+        ring = None
+        if need_grad and nt > 1:
+            L = ring_len(nz, nx)
+            if L <= 0:
+                raise RuntimeError(f"grid {nz}x{nx} has no interior inside the 50-cell boundary")
+            ring = torch.empty((nt - 1, B, L), device=vp.device, dtype=torch.float32)
+        _lib.call("seis_acoustic_forward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
+                  _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
+                  _lib.ptr(u_a), _lib.ptr(u_b), B, nz, nx, dt, h, NPML, S, _lib.stream_of(vp))
+        if need_grad:
+            ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
+            ctx.state = (ring, u_a, u_b)
+            ctx.meta = (B, dt, h, S)
+        return rec
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, grad_rec):
+        vp, d, wavelet, src_zx, rec_flat = ctx.saved_tensors
+        ring, u_a, u_b = ctx.state
+        ctx.state = None  # the backward consumes the final forward state
+        B, dt, h, S = ctx.meta
+        nz, nx = vp.shape
+        nt = wavelet.shape[0]
+        R = rec_flat.shape[0]
+        G = (B + S - 1) // S
+        grad_rec = grad_rec.contiguous().to(torch.float32)
+        mu_a = torch.empty_like(u_a)
+        mu_b = torch.empty_like(u_a)
+        gpart = torch.empty((G, nz, nx), device=vp.device, dtype=torch.float32)
+        grad = torch.empty_like(vp)
+        _lib.call("seis_acoustic_backward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
+                  _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(grad_rec), _lib.ptr(ring),
+                  _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(mu_a), _lib.ptr(mu_b), _lib.ptr(gpart),
+                  _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, _lib.stream_of(vp))
+        return grad, None, None, None, None, None, None, None, None
+
+
+def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None):
+    """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint."""
+    return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
+                              rec_flat.contiguous(), int(B), _f(dt), _f(h), shots_per_group)

@@ -1,0 +1,222 @@
+"""Parity of the HIP acoustic path against the FP64 oracle and the FP32 torch
+restatement of the reference path (same seeded inputs).
+
+Tolerances (relative L2): records and interior gradients within 1e-5 of the
+FP32 reference restatement and of the FP64 oracle; the measured errors are
+printed so the per-case numbers show up in the test log.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import acoustic as oa
+from oracle import ref_torch
+from tests.cases import acoustic_case, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _hip_records(t, c, S=None, vp=None):
+    from seistorch_amd.equations2d import acoustic as ac
+    vp = t["vp"] if vp is None else vp
+    rec = ac.propagate(vp, t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h, S)
+    torch.cuda.synchronize()
+    return rec
+
+
+def _split(rec, recs):
+    out, o = [], 0
+    for r in recs:
+        out.append(rec[:, o:o + len(r)])
+        o += len(r)
+    return out
+
+
+def test_single_step_matches_reference_and_oracle():
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case()
+    rng = np.random.default_rng(0)
+    h1 = rng.standard_normal((3, c.nz, c.nx)).astype(np.float32)
+    h2 = rng.standard_normal((3, c.nz, c.nx)).astype(np.float32)
+    y, h1o = ac._time_step(torch.from_numpy(c.vp).cuda(), torch.from_numpy(h1).cuda(),
+                           torch.from_numpy(h2).cuda(), torch.tensor(c.dt), torch.tensor(c.h),
+                           torch.from_numpy(c.d).cuda())
+    y = y.cpu().numpy()
+    yr, _ = ref_torch.time_step(torch.from_numpy(c.vp), torch.from_numpy(h1), torch.from_numpy(h2),
+                                torch.tensor(c.dt), torch.tensor(c.h), torch.from_numpy(c.d))
+    yo = oa.step(c.vp.astype(np.float64), h1.astype(np.float64), h2.astype(np.float64),
+                 c.d.astype(np.float64), c.dt, c.h)
+    e_ref, e_or = rel_l2(y, yr.numpy()), rel_l2(y, yo)
+    print(f"step: rel-L2 vs ref_torch {e_ref:.2e}, vs FP64 oracle {e_or:.2e}")
+    assert e_ref < 1e-6 and e_or < 1e-6
+    assert h1o.data_ptr() != 0
+
+
+def test_single_step_vjp_matches_autograd():
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case()
+    rng = np.random.default_rng(1)
+    h1 = rng.standard_normal((2, c.nz, c.nx))
+    h2 = rng.standard_normal((2, c.nz, c.nx))
+    gy = rng.standard_normal((2, c.nz, c.nx))
+    # FP64 autograd of the op-for-op restatement
+    vp64 = torch.from_numpy(c.vp.astype(np.float64)).requires_grad_(True)
+    a1 = torch.from_numpy(h1).requires_grad_(True)
+    a2 = torch.from_numpy(h2).requires_grad_(True)
+    y, _ = ref_torch.time_step(vp64, a1, a2, torch.tensor(c.dt, dtype=torch.float64),
+                               torch.tensor(c.h, dtype=torch.float64),
+                               torch.from_numpy(c.d.astype(np.float64)))
+    y.backward(torch.from_numpy(gy))
+    vp = torch.from_numpy(c.vp).cuda().requires_grad_(True)
+    b1 = torch.from_numpy(h1.astype(np.float32)).cuda().requires_grad_(True)
+    b2 = torch.from_numpy(h2.astype(np.float32)).cuda().requires_grad_(True)
+    yh, _ = ac._time_step(vp, b1, b2, c.dt, c.h, torch.from_numpy(c.d).cuda())
+    yh.backward(torch.from_numpy(gy.astype(np.float32)).cuda())
+    errs = [rel_l2(b1.grad.cpu(), a1.grad), rel_l2(b2.grad.cpu(), a2.grad),
+            rel_l2(vp.grad.cpu(), vp64.grad)]
+    print("vjp rel-L2 (h1, h2, vp):", ["%.2e" % e for e in errs])
+    assert max(errs) < 1e-5
+
+
+@pytest.mark.parametrize("S", [None, 1, 2, 4])
+def test_fused_forward_records(S):
+    c = acoustic_case(nshots=5)
+    t = to_dev(c)
+    rec = _hip_records(t, c, S).cpu().numpy()
+    ro = oa.forward(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec)
+    rr = ref_torch.forward(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec)
+    e_or = rel_l2(rec, np.concatenate([x[..., 0] for x in ro], 1))
+    e_ref = rel_l2(rec, torch.cat([x[..., 0] for x in rr], 1).numpy())
+    e_ref_or = rel_l2(torch.cat([x[..., 0] for x in rr], 1).numpy(),
+                      np.concatenate([x[..., 0] for x in ro], 1))
+    print(f"records S={S}: HIP vs oracle {e_or:.2e}, HIP vs ref_torch {e_ref:.2e}, "
+          f"ref_torch vs oracle {e_ref_or:.2e}")
+    assert e_or < TOL and e_ref < TOL
+
+
+def test_per_step_interface_matches_fused_loop():
+    """The drop-in _time_step driven by a reference-style loop gives the fused records."""
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case(nshots=2, nt=200)
+    t = to_dev(c)
+    rec_fused = _hip_records(t, c).cpu().numpy()
+    B = c.B
+    smask = torch.zeros((B, c.nz, c.nx), device="cuda")
+    smask[torch.arange(B), t["src"][:, 0].long(), t["src"][:, 1].long()] = 1.0
+    h1 = torch.zeros_like(smask)
+    h2 = torch.zeros_like(smask)
+    flat = t["rec"]
+    recs = []
+    for n in range(c.nt):
+        h1, h2 = ac._time_step(t["vp"], h1, h2, c.dt, c.h, t["d"])
+        h1 = h1 + smask * t["w"][n]
+        recs.append(h1.reshape(-1)[flat])
+    rec_step = torch.stack(recs).cpu().numpy()
+    e = rel_l2(rec_step, rec_fused)
+    print(f"per-step vs fused: {e:.2e}")
+    assert e < 1e-6
+
+
+def _fused_grad(c, t, obs, S=None):
+    from seistorch_amd.equations2d import acoustic as ac
+    vp = t["vp"].clone().requires_grad_(True)
+    rec = ac.propagate(vp, t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h, S)
+    J = ((rec - obs) ** 2).sum()
+    J.backward()
+    torch.cuda.synchronize()
+    return float(J), vp.grad.cpu().numpy(), rec
+
+
+@pytest.mark.parametrize("S", [None, 1, 4])
+def test_fused_gradient_interior(S):
+    c = acoustic_case(nshots=5, nt=350)
+    t = to_dev(c)
+    obs_o = oa.forward(c.vp_true, c.d, c.dt, c.h, c.w, c.src, c.rec)
+    obs = torch.from_numpy(np.concatenate([x[..., 0] for x in obs_o], 1).astype(np.float32)).cuda()
+    J, g, _ = _fused_grad(c, t, obs, S)
+    Jo, go, _ = oa.gradient(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec,
+                            [x.astype(np.float32) for x in obs_o])
+    Jr, gr, _ = ref_torch.gradient_ad(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec,
+                                      [torch.from_numpy(x.astype(np.float32)) for x in obs_o])
+    I = (slice(50, -50), slice(50, -50))
+    e_or = rel_l2(g[I], go[I])
+    e_ref = rel_l2(g[I], gr.numpy()[I])
+    e_ref_or = rel_l2(gr.numpy()[I], go[I])
+    print(f"grad S={S}: J hip {J:.6e} oracle {Jo:.6e}; interior rel-L2 HIP vs oracle {e_or:.2e}, "
+          f"HIP vs ref_torch AD {e_ref:.2e}, ref_torch AD vs oracle {e_ref_or:.2e}")
+    assert abs(J - Jo) <= 1e-5 * abs(Jo)
+    assert e_or < TOL and e_ref < TOL
+    # outside the undamped interior the boundary-saving gradient is defined as zero
+    mask = np.ones_like(g, bool); mask[I] = False
+    assert np.all(g[mask] == 0.0)
+
+
+def test_gradient_finite_difference_fp32():
+    """Central differences of the HIP forward misfit at 5 random interior cells."""
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case(nz_i=40, nx_i=60, nshots=2, nt=300)
+    t = to_dev(c)
+    obs = ac.propagate(torch.from_numpy(c.vp_true).cuda(), t["d"], t["w"], t["src"], t["rec"],
+                       c.B, c.dt, c.h)
+    _, g, _ = _fused_grad(c, t, obs)
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        i = int(rng.integers(55, c.nz - 55)); j = int(rng.integers(55, c.nx - 55))
+        vals = []
+        for sgn in (1.0, -1.0):
+            vp = t["vp"].clone().double()
+            vp[i, j] += sgn * 0.5  # 0.5 m/s
+            rec = ac.propagate(vp.float(), t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h)
+            vals.append(float(((rec.double() - obs.double()) ** 2).sum()))
+        fd = (vals[0] - vals[1]) / 1.0
+        print(f"FD cell ({i},{j}): fd {fd:.6e} adjoint {g[i, j]:.6e}")
+        assert abs(fd - g[i, j]) <= 2e-2 * max(abs(fd), 1e-2 * np.abs(g).max())
+
+
+def test_edge_cases():
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case(nz_i=12, nx_i=20, nshots=1, nt=3)
+    t = to_dev(c)
+    for nt in (1, 2, 3):
+        w = t["w"][:nt].contiguous()
+        rec = ac.propagate(t["vp"], t["d"], w, t["src"], t["rec"], 1, c.dt, c.h)
+        ro = oa.forward(c.vp, c.d, c.dt, c.h, c.w[:nt], c.src, c.rec)[0][..., 0]
+        np.testing.assert_allclose(rec.cpu().numpy(), ro, rtol=1e-5, atol=1e-7 * np.abs(ro).max())
+        vp = t["vp"].clone().requires_grad_(True)
+        r = ac.propagate(vp, t["d"], w, t["src"], t["rec"], 1, c.dt, c.h)
+        (r ** 2).sum().backward()
+        _, go, _ = oa.gradient(c.vp, c.d, c.dt, c.h, c.w[:nt], c.src, c.rec,
+                               [np.zeros_like(ro)[..., None]])
+        I = (slice(50, -50), slice(50, -50))
+        np.testing.assert_allclose(vp.grad.cpu().numpy()[I], go[I], rtol=1e-4,
+                                   atol=1e-6 * max(np.abs(go[I]).max(), 1e-30))
+    # no receivers: runs, returns (nt, 0)
+    rec = ac.propagate(t["vp"], t["d"], t["w"], t["src"], t["rec"][:0], 1, c.dt, c.h)
+    assert tuple(rec.shape) == (c.nt, 0)
+    # duplicate receivers and a receiver inside the damped strip
+    c2 = acoustic_case(nz_i=30, nx_i=40, nshots=2, nt=120)
+    c2.rec = [np.array([[50, 60], [50, 60], [10, 30], [60, 70]])] * 2
+    t2 = to_dev(c2)
+    obs = torch.zeros((c2.nt, 8), device="cuda")
+    J, g, rec = _fused_grad(c2, t2, obs)
+    ro = oa.forward(c2.vp, c2.d, c2.dt, c2.h, c2.w, c2.src, c2.rec)
+    np.testing.assert_allclose(rec.detach().cpu().numpy(),
+                               np.concatenate([x[..., 0] for x in ro], 1), rtol=1e-4,
+                               atol=1e-6)
+    _, go, _ = oa.gradient(c2.vp, c2.d, c2.dt, c2.h, c2.w, c2.src, c2.rec,
+                           [np.zeros((c2.nt, 4, 1))] * 2)
+    I = (slice(50, -50), slice(50, -50))
+    assert rel_l2(g[I], go[I]) < TOL
+
+
+def test_invalid_inputs_raise():
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case(nz_i=12, nx_i=20, nshots=1, nt=3)
+    t = to_dev(c)
+    with pytest.raises(RuntimeError):
+        ac.propagate(t["vp"].cpu(), t["d"], t["w"], t["src"], t["rec"], 1, c.dt, c.h)
+    with pytest.raises(RuntimeError):
+        ac._time_step(t["vp"], torch.zeros((1, 5, 5), device="cuda"),
+                      torch.zeros((1, 5, 5), device="cuda"), c.dt, c.h, t["d"])
