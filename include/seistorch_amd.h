@@ -43,6 +43,11 @@ int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
  * interior.  Replaces the ring shape of seistorch/equations2d/utils.py:50-69. */
 int64_t seis_ring_len(int64_t nz, int64_t nx, int32_t npml);
 
+/* Number of field-pair snapshots the forward stores: floor((nt-2)/snap_every)
+ * (0 if snap_every <= 0).  No reference counterpart: the reference keeps the
+ * ring only (seistorch/checkpoint_new.py:137-139). */
+int64_t seis_snapshot_count(int64_t nt, int32_t snap_every);
+
 /* ---------------- acoustic, constant density, PML ------------------------ */
 
 /* One time step y = A*(2/dt^2*h1 - B*h2 + vp^2*L h1), A = 1/(dt^-2 + d/dt),
@@ -68,24 +73,31 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
  * shot after step n, receivers rec_flat[r] (flat indices into (B, nz, nx))
  * recorded into rec_out[n*R + r].  If ring != NULL the ring of u^{n+1} is
  * stored in slot n of ring ((nt-1) x B x seis_ring_len) for n < nt-1 (the
- * boundary saving of seistorch/checkpoint_new.py:114-143).  On return u_a
- * holds u^{nt} and u_b holds u^{nt-1}.  shots_per_group shots share one
- * thread (model reads amortised over them). */
+ * boundary saving of seistorch/checkpoint_new.py:114-143).  If snaps != NULL,
+ * the pair (u^m, u^{m+1}) is also stored for every m = j*snap_every,
+ * 1 <= j <= seis_snapshot_count(nt, snap_every), in slot j-1 of snaps
+ * (count x 2 x B x nz x nx): the adjoint restarts its reverse-time
+ * reconstruction from them, which bounds FP32 drift.  On return u_a holds
+ * u^{nt} and u_b holds u^{nt-1}.  shots_per_group shots share one thread
+ * (model reads amortised over them). */
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
-                          float* rec_out, float* ring, float* u_a, float* u_b, int64_t B,
-                          int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                          int32_t shots_per_group, void* stream);
+                          float* rec_out, float* ring, float* snaps, int32_t snap_every,
+                          float* u_a, float* u_b, int64_t B, int64_t nz, int64_t nx, float dt,
+                          float h, int32_t npml, int32_t shots_per_group, void* stream);
 
 /* Adjoint of seis_acoustic_forward (replaces the boundary-saving backward of
  * seistorch/checkpoint_new.py:146-217 and acoustic.py:88-137, with the exact
- * time pairing): rec_grad[n*R + r] = dJ/d rec_out[n*R + r]; u_a/u_b hold the
- * final forward state (consumed); mu_a, mu_b are (B, nz, nx) work buffers;
+ * time pairing): rec_grad[n*R + r] = dJ/d rec_out[n*R + r]; ring and snaps
+ * (may be NULL) as written by seis_acoustic_forward with the same
+ * snap_every; u_a/u_b hold the final forward state (consumed); mu_a, mu_b are
+ * (B, nz, nx) work buffers;
  * grad_part is ceil(B/shots_per_group) x (nz, nx) work; grad (nz, nx) gets
  * dJ/dvp on the interior [npml+1, n-npml-1) and zero elsewhere. */
 int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                            const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
-                           const float* rec_grad, const float* ring, float* u_a, float* u_b,
+                           const float* rec_grad, const float* ring, const float* snaps,
+                           int32_t snap_every, float* u_a, float* u_b,
                            float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
                            int64_t nz, int64_t nx, float dt, float h, int32_t npml,
                            int32_t shots_per_group, void* stream);

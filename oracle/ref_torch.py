@@ -84,12 +84,20 @@ def l2(syn, obs):
     return loss
 
 
-def gradient_ad(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32):
-    """Autograd through the whole loop: returns (J, dJ/dc, records)."""
+def gradient_ad(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32, rec_cotangent=None):
+    """Autograd through the whole loop: returns (J, dJ/dc, records).
+
+    With ``rec_cotangent`` (nt, sum nrec) the records are backpropagated with
+    that cotangent instead of the L2 residual (J is then returned as 0).
+    """
     cp = torch.as_tensor(c).to(dtype).clone().requires_grad_(True)
     syn = forward(c, d, dt, h, wavelet, src, rec, dtype=dtype, c_param=cp)
-    J = l2(syn, obs)
-    J.backward()
+    if rec_cotangent is None:
+        J = l2(syn, obs)
+        J.backward()
+    else:
+        J = torch.zeros(())
+        torch.cat(syn, 1)[..., 0].backward(torch.as_tensor(rec_cotangent).to(dtype))
     return J.detach(), cp.grad.detach(), [s.detach() for s in syn]
 
 

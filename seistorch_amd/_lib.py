@@ -26,14 +26,15 @@ SIGNATURES = {
     "seis_profile_collect": (ctypes.c_int, [_i32, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(_i64)]),
     "seis_ring_len": (_i64, [_i64, _i64, _i32]),
+    "seis_snapshot_count": (_i64, [_i64, _i32]),
     "seis_acoustic_step": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _p]),
     "seis_acoustic_step_vjp": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64,
                                               _f, _f, _p]),
-    "seis_acoustic_forward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _p,
-                                             _i64, _i64, _i64, _f, _f, _i32, _i32, _p]),
-    "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _p,
-                                              _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32,
-                                              _i32, _p]),
+    "seis_acoustic_forward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
+                                             _p, _p, _i64, _i64, _i64, _f, _f, _i32, _i32, _p]),
+    "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
+                                              _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
+                                              _i32, _i32, _p]),
 }
 
 ABI_VERSION = 1

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
 
 // One reverse iteration k (see file header): mu^k over the whole grid into
 // m2 (was mu^{k+2}); on the interior the gradient term 2c*L(u^{k-1})*mu^k and
-// the rebuilt u^{k-2} = 2u^{k-1} - u^k + w[k-1]*src + c^2 dt^2 L u^{k-1} into
+// the rebuilt u^{k-2} = (2/dt^2 u^{k-1} + c^2 L u^{k-1} - (u^k - w[k-1]*src)/A)/B into
 // uc (was u^k).  recon: 0 none, 1 rebuild, 2 write zeros (u^0).
 __global__ __launch_bounds__(256) void k_bwd(
     const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ um,
@@ -230,9 +230,13 @@ __global__ __launch_bounds__(256) void k_bwd(
       const float Lu = lap5(um + off, z, x, i, g, k.ih2);
       gacc += Lu * mu;
       if (recon == 1) {
-        float v = 2.0f * um[off + i] - uc[off + i] + c2 * k.dt2 * Lu;
-        if (z == src[2 * b] && x == src[2 * b + 1]) v += *wn;
-        uc[off + i] = v;
+        // Invert the forward's own FP32 update u^k = A*(2/dt^2 u^{k-1} - B u^{k-2}
+        // + c^2 L u^{k-1}) with true divisions by its A and B: the shortcut
+        // 2u^{k-1} - u^k + c^2 dt^2 L assumes A*B == 1, which FP32 misses by
+        // ~1 ulp and which then drifts coherently over the reversed steps.
+        float un = uc[off + i];
+        if (z == src[2 * b] && x == src[2 * b + 1]) un -= *wn;
+        uc[off + i] = ((k.two_idt2 * um[off + i] + c2 * Lu) - un / A) / Bc;
       } else if (recon == 2) {
         uc[off + i] = 0.0f;
       }
@@ -278,6 +282,17 @@ __global__ __launch_bounds__(256) void k_fill(float* __restrict__ p, int64_t n, 
   if (t < n) p[t] = v;
 }
 
+// Two equal-size buffers copied in one launch (snapshot store / reload).
+__global__ __launch_bounds__(256) void k_copy2(const float* __restrict__ a, float* __restrict__ ao,
+                                               const float* __restrict__ b, float* __restrict__ bo,
+                                               int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    ao[t] = a[t];
+    bo[t] = b[t];
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sum_groups(const float* __restrict__ gpart, int G,
                                                     float* __restrict__ out, int64_t plane) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -285,6 +300,12 @@ __global__ __launch_bounds__(256) void k_sum_groups(const float* __restrict__ gp
   float s = 0.0f;
   for (int q = 0; q < G; ++q) s += gpart[(int64_t)q * plane + t];
   out[t] = s;
+}
+
+inline int copy2(const float* a, float* ao, const float* b, float* bo, int64_t n, hipStream_t s) {
+  k_copy2<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, ao, b, bo, n);
+  SEIS_CHECK_LAUNCH();
+  return 0;
 }
 
 inline dim3 tile_grid(const Grid& g, int zdim) {
@@ -316,6 +337,11 @@ int64_t seis_ring_len(int64_t nz, int64_t nx, int32_t npml) {
   const int64_t nzi = nz - 2 * (int64_t)npml - 2;
   if (npml < 0 || nxi < 1 || nzi < 1) return -1;
   return 2 * nxi + 2 * nzi;
+}
+
+int64_t seis_snapshot_count(int64_t nt, int32_t snap_every) {
+  if (snap_every <= 0 || nt < 2) return 0;
+  return (nt - 2) / snap_every;
 }
 
 int seis_acoustic_step(const float* vp, const float* d, const float* h1, const float* h2,
@@ -352,9 +378,9 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
 
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
-                          float* rec_out, float* ring, float* u_a, float* u_b, int64_t B,
-                          int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                          int32_t shots_per_group, void* stream) {
+                          float* rec_out, float* ring, float* snaps, int32_t snap_every,
+                          float* u_a, float* u_b, int64_t B, int64_t nz, int64_t nx, float dt,
+                          float h, int32_t npml, int32_t shots_per_group, void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
   SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b, "null pointer argument");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
@@ -363,6 +389,7 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml) : 0;
   SEIS_CHECK_ARG(!ring || ring_len > 0, "grid %lld x %lld has no interior inside npml=%d",
                  (long long)nz, (long long)nx, npml);
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
   const Grid g{(int)nz, (int)nx, npml, nz * nx};
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
@@ -388,6 +415,11 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
           prev, rec_flat, R, R ? rec_out + n * R : nullptr, slot, ring_len, (int)B, g);
       SEIS_CHECK_LAUNCH();
     }
+    // snapshot at time m = n: the pair (u^m, u^{m+1}), slot m/snap_every - 1
+    if (snaps && n >= snap_every && n % snap_every == 0 && n <= nt - 2) {
+      float* slot_p = snaps + (n / snap_every - 1) * 2 * B * g.plane;
+      if (int e = copy2(cur, slot_p, prev, slot_p + B * g.plane, B * g.plane, s)) return e;
+    }
     float* t = cur; cur = prev; prev = t;
   }
   return 0;
@@ -395,7 +427,8 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
 
 int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                            const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
-                           const float* rec_grad, const float* ring, float* u_a, float* u_b,
+                           const float* rec_grad, const float* ring, const float* snaps,
+                           int32_t snap_every, float* u_a, float* u_b,
                            float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
                            int64_t nz, int64_t nx, float dt, float h, int32_t npml,
                            int32_t shots_per_group, void* stream) {
@@ -409,6 +442,7 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   SEIS_CHECK_ARG(ring_len > 0, "grid %lld x %lld has no interior inside npml=%d", (long long)nz,
                  (long long)nx, npml);
   SEIS_CHECK_ARG(ring || nt <= 2, "ring storage is required for nt > 2");
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
   const Grid g{(int)nz, (int)nx, npml, nz * nx};
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
@@ -435,6 +469,12 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
           vp, d, X, Y, M2, grad_part, rec_flat, R ? rec_grad + (kk - 1) * R : nullptr, R, slot,
           ring_len, recon, (int)B, S, g, k);
       SEIS_CHECK_LAUNCH();
+    }
+    // reload the stored pair (u^m, u^{m+1}) at snapshot times m = kk-2
+    const int64_t m = kk - 2;
+    if (snaps && m >= snap_every && m % snap_every == 0 && m <= nt - 2) {
+      const float* slot_p = snaps + (m / snap_every - 1) * 2 * B * g.plane;
+      if (int e = copy2(slot_p, Y, slot_p + B * g.plane, X, B * g.plane, s)) return e;
     }
     float* t = X; X = Y; Y = t;
     t = M1; M1 = M2; M2 = t;

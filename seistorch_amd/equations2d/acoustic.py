@@ -103,6 +103,24 @@ def ring_len(nz, nx, npml=NPML):
     return int(_lib.load().seis_ring_len(nz, nx, npml))
 
 
+SNAP_MIN_EVERY = 50        # densest snapshot spacing chosen automatically
+SNAP_HBM_FRACTION = 0.25   # share of device memory the snapshots may take
+
+
+def default_snapshot_every(nt, B, nz, nx, device):
+    """Snapshot spacing: as dense as SNAP_MIN_EVERY allows within the budget.
+
+    The pair (u^m, u^{m+1}) every K steps bounds the FP32 drift of the
+    reverse-time reconstruction to K steps; 0 means pure boundary saving.
+    """
+    pair = 2 * B * nz * nx * 4
+    total = torch.cuda.get_device_properties(device).total_memory
+    max_snaps = int(SNAP_HBM_FRACTION * total // pair)
+    if max_snaps < 1 or nt <= SNAP_MIN_EVERY + 1:
+        return 0
+    return max(SNAP_MIN_EVERY, -(-(nt - 2) // max_snaps))
+
+
 def default_shots_per_group(B):
     """Shots advanced by one thread: amortises model and gradient traffic."""
     for s in (4, 2):
@@ -141,11 +159,14 @@ class AcousticLoop(torch.autograd.Function):
     """Whole forward loop -> records (nt, R); backward = hand-written adjoint.
 
     Inputs: vp, d (nz, nx); wavelet (nt,); src_zx (B, 2) int32 padded (z, x);
-    rec_flat (R,) int64 flat indices into (B, nz, nx); B; dt; h.
+    rec_flat (R,) int64 flat indices into (B, nz, nx); B; dt; h;
+    shots_per_group (None: automatic); snapshot_every (None: automatic, 0: ring
+    only as in the reference).
     """
 
     @staticmethod
-    def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group):
+    def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group,
+                snapshot_every):
         _lib.require_device(vp, d, wavelet, src_zx, rec_flat)
         nz, nx = vp.shape
         nt = wavelet.shape[0]
@@ -162,22 +183,31 @@ class AcousticLoop(torch.autograd.Function):
             if L <= 0:
                 raise RuntimeError(f"grid {nz}x{nx} has no interior inside the 50-cell boundary")
             ring = torch.empty((nt - 1, B, L), device=vp.device, dtype=torch.float32)
+        K = 0
+        snaps = None
+        if need_grad:
+            K = default_snapshot_every(nt, B, nz, nx, vp.device) if snapshot_every is None \
+                else int(snapshot_every)
+            n_snap = int(_lib.load().seis_snapshot_count(nt, K))
+            if n_snap > 0:
+                snaps = torch.empty((n_snap, 2, B, nz, nx), device=vp.device, dtype=torch.float32)
         _lib.call("seis_acoustic_forward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
-                  _lib.ptr(u_a), _lib.ptr(u_b), B, nz, nx, dt, h, NPML, S, _lib.stream_of(vp))
+                  _lib.ptr(snaps), K, _lib.ptr(u_a), _lib.ptr(u_b), B, nz, nx, dt, h, NPML, S,
+                  _lib.stream_of(vp))
         if need_grad:
             ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
-            ctx.state = (ring, u_a, u_b)
-            ctx.meta = (B, dt, h, S)
+            ctx.state = (ring, snaps, u_a, u_b)
+            ctx.meta = (B, dt, h, S, K)
         return rec
 
     @staticmethod
     @torch.autograd.function.once_differentiable
     def backward(ctx, grad_rec):
         vp, d, wavelet, src_zx, rec_flat = ctx.saved_tensors
-        ring, u_a, u_b = ctx.state
+        ring, snaps, u_a, u_b = ctx.state
         ctx.state = None  # the backward consumes the final forward state
-        B, dt, h, S = ctx.meta
+        B, dt, h, S, K = ctx.meta
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -189,12 +219,15 @@ class AcousticLoop(torch.autograd.Function):
         grad = torch.empty_like(vp)
         _lib.call("seis_acoustic_backward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(grad_rec), _lib.ptr(ring),
-                  _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(mu_a), _lib.ptr(mu_b), _lib.ptr(gpart),
+                  _lib.ptr(snaps), K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b),
+                  _lib.ptr(mu_a), _lib.ptr(mu_b), _lib.ptr(gpart),
                   _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, _lib.stream_of(vp))
-        return grad, None, None, None, None, None, None, None, None
+        return grad, None, None, None, None, None, None, None, None, None
 
 
-def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None):
+def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
+              snapshot_every=None):
     """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint."""
     return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
-                              rec_flat.contiguous(), int(B), _f(dt), _f(h), shots_per_group)
+                              rec_flat.contiguous(), int(B), _f(dt), _f(h), shots_per_group,
+                              snapshot_every)
