@@ -33,15 +33,20 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=60,
                    help="time steps of the bounded CPU sample (1 shot, full grid)")
+    p.add_argument("--nt", type=int, default=None,
+                   help="shorter time axis for profiler runs (the metric is quoted at full nt)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="also run a plain 1 GiB device copy (HBM-counter calibration)")
     return p.parse_args()
 
 
-def build_problem(dev, seed=20230503):
+def build_problem(dev, seed=20230503, nt_override=None):
     import torch
     from seistorch_amd import synthetic
     from seistorch_amd.pml import generate_pml_coefficients_2d
     from seistorch_amd.utils import pad_model, ricker_wave
     nz_i, nx_i, nshots, nt, dt, h, fm = 800, 2400, 64, 5000, 0.8e-3, 10.0, 8.0
+    nt = nt_override or nt
     true, init = synthetic.anomaly_model(nz_i, nx_i, seed=seed)
     vt = pad_model(true)
     vi = pad_model(init)
@@ -100,7 +105,7 @@ def main():
     import torch
     from seistorch_amd import _lib
     dev = torch.device("cuda:0")
-    P = build_problem(dev)
+    P = build_problem(dev, nt_override=args.nt)
     with torch.no_grad():
         from seistorch_amd.equations2d import acoustic as ac
         P["obs"] = ac.propagate(P["vp_true"], P["d"], P["w"], P["src"], P["rec"], P["B"],
@@ -157,6 +162,15 @@ def main():
         "kernels_ms_total": {k: round(v[0], 2) for k, v in kern.items()},
         "loss": float(loss),
     }
+    if args.copy_calibration:
+        n = 1 << 28  # 1 GiB of float32
+        a = torch.ones(n, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = 4 * n
+        del a, b
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, args.cpu_steps)
