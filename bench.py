@@ -113,6 +113,10 @@ def main():
     torch.cuda.synchronize()
     print(f"[bench] problem ready: {P['B']} shots, {P['nz']}x{P['nx']}, nt={P['nt']}",
           file=sys.stderr, flush=True)
+    from seistorch_amd import autotune
+    shape, tune = autotune.tune_acoustic(P["vp_init"], P["d"], P["w"], P["src"], P["rec"],
+                                         P["B"], P["dt"], P["h"])
+    print(f"[bench] autotuned launch shape {shape}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         gradient_step(P)
         torch.cuda.synchronize()
@@ -160,6 +164,10 @@ def main():
                      "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch,
                      "bytes_per_cell_update": BYTES_PER_CELL[dom]},
         "kernels_ms_total": {k: round(v[0], 2) for k, v in kern.items()},
+        "launch_shape": {"shots_per_group": shape[0], "rows_per_thread": shape[1],
+                         "tile_x": shape[2],
+                         "autotune_ms": {f"S{a}_rz{b}_tx{c}": round(v, 2)
+                                         for (a, b, c), v in tune.items()}},
         "loss": float(loss),
     }
     if args.copy_calibration:

@@ -164,3 +164,83 @@ def gradient_bs(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32, npml=4
     mask = torch.zeros_like(grad)
     mask[lo:hi_z, lo:hi_x] = 1.0
     return J.detach(), grad * mask, syn
+
+
+# ------------------------------------------------------------------ elastic
+
+def diff_using_roll(x, dim=-1, forward=True, padding_value=0):
+    """seistorch/equations2d/utils.py:3-48: forward -> x - roll(x, 1) with the
+    first index zeroed; backward -> roll(x, -1) - x with the last zeroed."""
+    if forward:
+        diff = x - torch.roll(x, shifts=1, dims=dim)
+        if dim == 1:
+            diff[:, 0] = padding_value
+        elif dim == 2:
+            diff[..., 0] = padding_value
+    else:
+        diff = torch.roll(x, shifts=-1, dims=dim) - x
+        if dim == 1:
+            diff[:, -1] = padding_value
+        elif dim == 2:
+            diff[..., -1] = padding_value
+    return diff
+
+
+def elastic_step(vp, vs, rho, vx, vz, txx, tzz, txz, dt, h, d):
+    """seistorch/equations2d/elastic.py:7-37, same expressions and order."""
+    lame_lambda = rho * (vp.pow(2) - 2 * vs.pow(2))
+    lame_mu = rho * (vs.pow(2))
+    c = 0.5 * dt * d
+    vx_x = diff_using_roll(vx, 2)
+    vz_z = diff_using_roll(vz, 1, False)
+    vx_z = diff_using_roll(vx, 1)
+    vz_x = diff_using_roll(vz, 2, False)
+    y_txx = (1 + c) ** -1 * (dt * h.pow(-1) * ((lame_lambda + 2 * lame_mu) * vx_x + lame_lambda * vz_z) + (1 - c) * txx)
+    y_tzz = (1 + c) ** -1 * (dt * h.pow(-1) * ((lame_lambda + 2 * lame_mu) * vz_z + lame_lambda * vx_x) + (1 - c) * tzz)
+    y_txz = (1 + c) ** -1 * (dt * lame_mu * h.pow(-1) * (vz_x + vx_z) + (1 - c) * txz)
+    txx_x = diff_using_roll(y_txx, 2, False)
+    txz_z = diff_using_roll(y_txz, 1, False)
+    tzz_z = diff_using_roll(y_tzz, 1)
+    txz_x = diff_using_roll(y_txz, 2)
+    y_vx = (1 + c) ** -1 * (dt * rho.pow(-1) * h.pow(-1) * (txx_x + txz_z) + (1 - c) * vx)
+    y_vz = (1 + c) ** -1 * (dt * rho.pow(-1) * h.pow(-1) * (txz_x + tzz_z) + (1 - c) * vz)
+    return y_vx, y_vz, y_txx, y_tzz, y_txz
+
+
+ELASTIC_FIELDS = ("vx", "vz", "txx", "tzz", "txz")
+
+
+def elastic_forward(vp, vs, rho, d, dt, h, wavelet, src, rec, src_type=("txx", "tzz"),
+                    rec_type=("vx", "vz"), dtype=torch.float32, params=None):
+    """Records as a list of (nt, nrec_b, nch) tensors (rnn.py:176-212, offset 0)."""
+    vp0, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(vp, d, dt, h, wavelet, src, rec,
+                                                                 dtype)
+    if params is None:
+        params = (vp0, torch.as_tensor(vs).to(dtype), torch.as_tensor(rho).to(dtype))
+    f = {n: torch.zeros_like(smask) for n in ELASTIC_FIELDS}
+    recs = {k: [] for k in rec_type}
+    for n in range(w.shape[0]):
+        new = elastic_step(*params, *(f[x] for x in ELASTIC_FIELDS), dt_t, h_t, d)
+        f = dict(zip(ELASTIC_FIELDS, new))
+        for st in src_type:
+            y = f[st]
+            y += smask * w[n]  # first-order equations add in place (source.py:49-52)
+        for k in rec_type:
+            recs[k].append(f[k][bidx, rz, rx])
+    stacked = torch.stack([torch.stack(recs[k], 0) for k in rec_type], 2)
+    return list(torch.split(stacked, counts, dim=1))
+
+
+def elastic_gradient_ad(vp, vs, rho, d, dt, h, wavelet, src, rec, obs, src_type=("txx", "tzz"),
+                        rec_type=("vx", "vz"), dtype=torch.float32, rec_cotangent=None):
+    """Autograd through the whole loop: (J, (g_vp, g_vs, g_rho), records)."""
+    ps = [torch.as_tensor(x).to(dtype).clone().requires_grad_(True) for x in (vp, vs, rho)]
+    syn = elastic_forward(vp, vs, rho, d, dt, h, wavelet, src, rec, src_type, rec_type, dtype,
+                          params=ps)
+    if rec_cotangent is None:
+        J = l2(syn, obs)
+        J.backward()
+    else:
+        J = torch.zeros(())
+        torch.cat(syn, 1).backward(torch.as_tensor(rec_cotangent).to(dtype))
+    return J.detach(), tuple(p.grad.detach() for p in ps), [s.detach() for s in syn]

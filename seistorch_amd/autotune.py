@@ -1,0 +1,74 @@
+"""Launch-shape autotuner for the fused time loops.
+
+For a given equation and problem shape (shots, nz, nx) it times the fused
+forward + adjoint over a short time axis for every candidate
+(shots_per_group, rows_per_thread, tile_x) and keeps the fastest.  Results live in a
+process-wide table (and optionally a JSON file), keyed by shape, and
+``propagate`` picks them up when the caller does not fix the launch shape.
+"""
+import itertools
+import json
+import os
+
+import torch
+
+_TABLE = {}
+CANDIDATES_S = (2, 4, 8)
+CANDIDATES_RZ = (1, 2, 4)
+CANDIDATES_TX = (64, 128, 256)
+
+
+def _key(eq, B, nz, nx):
+    return f"{eq}:{B}:{nz}:{nx}"
+
+
+def lookup(eq, B, nz, nx):
+    return _TABLE.get(_key(eq, B, nz, nx))
+
+
+def record(eq, B, nz, nx, shape):
+    _TABLE[_key(eq, B, nz, nx)] = tuple(int(v) for v in shape)
+
+
+def save(path):
+    with open(path, "w") as f:
+        json.dump({k: list(v) for k, v in _TABLE.items()}, f, indent=1)
+
+
+def load(path):
+    if os.path.exists(path):
+        with open(path) as f:
+            for k, v in json.load(f).items():
+                _TABLE[k] = tuple(v)
+
+
+def tune_acoustic(vp, d, wavelet, src_zx, rec_flat, B, dt, h, nt_probe=24, repeats=2,
+                  candidates_s=CANDIDATES_S, candidates_rz=CANDIDATES_RZ,
+                  candidates_tx=CANDIDATES_TX, verbose=False):
+    """Time forward+adjoint for each launch shape on the real grid; keep the best."""
+    from .equations2d import acoustic as ac
+    nz, nx = vp.shape
+    w = wavelet[:nt_probe].contiguous()
+    results = {}
+    for S, rz, tx in itertools.product(candidates_s, candidates_rz, candidates_tx):
+        if S > B:
+            continue
+        best = float("inf")
+        for _ in range(repeats):
+            v = vp.detach().clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            rec = ac.propagate(v, d, w, src_zx, rec_flat, B, dt, h, shots_per_group=S,
+                               snapshot_every=0, rows_per_thread=rz, tile_x=tx)
+            rec.backward(torch.ones_like(rec))
+            t1.record()
+            torch.cuda.synchronize()
+            best = min(best, t0.elapsed_time(t1))
+        results[(S, rz, tx)] = best
+        if verbose:
+            print(f"[autotune] acoustic S={S} rz={rz} tx={tx}: {best:.2f} ms", flush=True)
+    shape = min(results, key=results.get)
+    record("acoustic", B, nz, nx, shape)
+    return shape, results

@@ -20,6 +20,11 @@ COMMON_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     # no packed-FP32 instructions are generated (v_pk_{add,mul,fma}_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no implicit multiply-add contraction: every kernel rounds exactly as its
+    # source reads (explicit fmaf where a fused multiply-add is wanted), so the
+    # per-step and fused kernels agree bitwise and follow the reference's
+    # op-by-op FP32 elementwise rounding
+    "-ffp-contract=off",
     "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
 ]
 

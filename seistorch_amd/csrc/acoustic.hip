@@ -32,6 +32,26 @@ struct Grid {
   int64_t plane;  // nz*nx
 };
 
+// 5-point Laplacian from its taps: the h^-2-weighted taps accumulated in
+// row-major kernel order (z-1, x-1, centre, x+1, z+1) with fused multiply-adds,
+// the way a direct 3x3 convolution with weights h^-2*[[0,1,0],[1,-4,1],[0,1,0]]
+// accumulates (acoustic.py:60-71).  Every kernel uses this one helper.
+__device__ __forceinline__ float lap_taps(float zm, float xm, float c, float xp, float zp,
+                                          float ih2) {
+  float acc = ih2 * zm;
+  acc = __fmaf_rn(ih2, xm, acc);
+  acc = __fmaf_rn(-4.0f * ih2, c, acc);
+  acc = __fmaf_rn(ih2, xp, acc);
+  return __fmaf_rn(ih2, zp, acc);
+}
+
+// The damped update of acoustic.py:81-84 rounded op by op in its order:
+// A*((2/dt^2*u - B*u_prev) + c^2*L).
+__device__ __forceinline__ float acoustic_update(float A, float Bc, float two_idt2, float u,
+                                                 float up, float c2, float L) {
+  return A * ((two_idt2 * u - Bc * up) + c2 * L);
+}
+
 __device__ __forceinline__ float lap5(const float* __restrict__ f, int z, int x, int i,
                                       const Grid g, float ih2) {
   const float c = f[i];
@@ -39,7 +59,7 @@ __device__ __forceinline__ float lap5(const float* __restrict__ f, int z, int x,
   const float zp = z < g.nz - 1 ? f[i + g.nx] : 0.0f;
   const float xm = x > 0 ? f[i - 1] : 0.0f;
   const float xp = x < g.nx - 1 ? f[i + 1] : 0.0f;
-  return ((zm + zp) + (xm + xp) - 4.0f * c) * ih2;
+  return lap_taps(zm, xm, c, xp, zp, ih2);
 }
 
 __device__ __forceinline__ bool interior(int z, int x, const Grid g) {
@@ -82,7 +102,7 @@ __global__ __launch_bounds__(256) void k_step(const float* __restrict__ vp,
   const float A = 1.0f / (k.idt2 + dd * k.idt);
   const float Bc = k.idt2 - dd * k.idt;
   const float L = lap5(h1 + off, z, x, i, g, k.ih2);
-  y[off + i] = A * (k.two_idt2 * h1[off + i] - Bc * h2[off + i] + c * c * L);
+  y[off + i] = acoustic_update(A, Bc, k.two_idt2, h1[off + i], h2[off + i], c * c, L);
 }
 
 // d/d(h1), d/d(h2) of one step: gh1 = 2/dt^2*A*gy + L(c^2*A*gy) + pass,
@@ -110,7 +130,7 @@ __global__ __launch_bounds__(256) void k_step_vjp_fields(
   const float qzp = z < g.nz - 1 ? q(i + g.nx) : 0.0f;
   const float qxm = x > 0 ? q(i - 1) : 0.0f;
   const float qxp = x < g.nx - 1 ? q(i + 1) : 0.0f;
-  const float Lq = ((qzm + qzp) + (qxm + qxp) - 4.0f * qc) * k.ih2;
+  const float Lq = lap_taps(qzm, qxm, qc, qxp, qzp, k.ih2);
   const float ag = A * gb[i];
   float v = k.two_idt2 * ag + Lq;
   if (pass) v += pass[off + i];
@@ -138,33 +158,101 @@ __global__ __launch_bounds__(256) void k_step_vjp_model(
 
 // ------------------------------------------------------------ fused loop
 
+// Tiles of the fused loops: a 256-thread block is TX lanes along x by
+// 256/TX thread-rows; each thread owns RZ consecutive rows of one column for
+// S shots, so a tile is TX x (256/TX)*RZ cells.  The RZ+2 rows of the
+// stencilled field's column are loaded once into registers (fully unrolled,
+// so all loads of a thread are independent); the x-neighbours are L1 hits of
+// the rows the neighbouring lanes load, except at the tile's two x-edges,
+// where they fall in the neighbouring tiles' cache lines.
+//
+// XCD-aware tile order: workgroups are handed to the 8 XCDs round-robin, so
+// workgroup w runs on XCD w % 8.  Tile t = (w % 8) * per + w / 8 gives each
+// XCD a contiguous range of tiles in time order.  Tiles are numbered
+// x-fastest, so the x- and z-neighbours of a tile (which own its halo cache
+// lines) are dispatched by the same XCD within one tile row and find those
+// lines in its L2.
+constexpr int kBlock = 256;
+constexpr int kXcds = 8;
+
+struct TileMap {
+  int ntx, ntz, groups, per;  // tiles in x, tiles in z, shot groups, tiles per XCD
+};
+
+inline TileMap make_tiles(const Grid& g, int tx_w, int rz, int groups) {
+  TileMap m;
+  const int ty = kBlock / tx_w;
+  m.ntx = (g.nx + tx_w - 1) / tx_w;
+  m.ntz = (g.nz + ty * rz - 1) / (ty * rz);
+  m.groups = groups;
+  const int64_t n = (int64_t)m.ntx * m.ntz * groups;
+  m.per = (int)((n + kXcds - 1) / kXcds);
+  return m;
+}
+
+// -> false if this workgroup has no tile (padding of the XCD split)
+__device__ __forceinline__ bool tile_of(const TileMap m, int& tx, int& tzi, int& grp) {
+  const int w = blockIdx.x;
+  const int64_t t = (int64_t)(w % kXcds) * m.per + w / kXcds;
+  if (t >= (int64_t)m.ntx * m.ntz * m.groups) return false;
+  tx = (int)(t % m.ntx);
+  const int64_t r = t / m.ntx;
+  tzi = (int)(r % m.ntz);
+  grp = (int)(r / m.ntz);
+  return true;
+}
+
 // u^{n+1} = step(u^n, u^{n-1}) + w[n] at the source cell, written in place
 // over u^{n-1} (each cell of u^{n-1} is read only by the thread that
-// overwrites it).
-__global__ __launch_bounds__(256) void k_fwd(const float* __restrict__ vp,
-                                             const float* __restrict__ d,
-                                             const float* __restrict__ u, float* __restrict__ up,
-                                             const int32_t* __restrict__ src,
-                                             const float* __restrict__ wn, int B, int S, Grid g,
-                                             seis::TimeConsts k) {
-  const int x = blockIdx.x * kTX + threadIdx.x;
-  const int z = blockIdx.y * kTZ + threadIdx.y;
-  if (x >= g.nx || z >= g.nz) return;
-  const int i = z * g.nx + x;
-  const float c = vp[i];
-  const float dd = d[i];
-  const float A = 1.0f / (k.idt2 + dd * k.idt);
-  const float Bc = k.idt2 - dd * k.idt;
-  const float c2 = c * c;
-  const int b0 = blockIdx.z * S;
-  const int b1 = min(B, b0 + S);
+// overwrites it).  Same Laplacian and update helpers as the per-step kernel.
+template <int S, int RZ, int TX>
+__global__ __launch_bounds__(kBlock) void k_fwd(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ u,
+    float* __restrict__ up, const int32_t* __restrict__ src, const float* __restrict__ wn, int B,
+    Grid g, seis::TimeConsts k, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  constexpr int TY = kBlock / TX;
+  const int x = tx * TX + (int)(threadIdx.x % TX);
+  const int zb = (tzi * TY + (int)(threadIdx.x / TX)) * RZ;
+  if (x >= g.nx || zb >= g.nz) return;
+  const int b0 = grp * S;
+  const int nb = min(S, B - b0);
   const float w = *wn;
-  for (int b = b0; b < b1; ++b) {
-    const int64_t off = (int64_t)b * g.plane;
-    const float L = lap5(u + off, z, x, i, g, k.ih2);
-    float v = A * (k.two_idt2 * u[off + i] - Bc * up[off + i] + c2 * L);
-    if (z == src[2 * b] && x == src[2 * b + 1]) v += w;
-    up[off + i] = v;
+  float A[RZ], Bc[RZ], c2[RZ];
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int z = min(zb + r, g.nz - 1);
+    const float c = vp[z * g.nx + x];
+    const float dd = d[z * g.nx + x];
+    A[r] = 1.0f / (k.idt2 + dd * k.idt);
+    Bc[r] = k.idt2 - dd * k.idt;
+    c2[r] = c * c;
+  }
+  for (int s = 0; s < nb; ++s) {
+    const int b = b0 + s;
+    const float* f = u + (int64_t)b * g.plane;
+    float* o = up + (int64_t)b * g.plane;
+    const int sz = src[2 * b], sx = src[2 * b + 1];
+    float col[RZ + 2];
+#pragma unroll
+    for (int r = 0; r < RZ + 2; ++r) {
+      const int z = zb - 1 + r;
+      col[r] = (z >= 0 && z < g.nz) ? f[z * g.nx + x] : 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int z = zb + r;
+      if (z < g.nz) {
+        const int i = z * g.nx + x;
+        const float xm = x > 0 ? f[i - 1] : 0.0f;
+        const float xp = x < g.nx - 1 ? f[i + 1] : 0.0f;
+        const float L = lap_taps(col[r], xm, col[r + 1], xp, col[r + 2], k.ih2);
+        float v = acoustic_update(A[r], Bc[r], k.two_idt2, col[r + 1], o[i], c2[r], L);
+        if (z == sz && x == sx) v += w;
+        o[i] = v;
+      }
+    }
   }
 }
 
@@ -190,59 +278,100 @@ __global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
 // One reverse iteration k (see file header): mu^k over the whole grid into
 // m2 (was mu^{k+2}); on the interior the gradient term 2c*L(u^{k-1})*mu^k and
 // the rebuilt u^{k-2} = (2/dt^2 u^{k-1} + c^2 L u^{k-1} - (u^k - w[k-1]*src)/A)/B into
-// uc (was u^k).  recon: 0 none, 1 rebuild, 2 write zeros (u^0).
-__global__ __launch_bounds__(256) void k_bwd(
+// uc (was u^k).  recon: 0 none, 1 rebuild, 2 write zeros (u^0).  Tiles as
+// k_fwd; u^{k-1} and u^k are only touched on the interior and its ring rows.
+//
+// The reconstruction inverts the forward's own FP32 update u^k = A*(2/dt^2
+// u^{k-1} - B u^{k-2} + c^2 L u^{k-1}) with true divisions by its A and B: the
+// shortcut 2u^{k-1} - u^k + c^2 dt^2 L assumes A*B == 1, which FP32 misses by
+// ~1 ulp and which then drifts coherently over the reversed steps.
+template <int S, int RZ, int TX>
+__global__ __launch_bounds__(kBlock) void k_bwd(
     const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ um,
     float* __restrict__ uc, const float* __restrict__ m1, float* __restrict__ m2,
     float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
-    int recon, int B, int S, Grid g, seis::TimeConsts k) {
-  const int x = blockIdx.x * kTX + threadIdx.x;
-  const int z = blockIdx.y * kTZ + threadIdx.y;
-  if (x >= g.nx || z >= g.nz) return;
-  const int i = z * g.nx + x;
-  const float c = vp[i];
-  const float dd = d[i];
-  const float A = 1.0f / (k.idt2 + dd * k.idt);
-  const float Bc = k.idt2 - dd * k.idt;
-  const float c2 = c * c;
-  const float czm = z > 0 ? vp[i - g.nx] : 0.0f;
-  const float czp = z < g.nz - 1 ? vp[i + g.nx] : 0.0f;
-  const float cxm = x > 0 ? vp[i - 1] : 0.0f;
-  const float cxp = x < g.nx - 1 ? vp[i + 1] : 0.0f;
-  const float c2zm = czm * czm, c2zp = czp * czp, c2xm = cxm * cxm, c2xp = cxp * cxp;
-  const bool in = interior(z, x, g);
-  const int b0 = blockIdx.z * S;
-  const int b1 = min(B, b0 + S);
-  float gacc = 0.0f;
-  for (int b = b0; b < b1; ++b) {
+    int recon, int B, Grid g, seis::TimeConsts k, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  constexpr int TY = kBlock / TX;
+  const int x = tx * TX + (int)(threadIdx.x % TX);
+  const int zb = (tzi * TY + (int)(threadIdx.x / TX)) * RZ;
+  if (x >= g.nx || zb >= g.nz) return;
+  const int b0 = grp * S;
+  const int nb = min(S, B - b0);
+  const float w = *wn;
+  const bool xin = x > g.npml && x < g.nx - 1 - g.npml;
+  const int urow0 = g.npml, urow1 = g.nz - 1 - g.npml;  // rows of u^{k-1} that are read
+  // model column (rows zb-1 .. zb+RZ) and x-neighbours, shared by the shots
+  float c2col[RZ + 2], c2xm[RZ], c2xp[RZ], cc[RZ], A[RZ], Bc[RZ];
+#pragma unroll
+  for (int r = 0; r < RZ + 2; ++r) {
+    const int z = zb - 1 + r;
+    const float c = (z >= 0 && z < g.nz) ? vp[z * g.nx + x] : 0.0f;
+    c2col[r] = c * c;
+  }
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int z = min(zb + r, g.nz - 1);
+    const int i = z * g.nx + x;
+    const float cm = x > 0 ? vp[i - 1] : 0.0f;
+    const float cp = x < g.nx - 1 ? vp[i + 1] : 0.0f;
+    c2xm[r] = cm * cm;
+    c2xp[r] = cp * cp;
+    cc[r] = vp[i];
+    const float dd = d[i];
+    A[r] = 1.0f / (k.idt2 + dd * k.idt);
+    Bc[r] = k.idt2 - dd * k.idt;
+  }
+  float gsum[RZ];
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) gsum[r] = 0.0f;
+  for (int s = 0; s < nb; ++s) {
+    const int b = b0 + s;
     const int64_t off = (int64_t)b * g.plane;
-    const float* m = m1 + off;
-    const float mzm = z > 0 ? m[i - g.nx] : 0.0f;
-    const float mzp = z < g.nz - 1 ? m[i + g.nx] : 0.0f;
-    const float mxm = x > 0 ? m[i - 1] : 0.0f;
-    const float mxp = x < g.nx - 1 ? m[i + 1] : 0.0f;
-    const float mc = m[i];
-    const float Lm =
-        ((c2zm * mzm + c2zp * mzp) + (c2xm * mxm + c2xp * mxp) - 4.0f * (c2 * mc)) * k.ih2;
-    const float mu = A * (k.two_idt2 * mc - Bc * m2[off + i] + Lm);
-    m2[off + i] = mu;
-    if (in) {
-      const float Lu = lap5(um + off, z, x, i, g, k.ih2);
-      gacc += Lu * mu;
-      if (recon == 1) {
-        // Invert the forward's own FP32 update u^k = A*(2/dt^2 u^{k-1} - B u^{k-2}
-        // + c^2 L u^{k-1}) with true divisions by its A and B: the shortcut
-        // 2u^{k-1} - u^k + c^2 dt^2 L assumes A*B == 1, which FP32 misses by
-        // ~1 ulp and which then drifts coherently over the reversed steps.
-        float un = uc[off + i];
-        if (z == src[2 * b] && x == src[2 * b + 1]) un -= *wn;
-        uc[off + i] = ((k.two_idt2 * um[off + i] + c2 * Lu) - un / A) / Bc;
-      } else if (recon == 2) {
-        uc[off + i] = 0.0f;
+    const float* M = m1 + off;
+    float* M2 = m2 + off;
+    const float* U = um + off;
+    float* UC = uc + off;
+    const int sz = src[2 * b], sx = src[2 * b + 1];
+    float mcol[RZ + 2], ucol[RZ + 2];
+#pragma unroll
+    for (int r = 0; r < RZ + 2; ++r) {
+      const int z = zb - 1 + r;
+      mcol[r] = (z >= 0 && z < g.nz) ? M[z * g.nx + x] : 0.0f;
+      ucol[r] = (xin && z >= urow0 && z <= urow1) ? U[z * g.nx + x] : 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int z = zb + r;
+      if (z < g.nz) {
+        const int i = z * g.nx + x;
+        const float qxm = c2xm[r] * (x > 0 ? M[i - 1] : 0.0f);
+        const float qxp = c2xp[r] * (x < g.nx - 1 ? M[i + 1] : 0.0f);
+        const float Lm = lap_taps(c2col[r] * mcol[r], qxm, c2col[r + 1] * mcol[r + 1], qxp,
+                                  c2col[r + 2] * mcol[r + 2], k.ih2);
+        const float mu = acoustic_update(A[r], Bc[r], k.two_idt2, mcol[r + 1], M2[i], 1.0f, Lm);
+        M2[i] = mu;
+        if (xin && z > g.npml && z < g.nz - 1 - g.npml) {
+          const float Lu = lap_taps(ucol[r], U[i - 1], ucol[r + 1], U[i + 1], ucol[r + 2], k.ih2);
+          gsum[r] += Lu * mu;
+          if (recon == 1) {
+            float un = UC[i];
+            if (z == sz && x == sx) un -= w;
+            UC[i] = ((k.two_idt2 * ucol[r + 1] + c2col[r + 1] * Lu) - un / A[r]) / Bc[r];
+          } else if (recon == 2) {
+            UC[i] = 0.0f;
+          }
+        }
       }
     }
   }
-  if (in) gpart[(int64_t)blockIdx.z * g.plane + i] += 2.0f * c * gacc;
+  float* G = gpart + (int64_t)grp * g.plane;
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int z = zb + r;
+    if (xin && z > g.npml && z < g.nz - 1 - g.npml) G[z * g.nx + x] += 2.0f * cc[r] * gsum[r];
+  }
 }
 
 // After k_bwd: inject the adjoint source A*s^k at the receivers of mu^k (and
@@ -307,6 +436,46 @@ inline int copy2(const float* a, float* ao, const float* b, float* bo, int64_t n
   SEIS_CHECK_LAUNCH();
   return 0;
 }
+
+// Instantiated launch shapes: shots per thread S, rows per thread RZ, tile
+// width TX.
+#define SEIS_RZ_TX(K, S_)                                                              \
+  case S_ * 256 + 1 * 16 + 0: K<S_, 1, 64><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 2 * 16 + 0: K<S_, 2, 64><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 4 * 16 + 0: K<S_, 4, 64><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 1 * 16 + 1: K<S_, 1, 128><<<grid, kBlock, 0, s>>>(a..., tm); break; \
+  case S_ * 256 + 2 * 16 + 1: K<S_, 2, 128><<<grid, kBlock, 0, s>>>(a..., tm); break; \
+  case S_ * 256 + 4 * 16 + 1: K<S_, 4, 128><<<grid, kBlock, 0, s>>>(a..., tm); break; \
+  case S_ * 256 + 1 * 16 + 2: K<S_, 1, 256><<<grid, kBlock, 0, s>>>(a..., tm); break; \
+  case S_ * 256 + 2 * 16 + 2: K<S_, 2, 256><<<grid, kBlock, 0, s>>>(a..., tm); break; \
+  case S_ * 256 + 4 * 16 + 2: K<S_, 4, 256><<<grid, kBlock, 0, s>>>(a..., tm); break;
+
+inline int tx_code(int tile_x) { return tile_x == 64 ? 0 : tile_x == 128 ? 1 : 2; }
+
+inline bool valid_shape(int S, int RZ, int TX) {
+  return (S == 1 || S == 2 || S == 4 || S == 8) && (RZ == 1 || RZ == 2 || RZ == 4) &&
+         (TX == 64 || TX == 128 || TX == 256);
+}
+
+#define SEIS_DEFINE_LAUNCH(NAME, K)                                                   \
+  template <typename... Args>                                                        \
+  int NAME(int S, int RZ, int TX, const TileMap& tm, hipStream_t s, Args... a) {     \
+    const dim3 grid(tm.per * kXcds);                                                 \
+    switch (S * 256 + RZ * 16 + tx_code(TX)) {                                       \
+      SEIS_RZ_TX(K, 1)                                                               \
+      SEIS_RZ_TX(K, 2)                                                               \
+      SEIS_RZ_TX(K, 4)                                                               \
+      SEIS_RZ_TX(K, 8)                                                               \
+      default:                                                                       \
+        seis::set_error("unsupported launch shape (%d, %d, %d)", S, RZ, TX);         \
+        return 1;                                                                    \
+    }                                                                                \
+    SEIS_CHECK_LAUNCH();                                                             \
+    return 0;                                                                        \
+  }
+
+SEIS_DEFINE_LAUNCH(launch_fwd, k_fwd)
+SEIS_DEFINE_LAUNCH(launch_bwd, k_bwd)
 
 inline dim3 tile_grid(const Grid& g, int zdim) {
   return dim3((g.nx + kTX - 1) / kTX, (g.nz + kTZ - 1) / kTZ, zdim);
@@ -380,8 +549,13 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
                           float* u_a, float* u_b, int64_t B, int64_t nz, int64_t nx, float dt,
-                          float h, int32_t npml, int32_t shots_per_group, void* stream) {
+                          float h, int32_t npml, int32_t shots_per_group,
+                          int32_t rows_per_thread, int32_t tile_x, void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
+  SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, tile_x),
+                 "launch shape (shots_per_group %d, rows_per_thread %d, tile_x %d) not in "
+                 "{1,2,4,8} x {1,2,4} x {64,128,256}",
+                 shots_per_group, rows_per_thread, tile_x);
   SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b, "null pointer argument");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
@@ -397,6 +571,7 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   const int G = (int)((B + S - 1) / S);
   // After nt steps u^{nt} must sit in u_a: step n writes into the buffer that
   // holds u^{n-1}, so u^{n} lives in the step-0 "current" buffer iff n is even.
+  const TileMap tm = make_tiles(g, tile_x, rows_per_thread, G);
   float* cur = (nt % 2 == 0) ? u_a : u_b;
   float* prev = (nt % 2 == 0) ? u_b : u_a;
   if (int e = fill(u_a, B * g.plane, 0.0f, s)) return e;
@@ -404,9 +579,9 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   const int64_t aux = R + ((ring && nt > 1) ? B * ring_len : 0);
   for (int64_t n = 0; n < nt; ++n) {
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
-    k_fwd<<<tile_grid(g, G), dim3(kTX, kTZ), 0, s>>>(vp, d, cur, prev, src_zx, wavelet + n, (int)B, S,
-                                                    g, k);
-    SEIS_CHECK_LAUNCH();
+    if (int e = launch_fwd(S, rows_per_thread, tile_x, tm, s, vp, d, cur, prev, src_zx,
+                           wavelet + n, (int)B, g, k))
+      return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
@@ -431,8 +606,13 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            int32_t snap_every, float* u_a, float* u_b,
                            float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
                            int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                           int32_t shots_per_group, void* stream) {
+                           int32_t shots_per_group, int32_t rows_per_thread, int32_t tile_x,
+                           void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
+  SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, tile_x),
+                 "launch shape (shots_per_group %d, rows_per_thread %d, tile_x %d) not in "
+                 "{1,2,4,8} x {1,2,4} x {64,128,256}",
+                 shots_per_group, rows_per_thread, tile_x);
   SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b && mu_a && mu_b && grad_part && grad,
                  "null pointer argument");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
@@ -451,6 +631,7 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   if (int e = fill(mu_a, B * g.plane, 0.0f, s)) return e;
   if (int e = fill(mu_b, B * g.plane, 0.0f, s)) return e;
   if (int e = fill(grad_part, (int64_t)G * g.plane, 0.0f, s)) return e;
+  const TileMap tm = make_tiles(g, tile_x, rows_per_thread, G);
   float* X = u_b;   // u^{k-1}
   float* Y = u_a;   // u^k  -> u^{k-2}
   float* M1 = mu_a; // mu^{k+1}
@@ -458,9 +639,9 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   for (int64_t kk = nt; kk >= 1; --kk) {
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
-    k_bwd<<<tile_grid(g, G), dim3(kTX, kTZ), 0, s>>>(vp, d, X, Y, M1, M2, grad_part, src_zx,
-                                                    wavelet + (kk - 1), recon, (int)B, S, g, k);
-    SEIS_CHECK_LAUNCH();
+    if (int e = launch_bwd(S, rows_per_thread, tile_x, tm, s, vp, d, X, Y, M1, M2, grad_part, src_zx,
+                           wavelet + (kk - 1), recon, (int)B, g, k))
+      return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
     if (work > 0) {

@@ -121,12 +121,26 @@ def default_snapshot_every(nt, B, nz, nx, device):
     return max(SNAP_MIN_EVERY, -(-(nt - 2) // max_snaps))
 
 
+# launch shape when no tuned value is known: rows per thread, tile width
+ROWS_PER_THREAD = 2
+TILE_X = 256
+
+
 def default_shots_per_group(B):
     """Shots advanced by one thread: amortises model and gradient traffic."""
     for s in (4, 2):
-        if B % s == 0:
+        if B >= s:
             return s
     return 1
+
+
+def _launch_params(B, nz, nx, shape):
+    """(shots_per_group, rows_per_thread, tile_x): caller's, tuned, or default."""
+    from .. import autotune
+    tuned = autotune.lookup("acoustic", B, nz, nx)
+    default = tuned if tuned else (default_shots_per_group(B), ROWS_PER_THREAD, TILE_X)
+    shape = tuple(shape) if shape is not None else (None, None, None)
+    return tuple(int(v if v is not None else dv) for v, dv in zip(shape, default))
 
 
 def _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad):
@@ -159,19 +173,18 @@ class AcousticLoop(torch.autograd.Function):
     """Whole forward loop -> records (nt, R); backward = hand-written adjoint.
 
     Inputs: vp, d (nz, nx); wavelet (nt,); src_zx (B, 2) int32 padded (z, x);
-    rec_flat (R,) int64 flat indices into (B, nz, nx); B; dt; h;
-    shots_per_group (None: automatic); snapshot_every (None: automatic, 0: ring
-    only as in the reference).
+    rec_flat (R,) int64 flat indices into (B, nz, nx); B; dt; h; launch shape
+    (shots_per_group, rows_per_thread, tile_x), None entries automatic;
+    snapshot_every (None: automatic, 0: ring only as in the reference).
     """
 
     @staticmethod
-    def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group,
-                snapshot_every):
+    def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shape, snapshot_every):
         _lib.require_device(vp, d, wavelet, src_zx, rec_flat)
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
-        S = shots_per_group or default_shots_per_group(B)
+        S, rz, tx = _launch_params(B, nz, nx, shape)
         need_grad = ctx.needs_input_grad[0]
         _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad)
         u_a = torch.empty((B, nz, nx), device=vp.device, dtype=torch.float32)
@@ -194,11 +207,11 @@ class AcousticLoop(torch.autograd.Function):
         _lib.call("seis_acoustic_forward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K, _lib.ptr(u_a), _lib.ptr(u_b), B, nz, nx, dt, h, NPML, S,
-                  _lib.stream_of(vp))
+                  rz, tx, _lib.stream_of(vp))
         if need_grad:
             ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
             ctx.state = (ring, snaps, u_a, u_b)
-            ctx.meta = (B, dt, h, S, K)
+            ctx.meta = (B, dt, h, S, K, rz, tx)
         return rec
 
     @staticmethod
@@ -207,7 +220,7 @@ class AcousticLoop(torch.autograd.Function):
         vp, d, wavelet, src_zx, rec_flat = ctx.saved_tensors
         ring, snaps, u_a, u_b = ctx.state
         ctx.state = None  # the backward consumes the final forward state
-        B, dt, h, S, K = ctx.meta
+        B, dt, h, S, K, rz, tx = ctx.meta
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -221,13 +234,18 @@ class AcousticLoop(torch.autograd.Function):
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(grad_rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b),
                   _lib.ptr(mu_a), _lib.ptr(mu_b), _lib.ptr(gpart),
-                  _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, _lib.stream_of(vp))
+                  _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, rz, tx, _lib.stream_of(vp))
         return grad, None, None, None, None, None, None, None, None, None
 
 
 def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
-              snapshot_every=None):
-    """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint."""
+              snapshot_every=None, rows_per_thread=None, tile_x=None):
+    """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint.
+
+    The launch shape -- shots_per_group (1, 2, 4, 8), rows_per_thread
+    (1, 2, 4), tile_x (64, 128, 256) -- defaults to the autotuned values for
+    this shape (``seistorch_amd.autotune``), else to fixed defaults.
+    """
     return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
-                              rec_flat.contiguous(), int(B), _f(dt), _f(h), shots_per_group,
-                              snapshot_every)
+                              rec_flat.contiguous(), int(B), _f(dt), _f(h),
+                              (shots_per_group, rows_per_thread, tile_x), snapshot_every)

@@ -109,6 +109,29 @@ def test_fused_forward_records(S):
     assert e_ref < TOL and _oracle_ok(e_or, e_ref_or)
 
 
+def test_launch_shapes_agree():
+    """Records are bitwise identical for every (shots_per_group, rows_per_thread,
+    tile_x); the
+    gradient only changes by the shot summation order."""
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case(nshots=5, nt=150)
+    t = to_dev(c)
+    ref_rec, ref_g = None, None
+    for S, tz, tx in [(1, 1, 64), (2, 2, 128), (4, 4, 256), (8, 1, 256), (8, 4, 64),
+                      (2, 4, 128)]:
+        vp = t["vp"].clone().requires_grad_(True)
+        rec = ac.propagate(vp, t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h,
+                           shots_per_group=S, rows_per_thread=tz, tile_x=tx)
+        rec.backward(torch.ones_like(rec))
+        r = rec.detach().cpu().numpy()
+        g = vp.grad.cpu().numpy()
+        if ref_rec is None:
+            ref_rec, ref_g = r, g
+            continue
+        assert np.array_equal(r, ref_rec), (S, tz)
+        assert rel_l2(g, ref_g) < 1e-5, (S, tz, rel_l2(g, ref_g))
+
+
 def test_per_step_interface_matches_fused_loop():
     """The drop-in _time_step driven by a reference-style loop gives the fused records."""
     from seistorch_amd.equations2d import acoustic as ac
