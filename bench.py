@@ -165,8 +165,8 @@ def main():
                      "bytes_per_cell_update": BYTES_PER_CELL[dom]},
         "kernels_ms_total": {k: round(v[0], 2) for k, v in kern.items()},
         "launch_shape": {"shots_per_group": shape[0], "rows_per_thread": shape[1],
-                         "tile_x": shape[2],
-                         "autotune_ms": {f"S{a}_rz{b}_tx{c}": round(v, 2)
+                         "vec_x": shape[2],
+                         "autotune_ms": {f"S{a}_rz{b}_vx{c}": round(v, 2)
                                          for (a, b, c), v in tune.items()}},
         "loss": float(loss),
     }

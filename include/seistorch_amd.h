@@ -80,14 +80,14 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
  * reconstruction from them, which bounds FP32 drift.  On return u_a holds
  * u^{nt} and u_b holds u^{nt-1}.  Launch shape: shots_per_group (1, 2, 4
  * or 8) shots share one thread (model reads amortised over them), each
- * thread owns rows_per_thread (1, 2 or 4) consecutive rows, and a 256-thread
- * block is tile_x (64, 128 or 256) lanes wide. */
+ * thread owns rows_per_thread (1, 2 or 4) consecutive rows of vec_x (1, 2 or
+ * 4; must divide nx, pointers vec_x*4-byte aligned) consecutive cells. */
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
                           float* u_a, float* u_b, int64_t B, int64_t nz, int64_t nx, float dt,
                           float h, int32_t npml, int32_t shots_per_group,
-                          int32_t rows_per_thread, int32_t tile_x, void* stream);
+                          int32_t rows_per_thread, int32_t vec_x, void* stream);
 
 /* Adjoint of seis_acoustic_forward (replaces the boundary-saving backward of
  * seistorch/checkpoint_new.py:146-217 and acoustic.py:88-137, with the exact
@@ -103,7 +103,7 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            int32_t snap_every, float* u_a, float* u_b,
                            float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
                            int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                           int32_t shots_per_group, int32_t rows_per_thread, int32_t tile_x,
+                           int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                            void* stream);
 
 #ifdef __cplusplus

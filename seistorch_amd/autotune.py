@@ -2,7 +2,7 @@
 
 For a given equation and problem shape (shots, nz, nx) it times the fused
 forward + adjoint over a short time axis for every candidate
-(shots_per_group, rows_per_thread, tile_x) and keeps the fastest.  Results live in a
+(shots_per_group, rows_per_thread, vec_x) and keeps the fastest.  Results live in a
 process-wide table (and optionally a JSON file), keyed by shape, and
 ``propagate`` picks them up when the caller does not fix the launch shape.
 """
@@ -15,7 +15,7 @@ import torch
 _TABLE = {}
 CANDIDATES_S = (2, 4, 8)
 CANDIDATES_RZ = (1, 2, 4)
-CANDIDATES_TX = (64, 128, 256)
+CANDIDATES_VX = (1, 2, 4)
 
 
 def _key(eq, B, nz, nx):
@@ -44,14 +44,14 @@ def load(path):
 
 def tune_acoustic(vp, d, wavelet, src_zx, rec_flat, B, dt, h, nt_probe=24, repeats=2,
                   candidates_s=CANDIDATES_S, candidates_rz=CANDIDATES_RZ,
-                  candidates_tx=CANDIDATES_TX, verbose=False):
+                  candidates_vx=CANDIDATES_VX, verbose=False):
     """Time forward+adjoint for each launch shape on the real grid; keep the best."""
     from .equations2d import acoustic as ac
     nz, nx = vp.shape
     w = wavelet[:nt_probe].contiguous()
     results = {}
-    for S, rz, tx in itertools.product(candidates_s, candidates_rz, candidates_tx):
-        if S > B:
+    for S, rz, vx in itertools.product(candidates_s, candidates_rz, candidates_vx):
+        if S > B or nx % vx:
             continue
         best = float("inf")
         for _ in range(repeats):
@@ -61,14 +61,14 @@ def tune_acoustic(vp, d, wavelet, src_zx, rec_flat, B, dt, h, nt_probe=24, repea
             t1 = torch.cuda.Event(enable_timing=True)
             t0.record()
             rec = ac.propagate(v, d, w, src_zx, rec_flat, B, dt, h, shots_per_group=S,
-                               snapshot_every=0, rows_per_thread=rz, tile_x=tx)
+                               snapshot_every=0, rows_per_thread=rz, vec_x=vx)
             rec.backward(torch.ones_like(rec))
             t1.record()
             torch.cuda.synchronize()
             best = min(best, t0.elapsed_time(t1))
-        results[(S, rz, tx)] = best
+        results[(S, rz, vx)] = best
         if verbose:
-            print(f"[autotune] acoustic S={S} rz={rz} tx={tx}: {best:.2f} ms", flush=True)
+            print(f"[autotune] acoustic S={S} rz={rz} vx={vx}: {best:.2f} ms", flush=True)
     shape = min(results, key=results.get)
     record("acoustic", B, nz, nx, shape)
     return shape, results

@@ -158,13 +158,14 @@ __global__ __launch_bounds__(256) void k_step_vjp_model(
 
 // ------------------------------------------------------------ fused loop
 
-// Tiles of the fused loops: a 256-thread block is TX lanes along x by
-// 256/TX thread-rows; each thread owns RZ consecutive rows of one column for
-// S shots, so a tile is TX x (256/TX)*RZ cells.  The RZ+2 rows of the
+// Tiles of the fused loops: a 256-thread block is 64 lanes along x by 4
+// thread-rows.  Each lane owns VX consecutive x cells (one float, float2 or
+// float4 access; the grid width must be a multiple of VX) in RZ consecutive
+// rows, for S shots, so a tile is 64*VX x 4*RZ cells.  The RZ+2 rows of the
 // stencilled field's column are loaded once into registers (fully unrolled,
-// so all loads of a thread are independent); the x-neighbours are L1 hits of
-// the rows the neighbouring lanes load, except at the tile's two x-edges,
-// where they fall in the neighbouring tiles' cache lines.
+// all loads of a thread independent); the x-neighbours come from the lane's
+// own vector, plus one scalar load at each end of it (an L1 hit of the
+// neighbouring lane's vector).
 //
 // XCD-aware tile order: workgroups are handed to the 8 XCDs round-robin, so
 // workgroup w runs on XCD w % 8.  Tile t = (w % 8) * per + w / 8 gives each
@@ -173,17 +174,18 @@ __global__ __launch_bounds__(256) void k_step_vjp_model(
 // lines) are dispatched by the same XCD within one tile row and find those
 // lines in its L2.
 constexpr int kBlock = 256;
+constexpr int kLanes = 64;
+constexpr int kRowsOfThreads = kBlock / kLanes;
 constexpr int kXcds = 8;
 
 struct TileMap {
   int ntx, ntz, groups, per;  // tiles in x, tiles in z, shot groups, tiles per XCD
 };
 
-inline TileMap make_tiles(const Grid& g, int tx_w, int rz, int groups) {
+inline TileMap make_tiles(const Grid& g, int vx, int rz, int groups) {
   TileMap m;
-  const int ty = kBlock / tx_w;
-  m.ntx = (g.nx + tx_w - 1) / tx_w;
-  m.ntz = (g.nz + ty * rz - 1) / (ty * rz);
+  m.ntx = (g.nx + kLanes * vx - 1) / (kLanes * vx);
+  m.ntz = (g.nz + kRowsOfThreads * rz - 1) / (kRowsOfThreads * rz);
   m.groups = groups;
   const int64_t n = (int64_t)m.ntx * m.ntz * groups;
   m.per = (int)((n + kXcds - 1) / kXcds);
@@ -202,55 +204,98 @@ __device__ __forceinline__ bool tile_of(const TileMap m, int& tx, int& tzi, int&
   return true;
 }
 
+template <int VX>
+__device__ __forceinline__ void vload(const float* __restrict__ p, float (&o)[VX]) {
+  if constexpr (VX == 1) {
+    o[0] = *p;
+  } else if constexpr (VX == 2) {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    o[0] = t.x; o[1] = t.y;
+  } else {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+  }
+}
+
+template <int VX>
+__device__ __forceinline__ void vstore(float* __restrict__ p, const float (&o)[VX]) {
+  if constexpr (VX == 1) {
+    *p = o[0];
+  } else if constexpr (VX == 2) {
+    *reinterpret_cast<float2*>(p) = make_float2(o[0], o[1]);
+  } else {
+    *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+  }
+}
+
+template <int VX>
+__device__ __forceinline__ void vzero(float (&o)[VX]) {
+#pragma unroll
+  for (int e = 0; e < VX; ++e) o[e] = 0.0f;
+}
+
 // u^{n+1} = step(u^n, u^{n-1}) + w[n] at the source cell, written in place
 // over u^{n-1} (each cell of u^{n-1} is read only by the thread that
 // overwrites it).  Same Laplacian and update helpers as the per-step kernel.
-template <int S, int RZ, int TX>
+template <int S, int RZ, int VX>
 __global__ __launch_bounds__(kBlock) void k_fwd(
     const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ u,
     float* __restrict__ up, const int32_t* __restrict__ src, const float* __restrict__ wn, int B,
     Grid g, seis::TimeConsts k, TileMap tm) {
   int tx, tzi, grp;
   if (!tile_of(tm, tx, tzi, grp)) return;
-  constexpr int TY = kBlock / TX;
-  const int x = tx * TX + (int)(threadIdx.x % TX);
-  const int zb = (tzi * TY + (int)(threadIdx.x / TX)) * RZ;
-  if (x >= g.nx || zb >= g.nz) return;
+  const int x0 = (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
+  const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
+  if (x0 >= g.nx || zb >= g.nz) return;
   const int b0 = grp * S;
   const int nb = min(S, B - b0);
   const float w = *wn;
-  float A[RZ], Bc[RZ], c2[RZ];
+  float A[RZ][VX], Bc[RZ][VX], c2[RZ][VX];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int z = min(zb + r, g.nz - 1);
-    const float c = vp[z * g.nx + x];
-    const float dd = d[z * g.nx + x];
-    A[r] = 1.0f / (k.idt2 + dd * k.idt);
-    Bc[r] = k.idt2 - dd * k.idt;
-    c2[r] = c * c;
+    float c[VX], dd[VX];
+    vload<VX>(vp + z * g.nx + x0, c);
+    vload<VX>(d + z * g.nx + x0, dd);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      A[r][e] = 1.0f / (k.idt2 + dd[e] * k.idt);
+      Bc[r][e] = k.idt2 - dd[e] * k.idt;
+      c2[r][e] = c[e] * c[e];
+    }
   }
   for (int s = 0; s < nb; ++s) {
     const int b = b0 + s;
     const float* f = u + (int64_t)b * g.plane;
     float* o = up + (int64_t)b * g.plane;
     const int sz = src[2 * b], sx = src[2 * b + 1];
-    float col[RZ + 2];
+    float col[RZ + 2][VX];
 #pragma unroll
     for (int r = 0; r < RZ + 2; ++r) {
       const int z = zb - 1 + r;
-      col[r] = (z >= 0 && z < g.nz) ? f[z * g.nx + x] : 0.0f;
+      if (z >= 0 && z < g.nz) vload<VX>(f + z * g.nx + x0, col[r]);
+      else vzero<VX>(col[r]);
     }
 #pragma unroll
     for (int r = 0; r < RZ; ++r) {
       const int z = zb + r;
       if (z < g.nz) {
-        const int i = z * g.nx + x;
-        const float xm = x > 0 ? f[i - 1] : 0.0f;
-        const float xp = x < g.nx - 1 ? f[i + 1] : 0.0f;
-        const float L = lap_taps(col[r], xm, col[r + 1], xp, col[r + 2], k.ih2);
-        float v = acoustic_update(A[r], Bc[r], k.two_idt2, col[r + 1], o[i], c2[r], L);
-        if (z == sz && x == sx) v += w;
-        o[i] = v;
+        const int i = z * g.nx + x0;
+        const float xm = x0 > 0 ? f[i - 1] : 0.0f;
+        const float xp = x0 + VX < g.nx ? f[i + VX] : 0.0f;
+        float prev[VX], out[VX];
+        vload<VX>(o + i, prev);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+          const float left = e == 0 ? xm : col[r + 1][e - 1];
+          const float right = e == VX - 1 ? xp : col[r + 1][e + 1];
+          const float L = lap_taps(col[r][e], left, col[r + 1][e], right, col[r + 2][e], k.ih2);
+          float v = acoustic_update(A[r][e], Bc[r][e], k.two_idt2, col[r + 1][e], prev[e],
+                                    c2[r][e], L);
+          if (z == sz && x0 + e == sx) v += w;
+          out[e] = v;
+        }
+        vstore<VX>(o + i, out);
       }
     }
   }
@@ -279,13 +324,15 @@ __global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
 // m2 (was mu^{k+2}); on the interior the gradient term 2c*L(u^{k-1})*mu^k and
 // the rebuilt u^{k-2} = (2/dt^2 u^{k-1} + c^2 L u^{k-1} - (u^k - w[k-1]*src)/A)/B into
 // uc (was u^k).  recon: 0 none, 1 rebuild, 2 write zeros (u^0).  Tiles as
-// k_fwd; u^{k-1} and u^k are only touched on the interior and its ring rows.
+// k_fwd; u^{k-1} and u^k are only touched on vectors that reach the interior
+// and on the interior's rows and ring rows.  Vectors straddling the interior
+// edge write their non-interior cells back unchanged.
 //
 // The reconstruction inverts the forward's own FP32 update u^k = A*(2/dt^2
 // u^{k-1} - B u^{k-2} + c^2 L u^{k-1}) with true divisions by its A and B: the
 // shortcut 2u^{k-1} - u^k + c^2 dt^2 L assumes A*B == 1, which FP32 misses by
 // ~1 ulp and which then drifts coherently over the reversed steps.
-template <int S, int RZ, int TX>
+template <int S, int RZ, int VX>
 __global__ __launch_bounds__(kBlock) void k_bwd(
     const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ um,
     float* __restrict__ uc, const float* __restrict__ m1, float* __restrict__ m2,
@@ -293,39 +340,50 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
     int recon, int B, Grid g, seis::TimeConsts k, TileMap tm) {
   int tx, tzi, grp;
   if (!tile_of(tm, tx, tzi, grp)) return;
-  constexpr int TY = kBlock / TX;
-  const int x = tx * TX + (int)(threadIdx.x % TX);
-  const int zb = (tzi * TY + (int)(threadIdx.x / TX)) * RZ;
-  if (x >= g.nx || zb >= g.nz) return;
+  const int x0 = (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
+  const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
+  if (x0 >= g.nx || zb >= g.nz) return;
   const int b0 = grp * S;
   const int nb = min(S, B - b0);
   const float w = *wn;
-  const bool xin = x > g.npml && x < g.nx - 1 - g.npml;
+  bool xin[VX];
+  bool anyx = false;
+#pragma unroll
+  for (int e = 0; e < VX; ++e) {
+    xin[e] = x0 + e > g.npml && x0 + e < g.nx - 1 - g.npml;
+    anyx = anyx || xin[e];
+  }
   const int urow0 = g.npml, urow1 = g.nz - 1 - g.npml;  // rows of u^{k-1} that are read
-  // model column (rows zb-1 .. zb+RZ) and x-neighbours, shared by the shots
-  float c2col[RZ + 2], c2xm[RZ], c2xp[RZ], cc[RZ], A[RZ], Bc[RZ];
+  // model: c^2 on the column rows zb-1 .. zb+RZ and at the vector's two x-ends
+  float c2col[RZ + 2][VX], c2L[RZ], c2R[RZ], cc[RZ][VX], A[RZ][VX], Bc[RZ][VX];
 #pragma unroll
   for (int r = 0; r < RZ + 2; ++r) {
     const int z = zb - 1 + r;
-    const float c = (z >= 0 && z < g.nz) ? vp[z * g.nx + x] : 0.0f;
-    c2col[r] = c * c;
+    if (z >= 0 && z < g.nz) vload<VX>(vp + z * g.nx + x0, c2col[r]);
+    else vzero<VX>(c2col[r]);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) c2col[r][e] *= c2col[r][e];
   }
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int z = min(zb + r, g.nz - 1);
-    const int i = z * g.nx + x;
-    const float cm = x > 0 ? vp[i - 1] : 0.0f;
-    const float cp = x < g.nx - 1 ? vp[i + 1] : 0.0f;
-    c2xm[r] = cm * cm;
-    c2xp[r] = cp * cp;
-    cc[r] = vp[i];
-    const float dd = d[i];
-    A[r] = 1.0f / (k.idt2 + dd * k.idt);
-    Bc[r] = k.idt2 - dd * k.idt;
-  }
-  float gsum[RZ];
+    const int i = z * g.nx + x0;
+    const float cl = x0 > 0 ? vp[i - 1] : 0.0f;
+    const float cr = x0 + VX < g.nx ? vp[i + VX] : 0.0f;
+    c2L[r] = cl * cl;
+    c2R[r] = cr * cr;
+    float dd[VX];
+    vload<VX>(vp + i, cc[r]);
+    vload<VX>(d + i, dd);
 #pragma unroll
-  for (int r = 0; r < RZ; ++r) gsum[r] = 0.0f;
+    for (int e = 0; e < VX; ++e) {
+      A[r][e] = 1.0f / (k.idt2 + dd[e] * k.idt);
+      Bc[r][e] = k.idt2 - dd[e] * k.idt;
+    }
+  }
+  float gsum[RZ][VX];
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) vzero<VX>(gsum[r]);
   for (int s = 0; s < nb; ++s) {
     const int b = b0 + s;
     const int64_t off = (int64_t)b * g.plane;
@@ -334,34 +392,57 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
     const float* U = um + off;
     float* UC = uc + off;
     const int sz = src[2 * b], sx = src[2 * b + 1];
-    float mcol[RZ + 2], ucol[RZ + 2];
+    float mcol[RZ + 2][VX], ucol[RZ + 2][VX];
 #pragma unroll
     for (int r = 0; r < RZ + 2; ++r) {
       const int z = zb - 1 + r;
-      mcol[r] = (z >= 0 && z < g.nz) ? M[z * g.nx + x] : 0.0f;
-      ucol[r] = (xin && z >= urow0 && z <= urow1) ? U[z * g.nx + x] : 0.0f;
+      if (z >= 0 && z < g.nz) vload<VX>(M + z * g.nx + x0, mcol[r]);
+      else vzero<VX>(mcol[r]);
+      if (anyx && z >= urow0 && z <= urow1) vload<VX>(U + z * g.nx + x0, ucol[r]);
+      else vzero<VX>(ucol[r]);
     }
 #pragma unroll
     for (int r = 0; r < RZ; ++r) {
       const int z = zb + r;
       if (z < g.nz) {
-        const int i = z * g.nx + x;
-        const float qxm = c2xm[r] * (x > 0 ? M[i - 1] : 0.0f);
-        const float qxp = c2xp[r] * (x < g.nx - 1 ? M[i + 1] : 0.0f);
-        const float Lm = lap_taps(c2col[r] * mcol[r], qxm, c2col[r + 1] * mcol[r + 1], qxp,
-                                  c2col[r + 2] * mcol[r + 2], k.ih2);
-        const float mu = acoustic_update(A[r], Bc[r], k.two_idt2, mcol[r + 1], M2[i], 1.0f, Lm);
-        M2[i] = mu;
-        if (xin && z > g.npml && z < g.nz - 1 - g.npml) {
-          const float Lu = lap_taps(ucol[r], U[i - 1], ucol[r + 1], U[i + 1], ucol[r + 2], k.ih2);
-          gsum[r] += Lu * mu;
-          if (recon == 1) {
-            float un = UC[i];
-            if (z == sz && x == sx) un -= w;
-            UC[i] = ((k.two_idt2 * ucol[r + 1] + c2col[r + 1] * Lu) - un / A[r]) / Bc[r];
-          } else if (recon == 2) {
-            UC[i] = 0.0f;
+        const int i = z * g.nx + x0;
+        const float qL = c2L[r] * (x0 > 0 ? M[i - 1] : 0.0f);
+        const float qR = c2R[r] * (x0 + VX < g.nx ? M[i + VX] : 0.0f);
+        float mu[VX];
+        vload<VX>(M2 + i, mu);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+          const float left = e == 0 ? qL : c2col[r + 1][e - 1] * mcol[r + 1][e - 1];
+          const float right = e == VX - 1 ? qR : c2col[r + 1][e + 1] * mcol[r + 1][e + 1];
+          const float Lm = lap_taps(c2col[r][e] * mcol[r][e], left, c2col[r + 1][e] * mcol[r + 1][e],
+                                    right, c2col[r + 2][e] * mcol[r + 2][e], k.ih2);
+          mu[e] = acoustic_update(A[r][e], Bc[r][e], k.two_idt2, mcol[r + 1][e], mu[e], 1.0f, Lm);
+        }
+        vstore<VX>(M2 + i, mu);
+        if (anyx && z > g.npml && z < g.nz - 1 - g.npml) {
+          const float uL = x0 > 0 ? U[i - 1] : 0.0f;
+          const float uR = x0 + VX < g.nx ? U[i + VX] : 0.0f;
+          float uv[VX];
+          vload<VX>(UC + i, uv);
+#pragma unroll
+          for (int e = 0; e < VX; ++e) {
+            if (xin[e]) {
+              const float left = e == 0 ? uL : ucol[r + 1][e - 1];
+              const float right = e == VX - 1 ? uR : ucol[r + 1][e + 1];
+              const float Lu = lap_taps(ucol[r][e], left, ucol[r + 1][e], right, ucol[r + 2][e],
+                                        k.ih2);
+              gsum[r][e] += Lu * mu[e];
+              if (recon == 1) {
+                float un = uv[e];
+                if (z == sz && x0 + e == sx) un -= w;
+                uv[e] = ((k.two_idt2 * ucol[r + 1][e] + c2col[r + 1][e] * Lu) - un / A[r][e]) /
+                        Bc[r][e];
+              } else if (recon == 2) {
+                uv[e] = 0.0f;
+              }
+            }
           }
+          if (recon != 0) vstore<VX>(UC + i, uv);
         }
       }
     }
@@ -370,7 +451,14 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int z = zb + r;
-    if (xin && z > g.npml && z < g.nz - 1 - g.npml) G[z * g.nx + x] += 2.0f * cc[r] * gsum[r];
+    if (anyx && z > g.npml && z < g.nz - 1 - g.npml) {
+      float gv[VX];
+      vload<VX>(G + z * g.nx + x0, gv);
+#pragma unroll
+      for (int e = 0; e < VX; ++e)
+        if (xin[e]) gv[e] += 2.0f * cc[r][e] * gsum[r][e];
+      vstore<VX>(G + z * g.nx + x0, gv);
+    }
   }
 }
 
@@ -437,37 +525,35 @@ inline int copy2(const float* a, float* ao, const float* b, float* bo, int64_t n
   return 0;
 }
 
-// Instantiated launch shapes: shots per thread S, rows per thread RZ, tile
-// width TX.
-#define SEIS_RZ_TX(K, S_)                                                              \
-  case S_ * 256 + 1 * 16 + 0: K<S_, 1, 64><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 2 * 16 + 0: K<S_, 2, 64><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 4 * 16 + 0: K<S_, 4, 64><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 1 * 16 + 1: K<S_, 1, 128><<<grid, kBlock, 0, s>>>(a..., tm); break; \
-  case S_ * 256 + 2 * 16 + 1: K<S_, 2, 128><<<grid, kBlock, 0, s>>>(a..., tm); break; \
-  case S_ * 256 + 4 * 16 + 1: K<S_, 4, 128><<<grid, kBlock, 0, s>>>(a..., tm); break; \
-  case S_ * 256 + 1 * 16 + 2: K<S_, 1, 256><<<grid, kBlock, 0, s>>>(a..., tm); break; \
-  case S_ * 256 + 2 * 16 + 2: K<S_, 2, 256><<<grid, kBlock, 0, s>>>(a..., tm); break; \
-  case S_ * 256 + 4 * 16 + 2: K<S_, 4, 256><<<grid, kBlock, 0, s>>>(a..., tm); break;
+// Instantiated launch shapes: shots per thread S, rows per thread RZ, cells
+// per lane VX.
+#define SEIS_RZ_VX(K, S_)                                                              \
+  case S_ * 256 + 1 * 16 + 1: K<S_, 1, 1><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 2 * 16 + 1: K<S_, 2, 1><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 4 * 16 + 1: K<S_, 4, 1><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 1 * 16 + 2: K<S_, 1, 2><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 2 * 16 + 2: K<S_, 2, 2><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 4 * 16 + 2: K<S_, 4, 2><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 1 * 16 + 4: K<S_, 1, 4><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 2 * 16 + 4: K<S_, 2, 4><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
+  case S_ * 256 + 4 * 16 + 4: K<S_, 4, 4><<<grid, kBlock, 0, s>>>(a..., tm); break;
 
-inline int tx_code(int tile_x) { return tile_x == 64 ? 0 : tile_x == 128 ? 1 : 2; }
-
-inline bool valid_shape(int S, int RZ, int TX) {
+inline bool valid_shape(int S, int RZ, int VX) {
   return (S == 1 || S == 2 || S == 4 || S == 8) && (RZ == 1 || RZ == 2 || RZ == 4) &&
-         (TX == 64 || TX == 128 || TX == 256);
+         (VX == 1 || VX == 2 || VX == 4);
 }
 
 #define SEIS_DEFINE_LAUNCH(NAME, K)                                                   \
   template <typename... Args>                                                        \
-  int NAME(int S, int RZ, int TX, const TileMap& tm, hipStream_t s, Args... a) {     \
+  int NAME(int S, int RZ, int VX, const TileMap& tm, hipStream_t s, Args... a) {     \
     const dim3 grid(tm.per * kXcds);                                                 \
-    switch (S * 256 + RZ * 16 + tx_code(TX)) {                                       \
-      SEIS_RZ_TX(K, 1)                                                               \
-      SEIS_RZ_TX(K, 2)                                                               \
-      SEIS_RZ_TX(K, 4)                                                               \
-      SEIS_RZ_TX(K, 8)                                                               \
+    switch (S * 256 + RZ * 16 + VX) {                                                \
+      SEIS_RZ_VX(K, 1)                                                               \
+      SEIS_RZ_VX(K, 2)                                                               \
+      SEIS_RZ_VX(K, 4)                                                               \
+      SEIS_RZ_VX(K, 8)                                                               \
       default:                                                                       \
-        seis::set_error("unsupported launch shape (%d, %d, %d)", S, RZ, TX);         \
+        seis::set_error("unsupported launch shape (%d, %d, %d)", S, RZ, VX);         \
         return 1;                                                                    \
     }                                                                                \
     SEIS_CHECK_LAUNCH();                                                             \
@@ -486,6 +572,10 @@ inline int fill(float* p, int64_t n, float v, hipStream_t s) {
   k_fill<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
   SEIS_CHECK_LAUNCH();
   return 0;
+}
+
+inline bool aligned(const void* p, int bytes) {
+  return p == nullptr || (reinterpret_cast<uintptr_t>(p) % (uintptr_t)bytes) == 0;
 }
 
 int check_grid(int64_t B, int64_t nz, int64_t nx, float dt, float h) {
@@ -550,13 +640,18 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
                           float* u_a, float* u_b, int64_t B, int64_t nz, int64_t nx, float dt,
                           float h, int32_t npml, int32_t shots_per_group,
-                          int32_t rows_per_thread, int32_t tile_x, void* stream) {
+                          int32_t rows_per_thread, int32_t vec_x, void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
-  SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, tile_x),
-                 "launch shape (shots_per_group %d, rows_per_thread %d, tile_x %d) not in "
-                 "{1,2,4,8} x {1,2,4} x {64,128,256}",
-                 shots_per_group, rows_per_thread, tile_x);
+  SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, vec_x),
+                 "launch shape (shots_per_group %d, rows_per_thread %d, vec_x %d) not in "
+                 "{1,2,4,8} x {1,2,4} x {1,2,4}",
+                 shots_per_group, rows_per_thread, vec_x);
+  SEIS_CHECK_ARG(nx % vec_x == 0, "vec_x %d does not divide nx = %lld", vec_x, (long long)nx);
   SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b, "null pointer argument");
+  SEIS_CHECK_ARG(aligned(vp, 4 * vec_x) && aligned(d, 4 * vec_x) && aligned(u_a, 4 * vec_x) &&
+                     aligned(u_b, 4 * vec_x),
+                 "field and model pointers must be %d-byte aligned for vec_x %d", 4 * vec_x,
+                 vec_x);
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
   SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1");
@@ -571,7 +666,7 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   const int G = (int)((B + S - 1) / S);
   // After nt steps u^{nt} must sit in u_a: step n writes into the buffer that
   // holds u^{n-1}, so u^{n} lives in the step-0 "current" buffer iff n is even.
-  const TileMap tm = make_tiles(g, tile_x, rows_per_thread, G);
+  const TileMap tm = make_tiles(g, vec_x, rows_per_thread, G);
   float* cur = (nt % 2 == 0) ? u_a : u_b;
   float* prev = (nt % 2 == 0) ? u_b : u_a;
   if (int e = fill(u_a, B * g.plane, 0.0f, s)) return e;
@@ -579,7 +674,7 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   const int64_t aux = R + ((ring && nt > 1) ? B * ring_len : 0);
   for (int64_t n = 0; n < nt; ++n) {
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
-    if (int e = launch_fwd(S, rows_per_thread, tile_x, tm, s, vp, d, cur, prev, src_zx,
+    if (int e = launch_fwd(S, rows_per_thread, vec_x, tm, s, vp, d, cur, prev, src_zx,
                            wavelet + n, (int)B, g, k))
       return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
@@ -606,15 +701,21 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            int32_t snap_every, float* u_a, float* u_b,
                            float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
                            int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                           int32_t shots_per_group, int32_t rows_per_thread, int32_t tile_x,
+                           int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                            void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
-  SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, tile_x),
-                 "launch shape (shots_per_group %d, rows_per_thread %d, tile_x %d) not in "
-                 "{1,2,4,8} x {1,2,4} x {64,128,256}",
-                 shots_per_group, rows_per_thread, tile_x);
+  SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, vec_x),
+                 "launch shape (shots_per_group %d, rows_per_thread %d, vec_x %d) not in "
+                 "{1,2,4,8} x {1,2,4} x {1,2,4}",
+                 shots_per_group, rows_per_thread, vec_x);
+  SEIS_CHECK_ARG(nx % vec_x == 0, "vec_x %d does not divide nx = %lld", vec_x, (long long)nx);
   SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b && mu_a && mu_b && grad_part && grad,
                  "null pointer argument");
+  SEIS_CHECK_ARG(aligned(vp, 4 * vec_x) && aligned(d, 4 * vec_x) && aligned(u_a, 4 * vec_x) &&
+                     aligned(u_b, 4 * vec_x) && aligned(mu_a, 4 * vec_x) &&
+                     aligned(mu_b, 4 * vec_x) && aligned(grad_part, 4 * vec_x),
+                 "field and model pointers must be %d-byte aligned for vec_x %d", 4 * vec_x,
+                 vec_x);
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_grad)), "bad receiver arguments");
   SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1");
@@ -631,7 +732,7 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   if (int e = fill(mu_a, B * g.plane, 0.0f, s)) return e;
   if (int e = fill(mu_b, B * g.plane, 0.0f, s)) return e;
   if (int e = fill(grad_part, (int64_t)G * g.plane, 0.0f, s)) return e;
-  const TileMap tm = make_tiles(g, tile_x, rows_per_thread, G);
+  const TileMap tm = make_tiles(g, vec_x, rows_per_thread, G);
   float* X = u_b;   // u^{k-1}
   float* Y = u_a;   // u^k  -> u^{k-2}
   float* M1 = mu_a; // mu^{k+1}
@@ -639,7 +740,7 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   for (int64_t kk = nt; kk >= 1; --kk) {
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
-    if (int e = launch_bwd(S, rows_per_thread, tile_x, tm, s, vp, d, X, Y, M1, M2, grad_part, src_zx,
+    if (int e = launch_bwd(S, rows_per_thread, vec_x, tm, s, vp, d, X, Y, M1, M2, grad_part, src_zx,
                            wavelet + (kk - 1), recon, (int)B, g, k))
       return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);

@@ -121,9 +121,13 @@ def default_snapshot_every(nt, B, nz, nx, device):
     return max(SNAP_MIN_EVERY, -(-(nt - 2) // max_snaps))
 
 
-# launch shape when no tuned value is known: rows per thread, tile width
+# launch shape when no tuned value is known: rows per thread (cells per lane
+# along x is the widest of 4, 2, 1 that divides nx)
 ROWS_PER_THREAD = 2
-TILE_X = 256
+
+
+def default_vec_x(nx):
+    return 4 if nx % 4 == 0 else 2 if nx % 2 == 0 else 1
 
 
 def default_shots_per_group(B):
@@ -135,10 +139,10 @@ def default_shots_per_group(B):
 
 
 def _launch_params(B, nz, nx, shape):
-    """(shots_per_group, rows_per_thread, tile_x): caller's, tuned, or default."""
+    """(shots_per_group, rows_per_thread, vec_x): caller's, tuned, or default."""
     from .. import autotune
     tuned = autotune.lookup("acoustic", B, nz, nx)
-    default = tuned if tuned else (default_shots_per_group(B), ROWS_PER_THREAD, TILE_X)
+    default = tuned if tuned else (default_shots_per_group(B), ROWS_PER_THREAD, default_vec_x(nx))
     shape = tuple(shape) if shape is not None else (None, None, None)
     return tuple(int(v if v is not None else dv) for v, dv in zip(shape, default))
 
@@ -174,7 +178,7 @@ class AcousticLoop(torch.autograd.Function):
 
     Inputs: vp, d (nz, nx); wavelet (nt,); src_zx (B, 2) int32 padded (z, x);
     rec_flat (R,) int64 flat indices into (B, nz, nx); B; dt; h; launch shape
-    (shots_per_group, rows_per_thread, tile_x), None entries automatic;
+    (shots_per_group, rows_per_thread, vec_x), None entries automatic;
     snapshot_every (None: automatic, 0: ring only as in the reference).
     """
 
@@ -184,7 +188,7 @@ class AcousticLoop(torch.autograd.Function):
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
-        S, rz, tx = _launch_params(B, nz, nx, shape)
+        S, rz, vx = _launch_params(B, nz, nx, shape)
         need_grad = ctx.needs_input_grad[0]
         _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad)
         u_a = torch.empty((B, nz, nx), device=vp.device, dtype=torch.float32)
@@ -207,11 +211,11 @@ class AcousticLoop(torch.autograd.Function):
         _lib.call("seis_acoustic_forward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K, _lib.ptr(u_a), _lib.ptr(u_b), B, nz, nx, dt, h, NPML, S,
-                  rz, tx, _lib.stream_of(vp))
+                  rz, vx, _lib.stream_of(vp))
         if need_grad:
             ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
             ctx.state = (ring, snaps, u_a, u_b)
-            ctx.meta = (B, dt, h, S, K, rz, tx)
+            ctx.meta = (B, dt, h, S, K, rz, vx)
         return rec
 
     @staticmethod
@@ -220,7 +224,7 @@ class AcousticLoop(torch.autograd.Function):
         vp, d, wavelet, src_zx, rec_flat = ctx.saved_tensors
         ring, snaps, u_a, u_b = ctx.state
         ctx.state = None  # the backward consumes the final forward state
-        B, dt, h, S, K, rz, tx = ctx.meta
+        B, dt, h, S, K, rz, vx = ctx.meta
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -234,18 +238,19 @@ class AcousticLoop(torch.autograd.Function):
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(grad_rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b),
                   _lib.ptr(mu_a), _lib.ptr(mu_b), _lib.ptr(gpart),
-                  _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, rz, tx, _lib.stream_of(vp))
+                  _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, rz, vx, _lib.stream_of(vp))
         return grad, None, None, None, None, None, None, None, None, None
 
 
 def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
-              snapshot_every=None, rows_per_thread=None, tile_x=None):
+              snapshot_every=None, rows_per_thread=None, vec_x=None):
     """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint.
 
     The launch shape -- shots_per_group (1, 2, 4, 8), rows_per_thread
-    (1, 2, 4), tile_x (64, 128, 256) -- defaults to the autotuned values for
-    this shape (``seistorch_amd.autotune``), else to fixed defaults.
+    (1, 2, 4), vec_x (cells per lane along x: 1, 2, 4, dividing nx) --
+    defaults to the autotuned values for this shape (``seistorch_amd.autotune``),
+    else to fixed defaults.
     """
     return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
                               rec_flat.contiguous(), int(B), _f(dt), _f(h),
-                              (shots_per_group, rows_per_thread, tile_x), snapshot_every)
+                              (shots_per_group, rows_per_thread, vec_x), snapshot_every)

@@ -93,9 +93,11 @@ def test_single_step_vjp_matches_autograd():
     assert max(errs) < 1e-5
 
 
-@pytest.mark.parametrize("S", [None, 1, 2, 4])
-def test_fused_forward_records(S):
-    c = acoustic_case(nshots=5)
+@pytest.mark.parametrize("S,nx_i", [(None, 90), (1, 90), (2, 90), (4, 90), (None, 91),
+                                    (None, 92)])
+def test_fused_forward_records(S, nx_i):
+    """nx_i 90, 91, 92 -> padded nx 190, 191, 192: default vec_x 2, 1, 4."""
+    c = acoustic_case(nshots=5, nx_i=nx_i)
     t = to_dev(c)
     rec = _hip_records(t, c, S).cpu().numpy()
     ro = oa.forward(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec)
@@ -111,17 +113,16 @@ def test_fused_forward_records(S):
 
 def test_launch_shapes_agree():
     """Records are bitwise identical for every (shots_per_group, rows_per_thread,
-    tile_x); the
+    vec_x); the
     gradient only changes by the shot summation order."""
     from seistorch_amd.equations2d import acoustic as ac
-    c = acoustic_case(nshots=5, nt=150)
+    c = acoustic_case(nshots=5, nt=150, nx_i=96)  # nx = 196: every vec_x divides it
     t = to_dev(c)
     ref_rec, ref_g = None, None
-    for S, tz, tx in [(1, 1, 64), (2, 2, 128), (4, 4, 256), (8, 1, 256), (8, 4, 64),
-                      (2, 4, 128)]:
+    for S, tz, vx in [(1, 1, 1), (2, 2, 2), (4, 4, 4), (8, 1, 4), (8, 4, 1), (2, 4, 2)]:
         vp = t["vp"].clone().requires_grad_(True)
         rec = ac.propagate(vp, t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h,
-                           shots_per_group=S, rows_per_thread=tz, tile_x=tx)
+                           shots_per_group=S, rows_per_thread=tz, vec_x=vx)
         rec.backward(torch.ones_like(rec))
         r = rec.detach().cpu().numpy()
         g = vp.grad.cpu().numpy()
