@@ -33,6 +33,8 @@ int32_t seis_abi_version(void);
  * of one kernel since the previous collect. */
 #define SEIS_KERNEL_ACOUSTIC_FWD 0
 #define SEIS_KERNEL_ACOUSTIC_BWD 1
+#define SEIS_KERNEL_ELASTIC_FWD 2  /* stress + velocity stages of one step */
+#define SEIS_KERNEL_ELASTIC_BWD 3  /* both adjoint stages of one step */
 #define SEIS_PROFILE_KERNELS 8
 int seis_profile_enable(int32_t on);
 int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
@@ -105,6 +107,76 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            int64_t nz, int64_t nx, float dt, float h, int32_t npml,
                            int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                            void* stream);
+
+/* ---------------- elastic, isotropic velocity-stress, PML ---------------- */
+/* Fields are passed as an array of five (B, nz, nx) pointers in the order
+ * vx, vz, txx, tzz, txz (seistorch/eqconfigure.py Wavefield.elastic).  The
+ * model enters through six per-cell coefficient planes written by
+ * seis_elastic_coefficients: lam+2mu, lam, dt*mu/h, dt/(rho h), 1/(1+c), 1-c
+ * with lam = rho(vp^2 - 2vs^2), mu = rho vs^2, c = dt*d/2 (elastic.py:11-14). */
+
+/* coef (6 x nz x nx) from vp, vs, rho, d with the reference's FP32 order. */
+int seis_elastic_coefficients(const float* vp, const float* vs, const float* rho, const float* d,
+                              float* coef, int64_t nz, int64_t nx, float dt, float h,
+                              void* stream);
+
+/* One stage of the step, in place on the cells z0 <= z < z1, x0 <= x < x1:
+ * stage 0 updates the stresses from the velocities (elastic.py:16-24),
+ * stage 1 the velocities from the stresses (elastic.py:26-35); sign +1 is the
+ * forward step (stage 0 then 1, whole grid), sign -1 on the interior region
+ * is the reference's reverse step (elastic.py:39-122: stage 1 then 0). */
+int seis_elastic_stage(int32_t stage, const float* coef, float* const* fields, float sign,
+                       int32_t z0, int32_t z1, int32_t x0, int32_t x1, int64_t B, int64_t nz,
+                       int64_t nx, float dt, float h, int32_t shots_per_group, void* stream);
+
+/* Vector-Jacobian product of seis_elastic_stage (replaces torch.autograd
+ * through elastic.py:7-37): cot_in are cotangents of the stage's outputs,
+ * cot_out receives those of its inputs (no aliasing), fwd holds the stage's
+ * input fields.  If grad_coef != NULL, the gradient w.r.t. the coefficient
+ * planes is accumulated into it, grouped by shots_per_group: block q (4
+ * planes: lam+2mu, lam, dt*mu/h, dt/(rho h)) for shots q*S .. q*S+S-1. */
+int seis_elastic_stage_vjp(int32_t stage, const float* coef, float* const* cot_in,
+                           float* const* cot_out, float* const* fwd, float* grad_coef,
+                           float sign, int32_t z0, int32_t z1, int32_t x0, int32_t x1, int64_t B,
+                           int64_t nz, int64_t nx, float dt, float h, int32_t shots_per_group,
+                           void* stream);
+
+/* Sum the `groups` coefficient-gradient blocks and chain them to dJ/dvp,
+ * dJ/dvs, dJ/drho (nz x nx each). */
+int seis_elastic_chain(const float* grad_coef, int32_t groups, const float* vp, const float* vs,
+                       const float* rho, float* gvp, float* gvs, float* grho, int64_t nz,
+                       int64_t nx, float dt, float h, void* stream);
+
+/* Whole forward loop (rnn.py:176-212 with elastic.py:7-37, time offset 0):
+ * per step the two stages, then wavelet[n] added to every field whose bit is
+ * set in src_mask (bit f = field f) at src_zx[b] (source.py:47-52), then
+ * channel c of receiver r recorded from field rec_field[c] into
+ * rec_out[(n*R + r)*nch + c] (probe.py:42-44).  ring (if not NULL): the rings
+ * of all five fields of state n+1 in slot n ((nt-1) x 5 x B x ring_len);
+ * snaps (if not NULL): state m for m = j*snap_every <= nt-1 in slot j-1
+ * (5 x B x nz x nx each).  fields are zeroed first and hold state nt on
+ * return. */
+int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
+                         const int32_t* src_zx, int32_t src_mask, const int64_t* rec_flat,
+                         int64_t R, const int32_t* rec_field, int32_t nch, float* rec_out,
+                         float* ring, float* snaps, int32_t snap_every, float* const* fields,
+                         int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                         int32_t shots_per_group, void* stream);
+
+/* Adjoint of seis_elastic_forward with boundary-saving reconstruction
+ * (replaces seistorch/checkpoint.py:108-230 and elastic.py:39-122 with the
+ * exact time pairing).  fields: the final state from the forward (consumed);
+ * adj_a, adj_b: two sets of five (B, nz, nx) work buffers; grad_coef:
+ * ceil(B/S) x 4 planes, receives the coefficient gradients on the interior
+ * (pass them to seis_elastic_chain). */
+int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
+                          const int32_t* src_zx, int32_t src_mask, const int64_t* rec_flat,
+                          int64_t R, const int32_t* rec_field, int32_t nch,
+                          const float* rec_grad, const float* ring, const float* snaps,
+                          int32_t snap_every, float* const* fields, float* const* adj_a,
+                          float* const* adj_b, float* grad_coef, int64_t B, int64_t nz,
+                          int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
+                          void* stream);
 
 #ifdef __cplusplus
 }

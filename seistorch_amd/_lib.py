@@ -36,6 +36,19 @@ SIGNATURES = {
     "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
                                               _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
                                               _i32, _i32, _i32, _i32, _p]),
+    "seis_elastic_coefficients": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _f, _f, _p]),
+    "seis_elastic_stage": (ctypes.c_int, [_i32, _p, _p, _f, _i32, _i32, _i32, _i32, _i64, _i64,
+                                          _i64, _f, _f, _i32, _p]),
+    "seis_elastic_stage_vjp": (ctypes.c_int, [_i32, _p, _p, _p, _p, _p, _f, _i32, _i32, _i32,
+                                              _i32, _i64, _i64, _i64, _f, _f, _i32, _p]),
+    "seis_elastic_chain": (ctypes.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _i64, _i64, _f, _f,
+                                          _p]),
+    "seis_elastic_forward": (ctypes.c_int, [_p, _p, _i64, _p, _i32, _p, _i64, _p, _i32, _p, _p,
+                                            _p, _i32, _p, _i64, _i64, _i64, _f, _f, _i32, _i32,
+                                            _p]),
+    "seis_elastic_backward": (ctypes.c_int, [_p, _p, _i64, _p, _i32, _p, _i64, _p, _i32, _p, _p,
+                                             _p, _i32, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
+                                             _i32, _i32, _p]),
 }
 
 ABI_VERSION = 1
@@ -77,6 +90,11 @@ def call(name, *args):
 def ptr(t):
     """Device pointer of a tensor (None -> NULL)."""
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def ptr_array(tensors):
+    """A C array of device pointers (float* const*), kept alive by the caller."""
+    return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
 
 
 def stream_of(t):

@@ -20,17 +20,17 @@
 // once per S shot-updates.  Neighbour reads hit L1/L2.
 #include "common.h"
 #include "seistorch_amd.h"
+#include "tiles.h"
+
+using seis::Grid;
+using namespace seis::tiles;
 
 namespace {
 
 constexpr int kTX = 64;
 constexpr int kTZ = 4;
 
-struct Grid {
-  int nz, nx;
-  int npml;
-  int64_t plane;  // nz*nx
-};
+
 
 // 5-point Laplacian from its taps: the h^-2-weighted taps accumulated in
 // row-major kernel order (z-1, x-1, centre, x+1, z+1) with fused multiply-adds,
@@ -60,28 +60,6 @@ __device__ __forceinline__ float lap5(const float* __restrict__ f, int z, int x,
   const float xm = x > 0 ? f[i - 1] : 0.0f;
   const float xp = x < g.nx - 1 ? f[i + 1] : 0.0f;
   return lap_taps(zm, xm, c, xp, zp, ih2);
-}
-
-__device__ __forceinline__ bool interior(int z, int x, const Grid g) {
-  return z > g.npml && z < g.nz - 1 - g.npml && x > g.npml && x < g.nx - 1 - g.npml;
-}
-
-// j-th cell of the boundary-saving ring: top row, bottom row, left column,
-// right column, each restricted to the 5-point neighbours of the interior.
-__device__ __forceinline__ int64_t ring_cell(int64_t j, const Grid g) {
-  const int nxi = g.nx - 2 * g.npml - 2;
-  const int nzi = g.nz - 2 * g.npml - 2;
-  int z, x;
-  if (j < nxi) {
-    z = g.npml; x = g.npml + 1 + (int)j;
-  } else if (j < 2 * nxi) {
-    z = g.nz - 1 - g.npml; x = g.npml + 1 + (int)(j - nxi);
-  } else if (j < 2 * nxi + nzi) {
-    z = g.npml + 1 + (int)(j - 2 * nxi); x = g.npml;
-  } else {
-    z = g.npml + 1 + (int)(j - 2 * nxi - nzi); x = g.nx - 1 - g.npml;
-  }
-  return (int64_t)z * g.nx + x;
 }
 
 // ---------------------------------------------------------------- per step
@@ -157,82 +135,6 @@ __global__ __launch_bounds__(256) void k_step_vjp_model(
 }
 
 // ------------------------------------------------------------ fused loop
-
-// Tiles of the fused loops: a 256-thread block is 64 lanes along x by 4
-// thread-rows.  Each lane owns VX consecutive x cells (one float, float2 or
-// float4 access; the grid width must be a multiple of VX) in RZ consecutive
-// rows, for S shots, so a tile is 64*VX x 4*RZ cells.  The RZ+2 rows of the
-// stencilled field's column are loaded once into registers (fully unrolled,
-// all loads of a thread independent); the x-neighbours come from the lane's
-// own vector, plus one scalar load at each end of it (an L1 hit of the
-// neighbouring lane's vector).
-//
-// XCD-aware tile order: workgroups are handed to the 8 XCDs round-robin, so
-// workgroup w runs on XCD w % 8.  Tile t = (w % 8) * per + w / 8 gives each
-// XCD a contiguous range of tiles in time order.  Tiles are numbered
-// x-fastest, so the x- and z-neighbours of a tile (which own its halo cache
-// lines) are dispatched by the same XCD within one tile row and find those
-// lines in its L2.
-constexpr int kBlock = 256;
-constexpr int kLanes = 64;
-constexpr int kRowsOfThreads = kBlock / kLanes;
-constexpr int kXcds = 8;
-
-struct TileMap {
-  int ntx, ntz, groups, per;  // tiles in x, tiles in z, shot groups, tiles per XCD
-};
-
-inline TileMap make_tiles(const Grid& g, int vx, int rz, int groups) {
-  TileMap m;
-  m.ntx = (g.nx + kLanes * vx - 1) / (kLanes * vx);
-  m.ntz = (g.nz + kRowsOfThreads * rz - 1) / (kRowsOfThreads * rz);
-  m.groups = groups;
-  const int64_t n = (int64_t)m.ntx * m.ntz * groups;
-  m.per = (int)((n + kXcds - 1) / kXcds);
-  return m;
-}
-
-// -> false if this workgroup has no tile (padding of the XCD split)
-__device__ __forceinline__ bool tile_of(const TileMap m, int& tx, int& tzi, int& grp) {
-  const int w = blockIdx.x;
-  const int64_t t = (int64_t)(w % kXcds) * m.per + w / kXcds;
-  if (t >= (int64_t)m.ntx * m.ntz * m.groups) return false;
-  tx = (int)(t % m.ntx);
-  const int64_t r = t / m.ntx;
-  tzi = (int)(r % m.ntz);
-  grp = (int)(r / m.ntz);
-  return true;
-}
-
-template <int VX>
-__device__ __forceinline__ void vload(const float* __restrict__ p, float (&o)[VX]) {
-  if constexpr (VX == 1) {
-    o[0] = *p;
-  } else if constexpr (VX == 2) {
-    const float2 t = *reinterpret_cast<const float2*>(p);
-    o[0] = t.x; o[1] = t.y;
-  } else {
-    const float4 t = *reinterpret_cast<const float4*>(p);
-    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
-  }
-}
-
-template <int VX>
-__device__ __forceinline__ void vstore(float* __restrict__ p, const float (&o)[VX]) {
-  if constexpr (VX == 1) {
-    *p = o[0];
-  } else if constexpr (VX == 2) {
-    *reinterpret_cast<float2*>(p) = make_float2(o[0], o[1]);
-  } else {
-    *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
-  }
-}
-
-template <int VX>
-__device__ __forceinline__ void vzero(float (&o)[VX]) {
-#pragma unroll
-  for (int e = 0; e < VX; ++e) o[e] = 0.0f;
-}
 
 // u^{n+1} = step(u^n, u^{n-1}) + w[n] at the source cell, written in place
 // over u^{n-1} (each cell of u^{n-1} is read only by the thread that

@@ -1,0 +1,663 @@
+// 2D isotropic elastic velocity-stress kernels for gfx950.
+//
+// One time step (seistorch/equations2d/elastic.py:7-37) is two linear stages:
+//   T (stresses from velocities)
+//     txx' = a*(s*dt/h*((lam+2mu) Dx- vx + lam Dz+ vz) + b*txx)
+//     tzz' = a*(s*dt/h*((lam+2mu) Dz+ vz + lam Dx- vx) + b*tzz)
+//     txz' = a*(s*(dt*mu)/h*(Dx+ vz + Dz- vx) + b*txz)
+//   V (velocities from the new stresses)
+//     vx' = a*(s*(dt/rho)/h*(Dx+ txx' + Dz+ txz') + b*vx)
+//     vz' = a*(s*(dt/rho)/h*(Dx- txz' + Dz- tzz') + b*vz)
+// with a = 1/(1+c), b = 1-c, c = dt*d/2, D- / D+ the zero-ended differences
+// of seistorch/equations2d/utils.py:3-48, s = +1.  The forward step is T then
+// V; the reference's reverse step (elastic.py:39-122) is V then T with s = -1
+// on the interior region.  Each stage kernel works in place on a region.
+//
+// adjT / adjV are the exact transposes of the stages (out of place), with
+// optional accumulation of the gradients w.r.t. the per-cell coefficients
+// (lam+2mu, lam, dt*mu/h, dt/(rho h)) and, in the fused adjoint loop, the
+// reverse-time rebuild of the forward fields on the undamped interior from
+// the boundary-saving rings.  k_el_chain maps the coefficient gradients to
+// vp, vs and rho.
+//
+// The coefficients are precomputed per cell with the reference's FP32
+// operation order (k_el_coef), and the stage kernels round op by op in the
+// reference's expression order, so the forward matches its FP32 path.
+#include "common.h"
+#include "seistorch_amd.h"
+#include "tiles.h"
+
+using seis::Grid;
+using namespace seis::tiles;
+
+namespace {
+
+// coefficient planes, each (nz, nx)
+enum { kL2M = 0, kLam, kMc, kRc, kAinv, kBm, kNumCoef };
+
+struct Region {
+  int z0, z1, x0, x1;  // cells z0 <= z < z1, x0 <= x < x1
+};
+
+__device__ __forceinline__ bool in_region(int z, int x, const Region r) {
+  return z >= r.z0 && z < r.z1 && x >= r.x0 && x < r.x1;
+}
+
+struct Fields {
+  float* f[5];  // vx, vz, txx, tzz, txz, each B x (nz, nx)
+};
+
+struct CFields {
+  const float* f[5];
+};
+
+// One cell per thread (64 x 4 tiles), S shots per thread, XCD-aware order.
+__device__ __forceinline__ bool cell_of(const TileMap tm, const Grid g, int& z, int& x,
+                                        int& grp) {
+  int tx, tzi;
+  if (!tile_of(tm, tx, tzi, grp)) return false;
+  x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  return x < g.nx && z < g.nz;
+}
+
+__global__ __launch_bounds__(256) void k_el_coef(const float* __restrict__ vp,
+                                                 const float* __restrict__ vs,
+                                                 const float* __restrict__ rho,
+                                                 const float* __restrict__ d,
+                                                 float* __restrict__ cf, float dt, float ih,
+                                                 int64_t plane) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= plane) return;
+  const float p = vp[i], s = vs[i], r = rho[i];
+  const float lam = r * (p * p - 2.0f * (s * s));   // rho*(vp.pow(2)-2*vs.pow(2))
+  const float mu = r * (s * s);                      // rho*(vs.pow(2))
+  const float c = (0.5f * dt) * d[i];                // 0.5*dt*d
+  cf[kL2M * plane + i] = lam + 2.0f * mu;
+  cf[kLam * plane + i] = lam;
+  cf[kMc * plane + i] = (dt * mu) * ih;              // dt*lame_mu*h.pow(-1)
+  cf[kRc * plane + i] = (dt * (1.0f / r)) * ih;      // dt*rho.pow(-1)*h.pow(-1)
+  cf[kAinv * plane + i] = 1.0f / (1.0f + c);         // (1+c)**-1
+  cf[kBm * plane + i] = 1.0f - c;                    // (1-c)
+}
+
+// Stage T in place over the region: stresses from velocities.
+template <int S>
+__global__ __launch_bounds__(256) void k_el_T(const float* __restrict__ cf, float sdti,
+                                              float sgn, Region rg, Fields F, int B, Grid g,
+                                              TileMap tm) {
+  int z, x, grp;
+  if (!cell_of(tm, g, z, x, grp) || !in_region(z, x, rg)) return;
+  const int i = z * g.nx + x;
+  const int64_t P = g.plane;
+  const float l2m = cf[kL2M * P + i], lam = cf[kLam * P + i];
+  const float smc = sgn * cf[kMc * P + i];
+  const float a = cf[kAinv * P + i], bm = cf[kBm * P + i];
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int64_t off = (int64_t)(b0 + s) * P + i;
+    const float* vx = F.f[0] + off;
+    const float* vz = F.f[1] + off;
+    const float vx_x = x > 0 ? vx[0] - vx[-1] : 0.0f;
+    const float vz_z = z < g.nz - 1 ? vz[g.nx] - vz[0] : 0.0f;
+    const float vx_z = z > 0 ? vx[0] - vx[-g.nx] : 0.0f;
+    const float vz_x = x < g.nx - 1 ? vz[1] - vz[0] : 0.0f;
+    float* txx = F.f[2] + off;
+    float* tzz = F.f[3] + off;
+    float* txz = F.f[4] + off;
+    *txx = a * (sdti * (l2m * vx_x + lam * vz_z) + bm * *txx);
+    *tzz = a * (sdti * (l2m * vz_z + lam * vx_x) + bm * *tzz);
+    *txz = a * (smc * (vz_x + vx_z) + bm * *txz);
+  }
+}
+
+// Stage V in place over the region: velocities from stresses.
+template <int S>
+__global__ __launch_bounds__(256) void k_el_V(const float* __restrict__ cf, float sgn, Region rg,
+                                              Fields F, int B, Grid g, TileMap tm) {
+  int z, x, grp;
+  if (!cell_of(tm, g, z, x, grp) || !in_region(z, x, rg)) return;
+  const int i = z * g.nx + x;
+  const int64_t P = g.plane;
+  const float src = sgn * cf[kRc * P + i];
+  const float a = cf[kAinv * P + i], bm = cf[kBm * P + i];
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int64_t off = (int64_t)(b0 + s) * P + i;
+    const float* txx = F.f[2] + off;
+    const float* tzz = F.f[3] + off;
+    const float* txz = F.f[4] + off;
+    const float txx_x = x < g.nx - 1 ? txx[1] - txx[0] : 0.0f;
+    const float txz_z = z < g.nz - 1 ? txz[g.nx] - txz[0] : 0.0f;
+    const float tzz_z = z > 0 ? tzz[0] - tzz[-g.nx] : 0.0f;
+    const float txz_x = x > 0 ? txz[0] - txz[-1] : 0.0f;
+    float* vx = F.f[0] + off;
+    float* vz = F.f[1] + off;
+    *vx = a * (src * (txx_x + txz_z) + bm * *vx);
+    *vz = a * (src * (txz_x + tzz_z) + bm * *vz);
+  }
+}
+
+// Transpose of stage V.  in: cotangents of the stage outputs; out: of its
+// inputs.  fw: the stage's input fields (stresses used for the Rc gradient);
+// gmode 0 none, 1 every region cell, 2 interior cells only; with recon the
+// velocities in fw are rebuilt in place on the interior (the reverse of V).
+template <int S>
+__global__ __launch_bounds__(256) void k_el_adjV(const float* __restrict__ cf, float sgn,
+                                                 Region rg, CFields in, Fields out, Fields fw,
+                                                 float* __restrict__ gRc, int gmode, int recon,
+                                                 int B, Grid g, TileMap tm) {
+  int z, x, grp;
+  if (!cell_of(tm, g, z, x, grp)) return;
+  const int i = z * g.nx + x;
+  const int64_t P = g.plane;
+  const float* A = cf + kAinv * P;
+  const float* R = cf + kRc * P;
+  // s*a*Rc at the cell and the neighbours whose t_x / t_z reach it (0 outside)
+  auto aR = [&](int zz, int xx) {
+    return in_region(zz, xx, rg) ? sgn * A[zz * g.nx + xx] * R[zz * g.nx + xx] : 0.0f;
+  };
+  const bool reg = in_region(z, x, rg);
+  const float c0 = aR(z, x);
+  const float cxm = x > 0 ? aR(z, x - 1) : 0.0f;
+  const float czm = z > 0 ? aR(z - 1, x) : 0.0f;
+  const float cxp = x < g.nx - 1 ? aR(z, x + 1) : 0.0f;
+  const float czp = z < g.nz - 1 ? aR(z + 1, x) : 0.0f;
+  const float own = reg ? A[i] * cf[kBm * P + i] : 1.0f;
+  const bool grad = gRc && reg && (gmode == 1 || interior(z, x, g));
+  const bool rebuild = recon && interior(z, x, g);
+  const float rc = R[i];
+  float gacc = 0.0f;
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int64_t off = (int64_t)(b0 + s) * P + i;
+    const float* avx = in.f[0] + off;
+    const float* avz = in.f[1] + off;
+    // t_x = s*a*Rc*avx, t_z likewise
+    const float tx0 = c0 * avx[0];
+    const float tz0 = c0 * avz[0];
+    const float txm = x > 0 ? cxm * avx[-1] : 0.0f;
+    const float txzm = z > 0 ? czm * avx[-g.nx] : 0.0f;
+    const float tzxp = x < g.nx - 1 ? cxp * avz[1] : 0.0f;
+    const float tzzp = z < g.nz - 1 ? czp * avz[g.nx] : 0.0f;
+    const bool xl = x >= 1, xr = x <= g.nx - 2, zl = z >= 1, zr = z <= g.nz - 2;
+    // Dx+^T t_x, Dz+^T t_x + Dx-^T t_z, Dz-^T t_z
+    out.f[2][off] = in.f[2][off] + ((xl ? txm : 0.0f) - (xr ? tx0 : 0.0f));
+    out.f[4][off] = in.f[4][off] + ((zl ? txzm : 0.0f) - (zr ? tx0 : 0.0f)) +
+                    ((xl ? tz0 : 0.0f) - (xr ? tzxp : 0.0f));
+    out.f[3][off] = in.f[3][off] + ((zl ? tz0 : 0.0f) - (zr ? tzzp : 0.0f));
+    out.f[0][off] = own * avx[0];
+    out.f[1][off] = own * avz[0];
+    if (grad || rebuild) {
+      const float* txx = fw.f[2] + off;
+      const float* tzz = fw.f[3] + off;
+      const float* txz = fw.f[4] + off;
+      const float sx = (x < g.nx - 1 ? txx[1] - txx[0] : 0.0f) +
+                       (z < g.nz - 1 ? txz[g.nx] - txz[0] : 0.0f);
+      const float sz = (x > 0 ? txz[0] - txz[-1] : 0.0f) + (z > 0 ? tzz[0] - tzz[-g.nx] : 0.0f);
+      if (grad) gacc += A[i] * sgn * (sx * avx[0] + sz * avz[0]);
+      if (rebuild) {
+        // forward vx' = a*(Rc*sx + b*vx) with a = b = 1 on the interior
+        float* vx = fw.f[0] + off;
+        float* vz = fw.f[1] + off;
+        *vx = *vx - rc * sx;
+        *vz = *vz - rc * sz;
+      }
+    }
+  }
+  if (grad) gRc[(int64_t)grp * 4 * P + i] += gacc;
+}
+
+// Transpose of stage T (see k_el_adjV).  fw: the stage's input velocities
+// (for the coefficient gradients) and, with recon, the stresses to rebuild in
+// place on the interior (the reverse of T).  gradients: planes 0..2 of the
+// group's 4-plane block (lam+2mu, lam, Mc).
+template <int S>
+__global__ __launch_bounds__(256) void k_el_adjT(const float* __restrict__ cf, float sdti,
+                                                 float sgn, Region rg, CFields in, Fields out,
+                                                 Fields fw, float* __restrict__ gco, int gmode,
+                                                 int recon, int B, Grid g, TileMap tm) {
+  int z, x, grp;
+  if (!cell_of(tm, g, z, x, grp)) return;
+  const int i = z * g.nx + x;
+  const int64_t P = g.plane;
+  const float* A = cf + kAinv * P;
+  // per-neighbour coefficients of q1 = sdti*(l2m*u1 + lam*u2), q2 = sdti*(lam*u1
+  // + l2m*u2), q3 = s*Mc*u3 with u = a*cotangent inside the region, 0 outside
+  struct C3 { float a, l2m, lam, mc; };
+  auto co = [&](int zz, int xx) {
+    C3 r{0.0f, 0.0f, 0.0f, 0.0f};
+    if (in_region(zz, xx, rg)) {
+      const int j = zz * g.nx + xx;
+      r = C3{A[j], cf[kL2M * P + j], cf[kLam * P + j], cf[kMc * P + j]};
+    }
+    return r;
+  };
+  const bool reg = in_region(z, x, rg);
+  const C3 k0 = co(z, x);
+  const C3 kxp = x < g.nx - 1 ? co(z, x + 1) : C3{0, 0, 0, 0};
+  const C3 kzp = z < g.nz - 1 ? co(z + 1, x) : C3{0, 0, 0, 0};
+  const C3 kxm = x > 0 ? co(z, x - 1) : C3{0, 0, 0, 0};
+  const C3 kzm = z > 0 ? co(z - 1, x) : C3{0, 0, 0, 0};
+  const float bm = reg ? cf[kBm * P + i] : 1.0f;
+  const bool grad = gco && reg && (gmode == 1 || interior(z, x, g));
+  const bool rebuild = recon && interior(z, x, g);
+  const bool xl = x >= 1, xr = x <= g.nx - 2, zl = z >= 1, zr = z <= g.nz - 2;
+  float g_l2m = 0.0f, g_lam = 0.0f, g_mc = 0.0f;
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int64_t off = (int64_t)(b0 + s) * P + i;
+    const float* axx = in.f[2] + off;
+    const float* azz = in.f[3] + off;
+    const float* axz = in.f[4] + off;
+    const float u1 = k0.a * axx[0], u2 = k0.a * azz[0], u3 = k0.a * axz[0];
+    const float q1 = sdti * (k0.l2m * u1 + k0.lam * u2);
+    const float q2 = sdti * (k0.lam * u1 + k0.l2m * u2);
+    const float q3 = sgn * k0.mc * u3;
+    // q1 at x+1, q3 at z+1 (Dx-^T, Dz-^T); q2 at z-1, q3 at x-1 (Dz+^T, Dx+^T)
+    float q1xp = 0.0f, q3zp = 0.0f, q2zm = 0.0f, q3xm = 0.0f;
+    if (xr) q1xp = sdti * (kxp.l2m * (kxp.a * axx[1]) + kxp.lam * (kxp.a * azz[1]));
+    if (zr) q3zp = sgn * kzp.mc * (kzp.a * axz[g.nx]);
+    if (zl) q2zm = sdti * (kzm.lam * (kzm.a * axx[-g.nx]) + kzm.l2m * (kzm.a * azz[-g.nx]));
+    if (xl) q3xm = sgn * kxm.mc * (kxm.a * axz[-1]);
+    out.f[0][off] = in.f[0][off] + ((xl ? q1 : 0.0f) - q1xp) + ((zl ? q3 : 0.0f) - q3zp);
+    out.f[1][off] = in.f[1][off] + (q2zm - (zr ? q2 : 0.0f)) + (q3xm - (xr ? q3 : 0.0f));
+    if (reg) {
+      out.f[2][off] = bm * u1;
+      out.f[3][off] = bm * u2;
+      out.f[4][off] = bm * u3;
+    } else {
+      out.f[2][off] = axx[0];
+      out.f[3][off] = azz[0];
+      out.f[4][off] = axz[0];
+    }
+    if (grad || rebuild) {
+      const float* vx = fw.f[0] + off;
+      const float* vz = fw.f[1] + off;
+      const float vx_x = x > 0 ? vx[0] - vx[-1] : 0.0f;
+      const float vz_z = z < g.nz - 1 ? vz[g.nx] - vz[0] : 0.0f;
+      const float vx_z = z > 0 ? vx[0] - vx[-g.nx] : 0.0f;
+      const float vz_x = x < g.nx - 1 ? vz[1] - vz[0] : 0.0f;
+      if (grad) {
+        g_l2m += sdti * (vx_x * u1 + vz_z * u2);
+        g_lam += sdti * (vz_z * u1 + vx_x * u2);
+        g_mc += sgn * (vz_x + vx_z) * u3;
+      }
+      if (rebuild) {
+        // forward txx' = a*(dti*(...) + b*txx) with a = b = 1 on the interior
+        float* txx = fw.f[2] + off;
+        float* tzz = fw.f[3] + off;
+        float* txz = fw.f[4] + off;
+        *txx = *txx - sdti * (k0.l2m * vx_x + k0.lam * vz_z);
+        *tzz = *tzz - sdti * (k0.l2m * vz_z + k0.lam * vx_x);
+        *txz = *txz - sgn * k0.mc * (vz_x + vx_z);
+      }
+    }
+  }
+  if (grad) {
+    float* G = gco + (int64_t)grp * 4 * P + i;
+    G[0] += g_l2m;
+    G[P] += g_lam;
+    G[2 * P] += g_mc;
+  }
+}
+
+// Coefficient gradients (summed over the G group blocks) -> dJ/dvp, dvs, drho.
+__global__ __launch_bounds__(256) void k_el_chain(const float* __restrict__ gco, int G,
+                                                  const float* __restrict__ vp,
+                                                  const float* __restrict__ vs,
+                                                  const float* __restrict__ rho, float dti,
+                                                  float* __restrict__ gvp, float* __restrict__ gvs,
+                                                  float* __restrict__ grho, int64_t plane) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= plane) return;
+  float a = 0.0f, l = 0.0f, m = 0.0f, r = 0.0f;
+  for (int q = 0; q < G; ++q) {
+    const float* b = gco + (int64_t)q * 4 * plane + i;
+    a += b[0];
+    l += b[plane];
+    m += b[2 * plane];
+    r += b[3 * plane];
+  }
+  const float p = vp[i], s = vs[i], rh = rho[i];
+  // lam+2mu = rho vp^2, lam = rho (vp^2 - 2 vs^2), Mc = dt/h rho vs^2, Rc = dt/(h rho)
+  gvp[i] = 2.0f * rh * p * (a + l);
+  gvs[i] = -4.0f * rh * s * l + 2.0f * rh * s * dti * m;
+  grho[i] = p * p * a + (p * p - 2.0f * s * s) * l + s * s * dti * m - dti / (rh * rh) * r;
+}
+
+// After stage V of forward step n: add w[n] to the source fields, then (a
+// separate launch) record receivers and store the rings of all five fields.
+__global__ __launch_bounds__(256) void k_el_source(Fields F, const int32_t* __restrict__ src,
+                                                   const float* __restrict__ wn, float sign,
+                                                   int src_mask, int B, Grid g) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int64_t i = (int64_t)b * g.plane + (int64_t)src[2 * b] * g.nx + src[2 * b + 1];
+  const float w = sign * *wn;
+  for (int f = 0; f < 5; ++f)
+    if (src_mask & (1 << f)) F.f[f][i] += w;
+}
+
+// Receivers: rec_flat[r] (into (B, nz, nx)); channel c reads field rec_field[c].
+__global__ __launch_bounds__(256) void k_el_record(CFields F, const int64_t* __restrict__ rec,
+                                                   int64_t R, const int32_t* __restrict__ rfield,
+                                                   int nch, float* __restrict__ rec_row,
+                                                   float* __restrict__ ring_slot,
+                                                   int64_t ring_len, int B, Grid g) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R * nch) {
+    const int64_t r = t / nch;
+    const int c = (int)(t - r * nch);
+    rec_row[t] = F.f[rfield[c]][rec[r]];
+    return;
+  }
+  t -= R * nch;
+  if (ring_slot && t < 5 * (int64_t)B * ring_len) {
+    const int64_t per = (int64_t)B * ring_len;
+    const int f = (int)(t / per);
+    const int64_t rem = t - f * per;
+    const int64_t b = rem / ring_len;
+    const int64_t j = rem - b * ring_len;
+    ring_slot[t] = F.f[f][b * g.plane + seis::ring_cell(j, g)];
+  }
+}
+
+// Restore the rings of the fields in `mask` from a slot (5 x B x ring_len).
+__global__ __launch_bounds__(256) void k_el_ring_restore(Fields F, const float* __restrict__ slot,
+                                                         int mask, int64_t ring_len, int B,
+                                                         Grid g) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)B * ring_len;
+  if (t >= 5 * per) return;
+  const int f = (int)(t / per);
+  if (!(mask & (1 << f))) return;
+  const int64_t rem = t - f * per;
+  const int64_t b = rem / ring_len;
+  const int64_t j = rem - b * ring_len;
+  F.f[f][b * g.plane + seis::ring_cell(j, g)] = slot ? slot[t] : 0.0f;
+}
+
+// Adjoint source: cotangents of the records of step n into the field adjoints.
+__global__ __launch_bounds__(256) void k_el_inject(Fields adj, const int64_t* __restrict__ rec,
+                                                   int64_t R, const int32_t* __restrict__ rfield,
+                                                   int nch, const float* __restrict__ rgrad) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R * nch) return;
+  const int64_t r = t / nch;
+  const int c = (int)(t - r * nch);
+  atomicAdd(adj.f[rfield[c]] + rec[r], rgrad[t]);
+}
+
+__global__ __launch_bounds__(256) void k_el_fill(float* __restrict__ p, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_el_copy(const float* __restrict__ a,
+                                                 float* __restrict__ b, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+inline int zero(float* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return 0;
+  k_el_fill<<<blocks_for(n), 256, 0, s>>>(p, n);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+inline bool valid_s(int S) { return S == 1 || S == 2 || S == 4 || S == 8; }
+
+#define SEIS_EL_LAUNCH(K, S, grid, s, ...)                                  \
+  do {                                                                      \
+    switch (S) {                                                            \
+      case 1: K<1><<<grid, 256, 0, s>>>(__VA_ARGS__); break;               \
+      case 2: K<2><<<grid, 256, 0, s>>>(__VA_ARGS__); break;               \
+      case 4: K<4><<<grid, 256, 0, s>>>(__VA_ARGS__); break;               \
+      default: K<8><<<grid, 256, 0, s>>>(__VA_ARGS__); break;              \
+    }                                                                       \
+    SEIS_CHECK_LAUNCH();                                                    \
+  } while (0)
+
+int check(int64_t B, int64_t nz, int64_t nx, float dt, float h, int S) {
+  SEIS_CHECK_ARG(B >= 1 && B <= 65535, "batch size %lld outside [1, 65535]", (long long)B);
+  SEIS_CHECK_ARG(nz >= 2 && nx >= 2, "grid %lld x %lld too small", (long long)nz, (long long)nx);
+  SEIS_CHECK_ARG(nz * nx < (int64_t)1 << 31, "grid too large");
+  SEIS_CHECK_ARG(dt > 0.0f && h > 0.0f, "dt and h must be positive");
+  SEIS_CHECK_ARG(valid_s(S), "shots_per_group %d not in {1, 2, 4, 8}", S);
+  return 0;
+}
+
+Fields fields_of(float* const* p) { return Fields{{p[0], p[1], p[2], p[3], p[4]}}; }
+CFields cfields_of(float* const* p) { return CFields{{p[0], p[1], p[2], p[3], p[4]}}; }
+
+}  // namespace
+
+extern "C" {
+
+int seis_elastic_coefficients(const float* vp, const float* vs, const float* rho, const float* d,
+                              float* coef, int64_t nz, int64_t nx, float dt, float h,
+                              void* stream) {
+  SEIS_CHECK_ARG(vp && vs && rho && d && coef, "null pointer argument");
+  SEIS_CHECK_ARG(nz >= 1 && nx >= 1 && dt > 0.0f && h > 0.0f, "bad grid or time step");
+  const int64_t plane = nz * nx;
+  k_el_coef<<<blocks_for(plane), 256, 0, (hipStream_t)stream>>>(vp, vs, rho, d, coef, dt,
+                                                                1.0f / h, plane);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+int seis_elastic_stage(int32_t stage, const float* coef, float* const* fields, float sign,
+                       int32_t z0, int32_t z1, int32_t x0, int32_t x1, int64_t B, int64_t nz,
+                       int64_t nx, float dt, float h, int32_t shots_per_group, void* stream) {
+  if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
+  SEIS_CHECK_ARG(coef && fields, "null pointer argument");
+  for (int f = 0; f < 5; ++f) SEIS_CHECK_ARG(fields[f], "null field pointer");
+  SEIS_CHECK_ARG(stage == 0 || stage == 1, "stage must be 0 (stresses) or 1 (velocities)");
+  const Grid g{(int)nz, (int)nx, 0, nz * nx};
+  const int S = shots_per_group;
+  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  const Region rg{z0, z1, x0, x1};
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid(tm.per * kXcds);
+  const float dti = dt * (1.0f / h);
+  if (stage == 0)
+    SEIS_EL_LAUNCH(k_el_T, S, grid, s, coef, sign * dti, sign, rg, fields_of(fields), (int)B, g,
+                   tm);
+  else
+    SEIS_EL_LAUNCH(k_el_V, S, grid, s, coef, sign, rg, fields_of(fields), (int)B, g, tm);
+  return 0;
+}
+
+int seis_elastic_stage_vjp(int32_t stage, const float* coef, float* const* cot_in,
+                           float* const* cot_out, float* const* fwd, float* grad_coef,
+                           float sign, int32_t z0, int32_t z1, int32_t x0, int32_t x1, int64_t B,
+                           int64_t nz, int64_t nx, float dt, float h, int32_t shots_per_group,
+                           void* stream) {
+  if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
+  SEIS_CHECK_ARG(coef && cot_in && cot_out && fwd, "null pointer argument");
+  for (int f = 0; f < 5; ++f)
+    SEIS_CHECK_ARG(cot_in[f] && cot_out[f] && fwd[f], "null field pointer");
+  SEIS_CHECK_ARG(stage == 0 || stage == 1, "stage must be 0 (stresses) or 1 (velocities)");
+  const Grid g{(int)nz, (int)nx, 0, nz * nx};
+  const int S = shots_per_group;
+  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  const Region rg{z0, z1, x0, x1};
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid(tm.per * kXcds);
+  const float dti = dt * (1.0f / h);
+  const int gmode = grad_coef ? 1 : 0;
+  if (stage == 0)
+    SEIS_EL_LAUNCH(k_el_adjT, S, grid, s, coef, sign * dti, sign, rg, cfields_of(cot_in),
+                   fields_of(cot_out), fields_of(fwd), grad_coef, gmode, 0, (int)B, g, tm);
+  else
+    SEIS_EL_LAUNCH(k_el_adjV, S, grid, s, coef, sign, rg, cfields_of(cot_in), fields_of(cot_out),
+                   fields_of(fwd), grad_coef ? grad_coef + 3 * g.plane : nullptr, gmode, 0,
+                   (int)B, g, tm);
+  return 0;
+}
+
+int seis_elastic_chain(const float* grad_coef, int32_t groups, const float* vp, const float* vs,
+                       const float* rho, float* gvp, float* gvs, float* grho, int64_t nz,
+                       int64_t nx, float dt, float h, void* stream) {
+  SEIS_CHECK_ARG(grad_coef && vp && vs && rho && gvp && gvs && grho, "null pointer argument");
+  SEIS_CHECK_ARG(groups >= 1, "groups must be >= 1");
+  const int64_t plane = nz * nx;
+  k_el_chain<<<blocks_for(plane), 256, 0, (hipStream_t)stream>>>(
+      grad_coef, groups, vp, vs, rho, dt * (1.0f / h), gvp, gvs, grho, plane);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
+                         const int32_t* src_zx, int32_t src_mask, const int64_t* rec_flat,
+                         int64_t R, const int32_t* rec_field, int32_t nch, float* rec_out,
+                         float* ring, float* snaps, int32_t snap_every, float* const* fields,
+                         int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                         int32_t shots_per_group, void* stream) {
+  if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
+  SEIS_CHECK_ARG(coef && wavelet && src_zx && fields, "null pointer argument");
+  for (int f = 0; f < 5; ++f) SEIS_CHECK_ARG(fields[f], "null field pointer");
+  SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
+  SEIS_CHECK_ARG(R >= 0 && nch >= 1 && (R == 0 || (rec_flat && rec_field && rec_out)),
+                 "bad receiver arguments");
+  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml) : 0;
+  SEIS_CHECK_ARG(!ring || ring_len > 0, "grid has no interior inside npml=%d", npml);
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  const Grid g{(int)nz, (int)nx, npml, nz * nx};
+  const int S = shots_per_group;
+  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  const Region all{0, (int)nz, 0, (int)nx};
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid(tm.per * kXcds);
+  const float dti = dt * (1.0f / h);
+  const int64_t field = B * g.plane;
+  Fields F = fields_of(fields);
+  for (int f = 0; f < 5; ++f)
+    if (int e = zero(F.f[f], field, s)) return e;
+  for (int64_t n = 0; n < nt; ++n) {
+    SEIS_PROF(SEIS_KERNEL_ELASTIC_FWD, s);
+    SEIS_EL_LAUNCH(k_el_T, S, grid, s, coef, dti, 1.0f, all, F, (int)B, g, tm);
+    SEIS_EL_LAUNCH(k_el_V, S, grid, s, coef, 1.0f, all, F, (int)B, g, tm);
+    SEIS_PROF(SEIS_KERNEL_ELASTIC_FWD, s);
+    k_el_source<<<blocks_for(B), 256, 0, s>>>(F, src_zx, wavelet + n, 1.0f, src_mask, (int)B, g);
+    SEIS_CHECK_LAUNCH();
+    float* slot = (ring && n + 1 < nt) ? ring + n * 5 * B * ring_len : nullptr;
+    const int64_t work = R * nch + (slot ? 5 * B * ring_len : 0);
+    if (work > 0) {
+      k_el_record<<<blocks_for(work), 256, 0, s>>>(CFields{{F.f[0], F.f[1], F.f[2], F.f[3], F.f[4]}},
+                                                   rec_flat, R, rec_field, nch,
+                                                   R ? rec_out + n * R * nch : nullptr, slot,
+                                                   ring_len, (int)B, g);
+      SEIS_CHECK_LAUNCH();
+    }
+    // snapshot of state n+1 at n+1 = j*snap_every, slot j-1
+    const int64_t m = n + 1;
+    if (snaps && m % snap_every == 0 && m <= nt - 1) {
+      float* sp = snaps + (m / snap_every - 1) * 5 * field;
+      for (int f = 0; f < 5; ++f) {
+        k_el_copy<<<blocks_for(field), 256, 0, s>>>(F.f[f], sp + f * field, field);
+        SEIS_CHECK_LAUNCH();
+      }
+    }
+  }
+  return 0;
+}
+
+int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
+                          const int32_t* src_zx, int32_t src_mask, const int64_t* rec_flat,
+                          int64_t R, const int32_t* rec_field, int32_t nch,
+                          const float* rec_grad, const float* ring, const float* snaps,
+                          int32_t snap_every, float* const* fields, float* const* adj_a,
+                          float* const* adj_b, float* grad_coef, int64_t B, int64_t nz,
+                          int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
+                          void* stream) {
+  if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
+  SEIS_CHECK_ARG(coef && wavelet && src_zx && fields && adj_a && adj_b && grad_coef,
+                 "null pointer argument");
+  for (int f = 0; f < 5; ++f)
+    SEIS_CHECK_ARG(fields[f] && adj_a[f] && adj_b[f], "null field pointer");
+  SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
+  SEIS_CHECK_ARG(R >= 0 && nch >= 1 && (R == 0 || (rec_flat && rec_field && rec_grad)),
+                 "bad receiver arguments");
+  const int64_t ring_len = seis_ring_len(nz, nx, npml);
+  SEIS_CHECK_ARG(ring_len > 0, "grid has no interior inside npml=%d", npml);
+  SEIS_CHECK_ARG(ring || nt <= 1, "ring storage is required for nt > 1");
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  const Grid g{(int)nz, (int)nx, npml, nz * nx};
+  const int S = shots_per_group;
+  const int G = (int)((B + S - 1) / S);
+  const TileMap tm = make_tiles(g, 1, 1, G);
+  const Region all{0, (int)nz, 0, (int)nx};
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid(tm.per * kXcds);
+  const float dti = dt * (1.0f / h);
+  const int64_t field = B * g.plane;
+  Fields F = fields_of(fields);
+  Fields X = fields_of(adj_a), Y = fields_of(adj_b);
+  for (int f = 0; f < 5; ++f) {
+    if (int e = zero(X.f[f], field, s)) return e;
+    if (int e = zero(Y.f[f], field, s)) return e;
+  }
+  if (int e = zero(grad_coef, (int64_t)G * 4 * g.plane, s)) return e;
+  // state nt: remove the last source sample, inject the last records' cotangent
+  k_el_source<<<blocks_for(B), 256, 0, s>>>(F, src_zx, wavelet + (nt - 1), -1.0f, src_mask,
+                                            (int)B, g);
+  SEIS_CHECK_LAUNCH();
+  if (R > 0) {
+    k_el_inject<<<blocks_for(R * nch), 256, 0, s>>>(X, rec_flat, R, rec_field, nch,
+                                                    rec_grad + (nt - 1) * R * nch);
+    SEIS_CHECK_LAUNCH();
+  }
+  const int64_t rwork = 5 * B * ring_len;
+  for (int64_t n = nt - 1; n >= 0; --n) {
+    // reverse of forward step n: fields hold state n+1 (stresses pre-source);
+    // X holds the cotangent of state n+1.  recon rebuilds state n.
+    const int recon = n >= 1 ? 1 : 0;
+    SEIS_PROF(SEIS_KERNEL_ELASTIC_BWD, s);
+    SEIS_EL_LAUNCH(k_el_adjV, S, grid, s, coef, 1.0f, all,
+                   CFields{{X.f[0], X.f[1], X.f[2], X.f[3], X.f[4]}}, Y, F,
+                   grad_coef + 3 * g.plane, 2, recon, (int)B, g, tm);
+    if (n == 0) {  // state 0 is zero: the stage-T gradient sees zero velocities
+      if (int e = zero(F.f[0], field, s)) return e;
+      if (int e = zero(F.f[1], field, s)) return e;
+    }
+    if (recon) {
+      k_el_ring_restore<<<blocks_for(rwork), 256, 0, s>>>(F, ring + (n - 1) * 5 * B * ring_len,
+                                                         0x3, ring_len, (int)B, g);
+      SEIS_CHECK_LAUNCH();
+    }
+    SEIS_EL_LAUNCH(k_el_adjT, S, grid, s, coef, dti, 1.0f, all,
+                   CFields{{Y.f[0], Y.f[1], Y.f[2], Y.f[3], Y.f[4]}}, X, F, grad_coef, 2, recon,
+                   (int)B, g, tm);
+    SEIS_PROF(SEIS_KERNEL_ELASTIC_BWD, s);
+    if (n >= 1) {
+      k_el_ring_restore<<<blocks_for(rwork), 256, 0, s>>>(F, ring + (n - 1) * 5 * B * ring_len,
+                                                         0x1c, ring_len, (int)B, g);
+      SEIS_CHECK_LAUNCH();
+      // reload the stored state n at snapshot times (bounds FP32 drift)
+      if (snaps && n % snap_every == 0 && n <= nt - 1) {
+        const float* sp = snaps + (n / snap_every - 1) * 5 * field;
+        for (int f = 0; f < 5; ++f) {
+          k_el_copy<<<blocks_for(field), 256, 0, s>>>(sp + f * field, F.f[f], field);
+          SEIS_CHECK_LAUNCH();
+        }
+      }
+      // state n stresses back to pre-source of step n-1, cotangent of records n-1
+      k_el_source<<<blocks_for(B), 256, 0, s>>>(F, src_zx, wavelet + (n - 1), -1.0f, src_mask,
+                                                (int)B, g);
+      SEIS_CHECK_LAUNCH();
+      if (R > 0) {
+        k_el_inject<<<blocks_for(R * nch), 256, 0, s>>>(X, rec_flat, R, rec_field, nch,
+                                                        rec_grad + (n - 1) * R * nch);
+        SEIS_CHECK_LAUNCH();
+      }
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"
