@@ -22,7 +22,10 @@ import numpy as np
 
 METRIC = "Gcell-updates/s fwd+adjoint, % HBM roofline; wall-clock to target misfit"
 HBM_PEAK = 8.0e12  # B/s, MI355X
-BYTES_PER_CELL = {"k_fwd": 16.0, "k_bwd": 36.0}  # algorithmic bytes per cell-update
+# algorithmic bytes per cell-update of each timed kernel (fields touched once
+# per step, damping/coefficients read once per cell)
+BYTES_PER_CELL = {"k_fwd": 16.0, "k_bwd": 36.0, "k_el_fwd": 52.0, "k_el_bwd": 116.0}
+KERNEL_IDS = {"acoustic": (("k_fwd", 0), ("k_bwd", 1)), "elastic": (("k_el_fwd", 2), ("k_el_bwd", 3))}
 
 
 def parse():
@@ -35,6 +38,9 @@ def parse():
                    help="time steps of the bounded CPU sample (1 shot, full grid)")
     p.add_argument("--nt", type=int, default=None,
                    help="shorter time axis for profiler runs (the metric is quoted at full nt)")
+    p.add_argument("--workload", choices=["acoustic", "elastic"], default="acoustic",
+                   help="acoustic: 64-shot 800x2400 FWI gradient (the headline); elastic: "
+                        "32-shot 600x1800 vp/vs/rho gradient")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a plain 1 GiB device copy (HBM-counter calibration)")
     return p.parse_args()
@@ -62,6 +68,43 @@ def build_problem(dev, seed=20230503, nt_override=None):
     P["src_np"], P["rec_np"] = src, rec
     P["vp_init_np"], P["d_np"], P["w_np"] = vi, P["d"].cpu().numpy(), P["w"].cpu().numpy()
     return P
+
+
+def build_elastic_problem(dev, seed=20230503, nt_override=None):
+    import torch
+    from seistorch_amd import synthetic
+    from seistorch_amd.pml import generate_pml_coefficients_2d
+    from seistorch_amd.utils import pad_model, ricker_wave
+    nz_i, nx_i, nshots, nt, dt, h, fm = 600, 1800, 32, 6000, 0.5e-3, 5.0, 10.0
+    nt = nt_override or nt
+    vp, vs, rho = synthetic.elastic_models(nz_i, nx_i, seed=seed)
+    vp0, vs0, rho0 = synthetic.elastic_models(nz_i, nx_i, pert=0.0, seed=seed)
+    src, rec = synthetic.surface_acquisition(nx_i, nshots, src_depth=2)
+    P = dict(nz=nz_i + 100, nx=nx_i + 100, B=nshots, nt=nt, dt=dt, h=h, fm=fm, nz_i=nz_i,
+             nx_i=nx_i, elastic=True)
+    for k, a in (("vp_true", vp), ("vs_true", vs), ("rho_true", rho), ("vp_init", vp0),
+                 ("vs_init", vs0), ("rho_init", rho0)):
+        P[k] = torch.from_numpy(pad_model(a).astype(np.float32)).to(dev)
+    P["d"] = generate_pml_coefficients_2d((P["nz"], P["nx"]), 50).to(dev)
+    P["w"] = ricker_wave(fm, dt, nt, delay=160).to(dev)
+    P["src"] = torch.from_numpy(src.astype(np.int32)).to(dev)
+    P["rec"] = torch.from_numpy(synthetic.rec_flat_index(rec, P["nz"], P["nx"])).to(dev)
+    return P
+
+
+def elastic_records(P, vp, vs, rho):
+    from seistorch_amd.equations2d import elastic as el
+    return el.propagate(vp, vs, rho, P["d"], P["w"], P["src"], P["rec"], P["B"], P["dt"],
+                        P["h"], ("txx", "tzz"), ("vx", "vz"))
+
+
+def elastic_gradient_step(P):
+    import torch
+    m = [P[k].detach().requires_grad_(True) for k in ("vp_init", "vs_init", "rho_init")]
+    rec = elastic_records(P, *m)
+    loss = torch.sum((rec - P["obs"]) ** 2)
+    loss.backward()
+    return loss.detach(), m[0].grad
 
 
 def gradient_step(P):
@@ -105,20 +148,30 @@ def main():
     import torch
     from seistorch_amd import _lib
     dev = torch.device("cuda:0")
-    P = build_problem(dev, nt_override=args.nt)
-    with torch.no_grad():
-        from seistorch_amd.equations2d import acoustic as ac
-        P["obs"] = ac.propagate(P["vp_true"], P["d"], P["w"], P["src"], P["rec"], P["B"],
-                                P["dt"], P["h"])
+    elastic = args.workload == "elastic"
+    if elastic:
+        P = build_elastic_problem(dev, nt_override=args.nt)
+        with torch.no_grad():
+            P["obs"] = elastic_records(P, P["vp_true"], P["vs_true"], P["rho_true"])
+        step_fn = elastic_gradient_step
+    else:
+        P = build_problem(dev, nt_override=args.nt)
+        with torch.no_grad():
+            from seistorch_amd.equations2d import acoustic as ac
+            P["obs"] = ac.propagate(P["vp_true"], P["d"], P["w"], P["src"], P["rec"], P["B"],
+                                    P["dt"], P["h"])
+        step_fn = gradient_step
     torch.cuda.synchronize()
     print(f"[bench] problem ready: {P['B']} shots, {P['nz']}x{P['nx']}, nt={P['nt']}",
           file=sys.stderr, flush=True)
-    from seistorch_amd import autotune
-    shape, tune = autotune.tune_acoustic(P["vp_init"], P["d"], P["w"], P["src"], P["rec"],
-                                         P["B"], P["dt"], P["h"])
-    print(f"[bench] autotuned launch shape {shape}", file=sys.stderr, flush=True)
+    shape, tune = None, {}
+    if not elastic:
+        from seistorch_amd import autotune
+        shape, tune = autotune.tune_acoustic(P["vp_init"], P["d"], P["w"], P["src"], P["rec"],
+                                             P["B"], P["dt"], P["h"])
+        print(f"[bench] autotuned launch shape {shape}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
-        gradient_step(P)
+        step_fn(P)
         torch.cuda.synchronize()
         print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
     lib = _lib.load()
@@ -126,14 +179,14 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        loss, grad = gradient_step(P)
+        loss, grad = step_fn(P)
         print(f"[bench] step {i + 1}/{args.steps} enqueued", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     lib.seis_profile_enable(0)
     kern = {}
     import ctypes
-    for name, kid in (("k_fwd", 0), ("k_bwd", 1)):
+    for name, kid in KERNEL_IDS[args.workload]:
         ms = ctypes.c_double(0.0)
         cnt = ctypes.c_int64(0)
         _lib.call("seis_profile_collect", kid, ctypes.byref(ms), ctypes.byref(cnt))
@@ -154,7 +207,9 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded anomaly model, records from the true model)",
-        "config": {"workload": "acoustic FWI gradient, PML 50, fwd + boundary-saving adjoint",
+        "config": {"workload": ("elastic vp/vs/rho gradient, PML 50, fwd + boundary-saving "
+                                "adjoint" if elastic else
+                                "acoustic FWI gradient, PML 50, fwd + boundary-saving adjoint"),
                    "grid": [P["nz_i"], P["nx_i"]], "padded_grid": [P["nz"], P["nx"]],
                    "shots": P["B"], "nt": P["nt"], "dt": P["dt"], "dx": P["h"],
                    "receivers_per_shot": P["nx_i"], "f_peak_hz": P["fm"]},
@@ -164,10 +219,9 @@ def main():
                      "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch,
                      "bytes_per_cell_update": BYTES_PER_CELL[dom]},
         "kernels_ms_total": {k: round(v[0], 2) for k, v in kern.items()},
-        "launch_shape": {"shots_per_group": shape[0], "rows_per_thread": shape[1],
-                         "vec_x": shape[2],
-                         "autotune_ms": {f"S{a}_rz{b}_vx{c}": round(v, 2)
-                                         for (a, b, c), v in tune.items()}},
+        "launch_shape": None if shape is None else {
+            "shots_per_group": shape[0], "rows_per_thread": shape[1], "vec_x": shape[2],
+            "autotune_ms": {f"S{a}_rz{b}_vx{c}": round(v, 2) for (a, b, c), v in tune.items()}},
         "loss": float(loss),
     }
     if args.copy_calibration:
@@ -179,7 +233,7 @@ def main():
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = 4 * n
         del a, b
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not elastic:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -35,6 +35,8 @@ int32_t seis_abi_version(void);
 #define SEIS_KERNEL_ACOUSTIC_BWD 1
 #define SEIS_KERNEL_ELASTIC_FWD 2  /* stress + velocity stages of one step */
 #define SEIS_KERNEL_ELASTIC_BWD 3  /* both adjoint stages of one step */
+#define SEIS_KERNEL_HABC_FWD 4
+#define SEIS_KERNEL_HABC_BWD 5
 #define SEIS_PROFILE_KERNELS 8
 int seis_profile_enable(int32_t on);
 int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
@@ -107,6 +109,45 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            int64_t nz, int64_t nx, float dt, float h, int32_t npml,
                            int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                            void* stream);
+
+/* ---------------- acoustic, constant density, HABC ----------------------- */
+/* b is the (nz, nx) HABC weight field (seistorch/habc.py:252-291); the strips
+ * are 50 cells wide; multiple != 0 selects the free-surface variant (no top
+ * strip).  Grids need at least 102 cells per axis. */
+
+/* One HABC step y = HABC(h1, h2) (acoustic_habc.py:206-221 with habc()
+ * :147-204, including the torch.roll wrap of the 51-row strips).  y must not
+ * alias h1 or h2. */
+int seis_habc_step(const float* vp, const float* b, const float* h1, const float* h2, float* y,
+                   int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t multiple,
+                   void* stream);
+
+/* VJP of seis_habc_step: gh1 = (dy/dh1)^T gy, gh2 = (dy/dh2)^T gy and, if gvp
+ * != NULL, the shot-summed (dy/dvp)^T gy over every cell (replaces autograd
+ * through acoustic_habc.py:206-221). */
+int seis_habc_step_vjp(const float* vp, const float* b, const float* h1, const float* h2,
+                       const float* gy, float* gh1, float* gh2, float* gvp, int64_t B, int64_t nz,
+                       int64_t nx, float dt, float h, int32_t multiple, void* stream);
+
+/* Whole HABC forward loop, as seis_acoustic_forward (same records, ring and
+ * snapshot layouts) with three field buffers: on return u_a holds u^{nt},
+ * u_b u^{nt-1}, u_c is scratch. */
+int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int64_t nt,
+                      const int32_t* src_zx, const int64_t* rec_flat, int64_t R, float* rec_out,
+                      float* ring, float* snaps, int32_t snap_every, float* u_a, float* u_b,
+                      float* u_c, int64_t B, int64_t nz, int64_t nx, float dt, float h,
+                      int32_t npml, int32_t multiple, int32_t shots_per_group, void* stream);
+
+/* Adjoint of seis_habc_forward with boundary-saving reconstruction
+ * (acoustic_habc.py:223-268 + checkpoint_new.py, exact pairing): three
+ * cotangent buffers l_a, l_b, l_c; grad gets dJ/dvp on the interior. */
+int seis_habc_backward(const float* vp, const float* b, const float* wavelet, int64_t nt,
+                       const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
+                       const float* rec_grad, const float* ring, const float* snaps,
+                       int32_t snap_every, float* u_a, float* u_b, float* l_a, float* l_b,
+                       float* l_c, float* grad_part, float* grad, int64_t B, int64_t nz,
+                       int64_t nx, float dt, float h, int32_t npml, int32_t multiple,
+                       int32_t shots_per_group, void* stream);
 
 /* ---------------- elastic, isotropic velocity-stress, PML ---------------- */
 /* Fields are passed as an array of five (B, nz, nx) pointers in the order

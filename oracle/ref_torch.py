@@ -244,3 +244,98 @@ def elastic_gradient_ad(vp, vs, rho, d, dt, h, wavelet, src, rec, obs, src_type=
         J = torch.zeros(())
         torch.cat(syn, 1).backward(torch.as_tensor(rec_cotangent).to(dtype))
     return J.detach(), tuple(p.grad.detach() for p in ps), [s.detach() for s in syn]
+
+
+# --------------------------------------------------------------------- HABC
+
+def habc_masks(nz, nx, w=50, multiple=False):
+    """Trapezoid strip masks (seistorch/habc.py:4-40): top (w, nx), bottom (w, nx),
+    left (nz, w), right (nz, w); top is None with a free surface."""
+    i = torch.arange(w)[:, None]
+    j = torch.arange(nx)[None, :]
+    top = (j >= i) & (j <= nx - 1 - i)
+    bottom = torch.flip(top, [0])
+    r = torch.arange(nz)[:, None]
+    k = torch.arange(w)[None, :]
+    left = (k <= r) & (k <= nz - 1 - r)
+    right = torch.flip(left, [1])
+    if multiple:
+        top = None
+        left = left.clone(); right = right.clone()
+        left[:w] = True
+        right[:w] = True
+    return top, bottom, left, right
+
+
+def _one_way_strips(y, h1, h2, c, b, dt, h, w):
+    """Strips stacked (2, B, w+1, n) with row 0 at the outer edge, one-way
+    extrapolation along the row axis with torch.roll (acoustic_habc.py:79-101)."""
+    lam = 2 * c * dt / h
+    mu = c ** 2 * dt ** 2 / (h ** 2)
+    t1 = (2 - lam - mu) * h1
+    t2 = (lam + 2 * mu) * torch.roll(h1, shifts=-1, dims=2)
+    t3 = -mu * torch.roll(h1, shifts=-2, dims=2)
+    t4 = (lam - 1) * h2
+    t5 = -lam * torch.roll(h2, shifts=-1, dims=2)
+    u_one = t1 + t2 + t3 + t4 + t5
+    return (u_one * b + (1 - b) * y)[:, :, :w, :]
+
+
+def habc_step(c, h1, h2, dt, h, b, w=50, multiple=False):
+    """FP32 restatement of acoustic_habc._time_step (:206-221) with habc() (:147-204)."""
+    y = torch.mul((dt ** -2).pow(-1), (2 / dt ** 2 * h1 - torch.mul((dt ** -2), h2)
+                                       + torch.mul(c.pow(2), laplacian(h1, h))))
+    Bn, nz, nx = y.shape
+    fields = [y, h1, h2, c.unsqueeze(0).expand(Bn, nz, nx), b.unsqueeze(0).expand(Bn, nz, nx)]
+    # top / bottom strips: rows from the edge inwards
+    tb = [torch.stack([f[:, :w + 1, :], torch.flip(f, [1])[:, :w + 1, :]]) for f in fields]
+    # left / right strips: columns from the edge inwards, as rows
+    lr = [torch.stack([f.transpose(1, 2)[:, :w + 1, :],
+                       torch.flip(f.transpose(1, 2), [1])[:, :w + 1, :]]) for f in fields]
+    tbv = _one_way_strips(*tb, dt, h, w)
+    lrv = _one_way_strips(*lr, dt, h, w)
+    y_top = tbv[0]                                  # (B, w, nx), rows 0..w-1
+    y_bottom = torch.flip(tbv[1], [1])              # rows nz-w..nz-1
+    y_left = lrv[0].transpose(1, 2)                 # (B, nz, w), cols 0..w-1
+    y_right = torch.flip(lrv[1], [1]).transpose(1, 2)  # cols nx-w..nx-1
+    top, bottom, left, right = habc_masks(nz, nx, w, multiple)
+    y = y.clone()
+    if top is not None:
+        y[:, :w, :][:, top] = y_top[:, top]
+    y[:, -w:, :][:, bottom] = y_bottom[:, bottom]
+    y[:, :, :w][:, left] = y_left[:, left]
+    y[:, :, -w:][:, right] = y_right[:, right]
+    d = torch.arange(w)
+    if not multiple:
+        y[:, d, d] = 0.5 * y_top[:, d, d] + 0.5 * y_left[:, d, d]
+        y[:, d, nx - w + d] = 0.5 * y_top[:, d, nx - w + d] + 0.5 * y_right[:, d, d]
+    y[:, nz - w + d, d] = 0.5 * y_bottom[:, d, d] + 0.5 * y_left[:, nz - w + d, d]
+    y[:, nz - w + d, nx - w + d] = 0.5 * y_bottom[:, d, nx - w + d] + 0.5 * y_right[:, nz - w + d, d]
+    return y, h1
+
+
+def habc_forward(c, b, dt, h, wavelet, src, rec, dtype=torch.float32, c_param=None,
+                 multiple=False):
+    """Records of the HABC loop (rnn.py:178-212), list of (nt, nrec_b, 1)."""
+    c0, b, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(c, b, dt, h, wavelet, src, rec,
+                                                                dtype)
+    c = c0 if c_param is None else c_param
+    h1 = torch.zeros_like(smask)
+    h2 = torch.zeros_like(smask)
+    recs = []
+    for n in range(w.shape[0]):
+        h1, h2 = habc_step(c, h1, h2, dt_t, h_t, b, multiple=multiple)
+        y = h1.clone()
+        y += smask * w[n]
+        h1 = y
+        recs.append(h1[bidx, rz, rx])
+    stacked = torch.stack(recs, 0).unsqueeze(2)
+    return list(torch.split(stacked, counts, dim=1))
+
+
+def habc_gradient_ad(c, b, dt, h, wavelet, src, rec, rec_cotangent, dtype=torch.float32,
+                     multiple=False):
+    cp = torch.as_tensor(c).to(dtype).clone().requires_grad_(True)
+    syn = habc_forward(c, b, dt, h, wavelet, src, rec, dtype, c_param=cp, multiple=multiple)
+    torch.cat(syn, 1)[..., 0].backward(torch.as_tensor(rec_cotangent).to(dtype))
+    return cp.grad.detach(), [s.detach() for s in syn]

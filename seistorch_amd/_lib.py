@@ -36,6 +36,14 @@ SIGNATURES = {
     "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
                                               _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
                                               _i32, _i32, _i32, _i32, _p]),
+    "seis_habc_step": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32, _p]),
+    "seis_habc_step_vjp": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f,
+                                          _f, _i32, _p]),
+    "seis_habc_forward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32, _p,
+                                         _p, _p, _i64, _i64, _i64, _f, _f, _i32, _i32, _i32, _p]),
+    "seis_habc_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32, _p,
+                                          _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32,
+                                          _i32, _i32, _p]),
     "seis_elastic_coefficients": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _f, _f, _p]),
     "seis_elastic_stage": (ctypes.c_int, [_i32, _p, _p, _f, _i32, _i32, _i32, _i32, _i64, _i64,
                                           _i64, _f, _f, _i32, _p]),

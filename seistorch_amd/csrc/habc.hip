@@ -1,0 +1,575 @@
+// Acoustic HABC (hybrid absorbing boundary) kernels for gfx950.
+//
+// One step (seistorch/equations2d/acoustic_habc.py:206-221):
+//   y = (1/dt^-2)(2/dt^2 u - dt^-2 p + c^2 L u)   (u = u^n, p = u^{n-1})
+// then in each 50-cell boundary strip s (top, bottom, left, right) with
+// distance i from its outer edge and inward unit step e:
+//   u1_s = (2-l-m) u + (l+2m) u[+e] - m u[+2e] + (l-1) p - l p[+e]
+//   y_s  = b u1_s + (1-b) y,   l = 2c dt/h, m = c^2 dt^2/h^2   (:79-101)
+// The cell takes y, then y_top, y_bottom, y_left, y_right where that strip's
+// trapezoid mask holds (:167-183, masks of seistorch/habc.py:4-40), then the
+// four diagonals take 0.5 y_s1 + 0.5 y_s2 (:185-202).  The reference works on
+// 51-row strips shifted with torch.roll, so at strip row 49 the "u[+2e]" term
+// is the strip's outer-edge cell; that matters where the blend weight of a
+// row-49 cell is nonzero (the far diagonal ends) and is reproduced here.
+//
+// The adjoint is the exact transpose of that linear map in gather form: each
+// cell collects the undamped-stencil terms weighted by the blend's (1-b) and
+// the one-way terms from the up-to-three strip cells whose one-way stencil
+// reaches it.  The forward uses three rotating field buffers (the blend reads
+// u^{n-1} one cell inward, so it cannot overwrite it in place); the adjoint
+// three rotating cotangent buffers for the same reason.
+#include "common.h"
+#include "seistorch_amd.h"
+#include "tiles.h"
+
+using seis::Grid;
+using seis::lap_taps;
+using seis::acoustic_update;
+using seis::lap5;
+using namespace seis::tiles;
+
+namespace {
+
+constexpr int kW = 50;
+
+struct HabcCtx {
+  Grid g;
+  int multiple;
+  float dt, h;
+};
+
+__device__ __forceinline__ bool m_top(int z, int x, const HabcCtx& c) {
+  return !c.multiple && z < kW && x >= z && x <= c.g.nx - 1 - z;
+}
+__device__ __forceinline__ bool m_bottom(int z, int x, const HabcCtx& c) {
+  const int i = c.g.nz - 1 - z;
+  return z >= c.g.nz - kW && x >= i && x <= c.g.nx - 1 - i;
+}
+__device__ __forceinline__ bool m_left(int z, int x, const HabcCtx& c) {
+  return x < kW && ((z >= x && z <= c.g.nz - 1 - x) || (c.multiple && z < kW));
+}
+__device__ __forceinline__ bool m_right(int z, int x, const HabcCtx& c) {
+  const int i = c.g.nx - 1 - x;
+  return x >= c.g.nx - kW && ((z >= i && z <= c.g.nz - 1 - i) || (c.multiple && z < kW));
+}
+
+// strip s: 0 top, 1 bottom, 2 left, 3 right
+__device__ __forceinline__ int sdist(int s, int z, int x, const Grid& g) {
+  return s == 0 ? z : s == 1 ? g.nz - 1 - z : s == 2 ? x : g.nx - 1 - x;
+}
+__device__ __forceinline__ int sstep(int s, const Grid& g) {
+  return s == 0 ? g.nx : s == 1 ? -g.nx : s == 2 ? 1 : -1;
+}
+__device__ __forceinline__ int sedge(int s, int z, int x, const Grid& g) {
+  return s == 0 ? x : s == 1 ? (g.nz - 1) * g.nx + x : s == 2 ? z * g.nx : z * g.nx + g.nx - 1;
+}
+__device__ __forceinline__ bool near_edge(int z, int x, const Grid& g, int margin) {
+  return z < margin || z >= g.nz - margin || x < margin || x >= g.nx - margin;
+}
+
+struct OwCoef {
+  float k0, k1, k2, k3, k4;
+};
+
+// one-way coefficients with the reference's FP32 order: l = 2*c*dt/dh,
+// m = c**2*dt**2/(dh**2), (2-l-m), (l+2m), -m, (l-1), -l
+__device__ __forceinline__ OwCoef ow_coef(float c, const HabcCtx& x) {
+  const float lam = ((2.0f * c) * x.dt) / x.h;
+  const float mu = ((c * c) * (x.dt * x.dt)) / (x.h * x.h);
+  return OwCoef{(2.0f - lam) - mu, lam + 2.0f * mu, -mu, lam - 1.0f, -lam};
+}
+
+// blended strip value y_s at cell i (flat) for fields u, p
+__device__ __forceinline__ float strip_value(int s, int z, int x, int i, const float* u,
+                                             const float* p, const OwCoef& k, float b, float y,
+                                             const Grid& g) {
+  const int e = sstep(s, g);
+  const float u2 = sdist(s, z, x, g) == kW - 1 ? u[sedge(s, z, x, g)] : u[i + 2 * e];
+  const float t1 = k.k0 * u[i];
+  const float t2 = k.k1 * u[i + e];
+  const float t3 = k.k2 * u2;
+  const float t4 = k.k3 * p[i];
+  const float t5 = k.k4 * p[i + e];
+  const float one = (((t1 + t2) + t3) + t4) + t5;
+  return one * b + (1.0f - b) * y;
+}
+
+// Linear weights of the final assignment at a cell: value = alpha*y +
+// sum_s beta_s*u1_s.
+struct HabcW {
+  float alpha;
+  float beta[4];
+};
+
+__device__ __forceinline__ HabcW habc_weights(int z, int x, const HabcCtx& c, float b) {
+  HabcW w{1.0f, {0.0f, 0.0f, 0.0f, 0.0f}};
+  const Grid& g = c.g;
+  if (!near_edge(z, x, g, kW)) return w;
+  int last = -1;
+  if (m_top(z, x, c)) last = 0;
+  if (m_bottom(z, x, c)) last = 1;
+  if (m_left(z, x, c)) last = 2;
+  if (m_right(z, x, c)) last = 3;
+  int d1 = -1, d2 = -1;
+  if (!c.multiple && z < kW && x == z) { d1 = 0; d2 = 2; }
+  if (!c.multiple && z < kW && x == g.nx - kW + z) { d1 = 0; d2 = 3; }
+  if (z >= g.nz - kW && x == z - (g.nz - kW)) { d1 = 1; d2 = 2; }
+  if (z >= g.nz - kW && x == g.nx - kW + (z - (g.nz - kW))) { d1 = 1; d2 = 3; }
+  if (d1 >= 0) {
+    w.alpha = 0.5f * (1.0f - b) + 0.5f * (1.0f - b);
+    w.beta[d1] = 0.5f * b;
+    w.beta[d2] = 0.5f * b;
+  } else if (last >= 0) {
+    w.alpha = 1.0f - b;
+    w.beta[last] = b;
+  }
+  return w;
+}
+
+// Forward step for S shots per thread: out = HABC(u, p) (+ w at the source
+// cells when src != nullptr).  Cells are 64 x 4 tiles in XCD-aware order.
+template <int S>
+__global__ __launch_bounds__(256) void k_habc_fwd(const float* __restrict__ vp,
+                                                  const float* __restrict__ bw,
+                                                  const float* __restrict__ u,
+                                                  const float* __restrict__ p,
+                                                  float* __restrict__ out,
+                                                  const int32_t* __restrict__ src,
+                                                  const float* __restrict__ wn, int B,
+                                                  HabcCtx hc, seis::TimeConsts k, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x >= g.nx || z >= g.nz) return;
+  const int i = z * g.nx + x;
+  const float c = vp[i];
+  const float A0 = 1.0f / k.idt2;
+  const float B0 = k.idt2;
+  const bool edge = near_edge(z, x, g, kW);
+  const float b = edge ? bw[i] : 0.0f;
+  const OwCoef kc = ow_coef(c, hc);
+  const bool mt = m_top(z, x, hc), mb = m_bottom(z, x, hc), ml = m_left(z, x, hc),
+             mr = m_right(z, x, hc);
+  int d1 = -1, d2 = -1;
+  if (!hc.multiple && z < kW && x == z) { d1 = 0; d2 = 2; }
+  if (!hc.multiple && z < kW && x == g.nx - kW + z) { d1 = 0; d2 = 3; }
+  if (z >= g.nz - kW && x == z - (g.nz - kW)) { d1 = 1; d2 = 2; }
+  if (z >= g.nz - kW && x == g.nx - kW + (z - (g.nz - kW))) { d1 = 1; d2 = 3; }
+  const float w = src ? *wn : 0.0f;
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int bb = b0 + s;
+    const int64_t off = (int64_t)bb * g.plane;
+    const float* U = u + off;
+    const float* P = p + off;
+    const float L = lap5(U, z, x, i, g, k.ih2);
+    const float y = acoustic_update(A0, B0, k.two_idt2, U[i], P[i], c * c, L);
+    float v = y;
+    if (edge) {
+      if (mt) v = strip_value(0, z, x, i, U, P, kc, b, y, g);
+      if (mb) v = strip_value(1, z, x, i, U, P, kc, b, y, g);
+      if (ml) v = strip_value(2, z, x, i, U, P, kc, b, y, g);
+      if (mr) v = strip_value(3, z, x, i, U, P, kc, b, y, g);
+      if (d1 >= 0)
+        v = 0.5f * strip_value(d1, z, x, i, U, P, kc, b, y, g) +
+            0.5f * strip_value(d2, z, x, i, U, P, kc, b, y, g);
+    }
+    if (src && z == src[2 * bb] && x == src[2 * bb + 1]) v += w;
+    out[off + i] = v;
+  }
+}
+
+// Transpose of the step: out = H_u^T y1 + H_p^T y2 (either may be null).
+// With recon/grad (fused adjoint, y1 = lambda^{k+1}, y2 = lambda^{k+2}):
+// gradient 2c*A0*L(u^{k-1})*lambda^k on the interior (lambda^k without the
+// adjoint source, which k_habc_post adds with its gradient share) and the
+// interior rebuild of u^{k-2} over uc.  recon: 0 none, 1 rebuild, 2 zeros.
+template <int S>
+__global__ __launch_bounds__(256) void k_habc_adj(
+    const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
+    const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k,
+    TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x >= g.nx || z >= g.nz) return;
+  const int i = z * g.nx + x;
+  const float A0 = 1.0f / k.idt2;
+  const float B0 = k.idt2;
+  const float c = vp[i];
+  // undamped-stencil weights q(n) = c(n)^2*A0*alpha(n) at the cell and its neighbours
+  const bool deep = !near_edge(z, x, g, kW + 3);
+  auto qw = [&](int zz, int xx) {
+    const int j = zz * g.nx + xx;
+    const float cn = vp[j];
+    const float al = deep ? 1.0f : habc_weights(zz, xx, hc, bw[j]).alpha;
+    return (cn * cn) * A0 * al;
+  };
+  const float q0 = qw(z, x);
+  const float qzm = z > 0 ? qw(z - 1, x) : 0.0f;
+  const float qzp = z < g.nz - 1 ? qw(z + 1, x) : 0.0f;
+  const float qxm = x > 0 ? qw(z, x - 1) : 0.0f;
+  const float qxp = x < g.nx - 1 ? qw(z, x + 1) : 0.0f;
+  const float al0 = deep ? 1.0f : habc_weights(z, x, hc, bw[i]).alpha;
+  // one-way contributions: (flat offset, coefficient on y1, coefficient on y2)
+  int on[16];
+  float oc1[16], oc2[16];
+  int no = 0;
+  if (!deep) {
+    for (int s = 0; s < 4; ++s) {
+      const int dj = sdist(s, z, x, g);
+      const int e = sstep(s, g);
+      for (int m = 0; m < 3; ++m) {
+        const int dn = dj - m;
+        if (dn < 0 || dn > kW - 1 || (m == 2 && dn == kW - 1)) continue;
+        const int n = i - m * e;
+        const int nz_ = n / g.nx, nx_ = n - nz_ * g.nx;
+        const float be = habc_weights(nz_, nx_, hc, bw[n]).beta[s];
+        if (be == 0.0f) continue;
+        const OwCoef kc = ow_coef(vp[n], hc);
+        const float c1 = m == 0 ? kc.k0 : m == 1 ? kc.k1 : kc.k2;
+        const float c2 = m == 0 ? kc.k3 : m == 1 ? kc.k4 : 0.0f;
+        on[no] = n; oc1[no] = be * c1; oc2[no] = be * c2; ++no;
+      }
+      if (dj == 0) {  // the edge cell is the "+2" neighbour of the row-49 cell (roll wrap)
+        const int n = i + (kW - 1) * e;
+        const int nz_ = n / g.nx, nx_ = n - nz_ * g.nx;
+        const float be = habc_weights(nz_, nx_, hc, bw[n]).beta[s];
+        if (be != 0.0f) {
+          on[no] = n; oc1[no] = be * ow_coef(vp[n], hc).k2; oc2[no] = 0.0f; ++no;
+        }
+      }
+    }
+  }
+  const bool in = interior(z, x, g);
+  const float w = wn ? *wn : 0.0f;
+  float gacc = 0.0f;
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int bb = b0 + s;
+    const int64_t off = (int64_t)bb * g.plane;
+    float acc = 0.0f;
+    if (y1) {
+      const float* Y = y1 + off;
+      const float Lq = lap_taps(z > 0 ? qzm * Y[i - g.nx] : 0.0f, x > 0 ? qxm * Y[i - 1] : 0.0f,
+                                q0 * Y[i], x < g.nx - 1 ? qxp * Y[i + 1] : 0.0f,
+                                z < g.nz - 1 ? qzp * Y[i + g.nx] : 0.0f, k.ih2);
+      acc = (k.two_idt2 * (A0 * al0)) * Y[i] + Lq;
+      for (int o = 0; o < no; ++o) acc += oc1[o] * Y[on[o]];
+    }
+    if (y2) {
+      const float* Y = y2 + off;
+      acc += -(B0 * (A0 * al0)) * Y[i];
+      for (int o = 0; o < no; ++o) acc += oc2[o] * Y[on[o]];
+    }
+    out[off + i] = acc;
+    if (in && recon >= 0 && um) {
+      const float Lu = lap5(um + off, z, x, i, g, k.ih2);
+      gacc += Lu * acc;
+      if (recon == 1) {
+        float un = uc[off + i];
+        if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
+        uc[off + i] = ((k.two_idt2 * um[off + i] + (c * c) * Lu) - un / A0) / B0;
+      } else if (recon == 2) {
+        uc[off + i] = 0.0f;
+      }
+    }
+  }
+  if (in && um && gpart) gpart[(int64_t)grp * g.plane + i] += 2.0f * c * A0 * gacc;
+}
+
+// Per-step model gradient (all cells): sum over shots of
+// alpha*A0*2c*L(u)*gy + sum_s beta_s*gy*(dl/dc (u1+p0-u0-p1) + dm/dc (2u1-u0-u2)).
+__global__ __launch_bounds__(256) void k_habc_vjp_model(const float* __restrict__ vp,
+                                                        const float* __restrict__ bw,
+                                                        const float* __restrict__ u,
+                                                        const float* __restrict__ p,
+                                                        const float* __restrict__ gy,
+                                                        float* __restrict__ gvp, int B,
+                                                        HabcCtx hc, seis::TimeConsts k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const Grid& g = hc.g;
+  if (i >= g.plane) return;
+  const int z = (int)(i / g.nx), x = (int)(i - (int64_t)z * g.nx);
+  const float c = vp[i];
+  const float A0 = 1.0f / k.idt2;
+  const HabcW wt = habc_weights(z, x, hc, near_edge(z, x, g, kW) ? bw[i] : 0.0f);
+  const float dl = 2.0f * hc.dt / hc.h;
+  const float dm = 2.0f * c * (hc.dt * hc.dt) / (hc.h * hc.h);
+  float acc = 0.0f;
+  for (int b = 0; b < B; ++b) {
+    const int64_t off = (int64_t)b * g.plane;
+    const float* U = u + off;
+    const float* P = p + off;
+    const float gv = gy[off + i];
+    float t = wt.alpha * A0 * 2.0f * c * lap5(U, z, x, (int)i, g, k.ih2);
+    for (int s = 0; s < 4; ++s) {
+      if (wt.beta[s] == 0.0f) continue;
+      const int e = sstep(s, g);
+      const float u2 = sdist(s, z, x, g) == kW - 1 ? U[sedge(s, z, x, g)] : U[i + 2 * e];
+      t += wt.beta[s] * (dl * (U[i + e] + P[i] - U[i] - P[i + e]) + dm * (2.0f * U[i + e] - U[i] - u2));
+    }
+    acc += t * gv;
+  }
+  gvp[i] = acc;
+}
+
+// After k_habc_adj in the fused adjoint: adjoint source s^k at the receivers
+// (and its gradient share 2c*A0*L(u^{k-1})*s^k), ring restore of u^{k-2}.
+__global__ __launch_bounds__(256) void k_habc_post(
+    const float* __restrict__ vp, const float* __restrict__ um, float* __restrict__ uc,
+    float* __restrict__ lam, float* __restrict__ gpart, const int64_t* __restrict__ rec,
+    const float* __restrict__ rgrad, int64_t R, const float* __restrict__ ring_slot,
+    int64_t ring_len, int recon, int B, int S, Grid g, seis::TimeConsts k) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R) {
+    const int64_t f = rec[t];
+    const int64_t b = f / g.plane;
+    const int i = (int)(f - b * g.plane);
+    const int z = i / g.nx, x = i - z * g.nx;
+    const float sk = rgrad[t];
+    atomicAdd(lam + f, sk);
+    if (seis::interior(z, x, g)) {
+      const float Lu = lap5(um + b * g.plane, z, x, i, g, k.ih2);
+      atomicAdd(gpart + (b / S) * g.plane + i, 2.0f * vp[i] * (1.0f / k.idt2) * Lu * sk);
+    }
+    return;
+  }
+  t -= R;
+  if (recon != 0 && t < (int64_t)B * ring_len) {
+    const int64_t b = t / ring_len;
+    const int64_t j = t - b * ring_len;
+    uc[b * g.plane + seis::ring_cell(j, g)] = recon == 1 ? ring_slot[t] : 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_habc_record(const float* __restrict__ u,
+                                                     const int64_t* __restrict__ rec, int64_t R,
+                                                     float* __restrict__ rec_row,
+                                                     float* __restrict__ ring_slot,
+                                                     int64_t ring_len, int B, Grid g) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R) {
+    rec_row[t] = u[rec[t]];
+    return;
+  }
+  t -= R;
+  if (ring_slot && t < (int64_t)B * ring_len) {
+    const int64_t b = t / ring_len;
+    const int64_t j = t - b * ring_len;
+    ring_slot[t] = u[b * g.plane + seis::ring_cell(j, g)];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_habc_fill(float* __restrict__ p, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_habc_copy2(const float* __restrict__ a,
+                                                    float* __restrict__ ao,
+                                                    const float* __restrict__ b,
+                                                    float* __restrict__ bo, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    ao[t] = a[t];
+    bo[t] = b[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_habc_sum(const float* __restrict__ gpart, int G,
+                                                  float* __restrict__ out, int64_t plane) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= plane) return;
+  float s = 0.0f;
+  for (int q = 0; q < G; ++q) s += gpart[(int64_t)q * plane + t];
+  out[t] = s;
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+#define SEIS_HABC_LAUNCH(K, S, grid, s, ...)                      \
+  do {                                                            \
+    switch (S) {                                                  \
+      case 1: K<1><<<grid, 256, 0, s>>>(__VA_ARGS__); break;     \
+      case 2: K<2><<<grid, 256, 0, s>>>(__VA_ARGS__); break;     \
+      case 4: K<4><<<grid, 256, 0, s>>>(__VA_ARGS__); break;     \
+      default: K<8><<<grid, 256, 0, s>>>(__VA_ARGS__); break;    \
+    }                                                             \
+    SEIS_CHECK_LAUNCH();                                          \
+  } while (0)
+
+int habc_check(int64_t B, int64_t nz, int64_t nx, float dt, float h, int S) {
+  SEIS_CHECK_ARG(B >= 1 && B <= 65535, "batch size %lld outside [1, 65535]", (long long)B);
+  SEIS_CHECK_ARG(nz >= 2 * kW + 2 && nx >= 2 * kW + 2,
+                 "HABC needs at least %d cells per axis, got %lld x %lld", 2 * kW + 2,
+                 (long long)nz, (long long)nx);
+  SEIS_CHECK_ARG(nz * nx < (int64_t)1 << 31, "grid too large");
+  SEIS_CHECK_ARG(dt > 0.0f && h > 0.0f, "dt and h must be positive");
+  SEIS_CHECK_ARG(S == 1 || S == 2 || S == 4 || S == 8, "shots_per_group %d not in {1,2,4,8}", S);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int seis_habc_step(const float* vp, const float* b, const float* h1, const float* h2, float* y,
+                   int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t multiple,
+                   void* stream) {
+  if (int e = habc_check(B, nz, nx, dt, h, 1)) return e;
+  SEIS_CHECK_ARG(vp && b && h1 && h2 && y, "null pointer argument");
+  const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, dt, h};
+  const int S = B >= 4 ? 4 : 1;
+  const TileMap tm = make_tiles(hc.g, 1, 1, (int)((B + S - 1) / S));
+  hipStream_t s = (hipStream_t)stream;
+  SEIS_HABC_LAUNCH(k_habc_fwd, S, dim3(tm.per * kXcds), s, vp, b, h1, h2, y, nullptr, nullptr,
+                   (int)B, hc, seis::make_consts(dt, h), tm);
+  return 0;
+}
+
+int seis_habc_step_vjp(const float* vp, const float* b, const float* h1, const float* h2,
+                       const float* gy, float* gh1, float* gh2, float* gvp, int64_t B, int64_t nz,
+                       int64_t nx, float dt, float h, int32_t multiple, void* stream) {
+  if (int e = habc_check(B, nz, nx, dt, h, 1)) return e;
+  SEIS_CHECK_ARG(vp && b && h1 && h2 && gy && gh1 && gh2, "null pointer argument");
+  const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, dt, h};
+  const int S = B >= 4 ? 4 : 1;
+  const TileMap tm = make_tiles(hc.g, 1, 1, (int)((B + S - 1) / S));
+  const seis::TimeConsts k = seis::make_consts(dt, h);
+  hipStream_t s = (hipStream_t)stream;
+  SEIS_HABC_LAUNCH(k_habc_adj, S, dim3(tm.per * kXcds), s, vp, b, gy, nullptr, gh1, nullptr,
+                   nullptr, nullptr, nullptr, nullptr, -1, (int)B, hc, k, tm);
+  SEIS_HABC_LAUNCH(k_habc_adj, S, dim3(tm.per * kXcds), s, vp, b, nullptr, gy, gh2, nullptr,
+                   nullptr, nullptr, nullptr, nullptr, -1, (int)B, hc, k, tm);
+  if (gvp) {
+    k_habc_vjp_model<<<nblk(nz * nx), 256, 0, s>>>(vp, b, h1, h2, gy, gvp, (int)B, hc, k);
+    SEIS_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int64_t nt,
+                      const int32_t* src_zx, const int64_t* rec_flat, int64_t R, float* rec_out,
+                      float* ring, float* snaps, int32_t snap_every, float* u_a, float* u_b,
+                      float* u_c, int64_t B, int64_t nz, int64_t nx, float dt, float h,
+                      int32_t npml, int32_t multiple, int32_t shots_per_group, void* stream) {
+  if (int e = habc_check(B, nz, nx, dt, h, shots_per_group)) return e;
+  SEIS_CHECK_ARG(vp && b && wavelet && src_zx && u_a && u_b && u_c, "null pointer argument");
+  SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
+  SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
+  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml) : 0;
+  SEIS_CHECK_ARG(!ring || ring_len > 0, "grid has no interior inside npml=%d", npml);
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  const HabcCtx hc{Grid{(int)nz, (int)nx, npml, nz * nx}, multiple ? 1 : 0, dt, h};
+  const Grid& g = hc.g;
+  const int S = shots_per_group;
+  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  const seis::TimeConsts k = seis::make_consts(dt, h);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t field = B * g.plane;
+  // u^m lives in P[m mod 3]; chosen so that u^{nt} ends in u_a, u^{nt-1} in u_b
+  float* P[3];
+  P[nt % 3] = u_a;
+  P[(nt + 2) % 3] = u_b;
+  P[(nt + 1) % 3] = u_c;
+  for (float* q : {u_a, u_b, u_c}) {
+    k_habc_fill<<<nblk(field), 256, 0, s>>>(q, field);
+    SEIS_CHECK_LAUNCH();
+  }
+  for (int64_t n = 0; n < nt; ++n) {
+    float* un = P[n % 3];
+    float* up = P[(n + 2) % 3];
+    float* out = P[(n + 1) % 3];
+    SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
+    SEIS_HABC_LAUNCH(k_habc_fwd, S, dim3(tm.per * kXcds), s, vp, b, un, up, out, src_zx,
+                     wavelet + n, (int)B, hc, k, tm);
+    SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
+    float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
+    const int64_t work = R + (slot ? B * ring_len : 0);
+    if (work > 0) {
+      k_habc_record<<<nblk(work), 256, 0, s>>>(out, rec_flat, R, R ? rec_out + n * R : nullptr,
+                                               slot, ring_len, (int)B, g);
+      SEIS_CHECK_LAUNCH();
+    }
+    if (snaps && n >= snap_every && n % snap_every == 0 && n <= nt - 2) {
+      float* sp = snaps + (n / snap_every - 1) * 2 * field;
+      k_habc_copy2<<<nblk(field), 256, 0, s>>>(un, sp, out, sp + field, field);
+      SEIS_CHECK_LAUNCH();
+    }
+  }
+  return 0;
+}
+
+int seis_habc_backward(const float* vp, const float* b, const float* wavelet, int64_t nt,
+                       const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
+                       const float* rec_grad, const float* ring, const float* snaps,
+                       int32_t snap_every, float* u_a, float* u_b, float* l_a, float* l_b,
+                       float* l_c, float* grad_part, float* grad, int64_t B, int64_t nz,
+                       int64_t nx, float dt, float h, int32_t npml, int32_t multiple,
+                       int32_t shots_per_group, void* stream) {
+  if (int e = habc_check(B, nz, nx, dt, h, shots_per_group)) return e;
+  SEIS_CHECK_ARG(vp && b && wavelet && src_zx && u_a && u_b && l_a && l_b && l_c && grad_part &&
+                     grad,
+                 "null pointer argument");
+  SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
+  SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_grad)), "bad receiver arguments");
+  const int64_t ring_len = seis_ring_len(nz, nx, npml);
+  SEIS_CHECK_ARG(ring_len > 0, "grid has no interior inside npml=%d", npml);
+  SEIS_CHECK_ARG(ring || nt <= 2, "ring storage is required for nt > 2");
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  const HabcCtx hc{Grid{(int)nz, (int)nx, npml, nz * nx}, multiple ? 1 : 0, dt, h};
+  const Grid& g = hc.g;
+  const int S = shots_per_group;
+  const int G = (int)((B + S - 1) / S);
+  const TileMap tm = make_tiles(g, 1, 1, G);
+  const seis::TimeConsts k = seis::make_consts(dt, h);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t field = B * g.plane;
+  for (float* q : {l_a, l_b, l_c}) {
+    k_habc_fill<<<nblk(field), 256, 0, s>>>(q, field);
+    SEIS_CHECK_LAUNCH();
+  }
+  k_habc_fill<<<nblk((int64_t)G * g.plane), 256, 0, s>>>(grad_part, (int64_t)G * g.plane);
+  SEIS_CHECK_LAUNCH();
+  float* X = u_b;                  // u^{k-1}
+  float* Y = u_a;                  // u^k -> u^{k-2}
+  float* L1 = l_a;                 // lambda^{k+1}
+  float* L2 = l_b;                 // lambda^{k+2}
+  float* L0 = l_c;                 // lambda^k
+  for (int64_t kk = nt; kk >= 1; --kk) {
+    const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
+    SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
+    SEIS_HABC_LAUNCH(k_habc_adj, S, dim3(tm.per * kXcds), s, vp, b, L1, L2, L0, X, Y, grad_part,
+                     src_zx, wavelet + (kk - 1), recon, (int)B, hc, k, tm);
+    SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
+    const int64_t work = R + (recon ? B * ring_len : 0);
+    if (work > 0) {
+      const float* slot = recon == 1 ? ring + (kk - 3) * B * ring_len : nullptr;
+      k_habc_post<<<nblk(work), 256, 0, s>>>(vp, X, Y, L0, grad_part, rec_flat,
+                                             R ? rec_grad + (kk - 1) * R : nullptr, R, slot,
+                                             ring_len, recon, (int)B, S, g, k);
+      SEIS_CHECK_LAUNCH();
+    }
+    const int64_t m = kk - 2;
+    if (snaps && m >= snap_every && m % snap_every == 0 && m <= nt - 2) {
+      const float* sp = snaps + (m / snap_every - 1) * 2 * field;
+      k_habc_copy2<<<nblk(field), 256, 0, s>>>(sp, Y, sp + field, X, field);
+      SEIS_CHECK_LAUNCH();
+    }
+    float* t = X; X = Y; Y = t;
+    t = L2; L2 = L1; L1 = L0; L0 = t;
+  }
+  k_habc_sum<<<nblk(g.plane), 256, 0, s>>>(grad_part, G, grad, g.plane);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // extern "C"

@@ -37,6 +37,36 @@ __device__ __forceinline__ int64_t ring_cell(int64_t j, const Grid g) {
   return (int64_t)z * g.nx + x;
 }
 
+// 5-point Laplacian from its taps: the h^-2-weighted taps accumulated in
+// row-major kernel order (z-1, x-1, centre, x+1, z+1) with fused multiply-adds,
+// the way a direct 3x3 convolution with weights h^-2*[[0,1,0],[1,-4,1],[0,1,0]]
+// accumulates (acoustic.py:60-71).  Every kernel uses this one helper.
+__device__ __forceinline__ float lap_taps(float zm, float xm, float c, float xp, float zp,
+                                          float ih2) {
+  float acc = ih2 * zm;
+  acc = __fmaf_rn(ih2, xm, acc);
+  acc = __fmaf_rn(-4.0f * ih2, c, acc);
+  acc = __fmaf_rn(ih2, xp, acc);
+  return __fmaf_rn(ih2, zp, acc);
+}
+
+// The damped update of acoustic.py:81-84 rounded op by op in its order:
+// A*((2/dt^2*u - B*u_prev) + c^2*L).
+__device__ __forceinline__ float acoustic_update(float A, float Bc, float two_idt2, float u,
+                                                 float up, float c2, float L) {
+  return A * ((two_idt2 * u - Bc * up) + c2 * L);
+}
+
+__device__ __forceinline__ float lap5(const float* __restrict__ f, int z, int x, int i,
+                                      const Grid g, float ih2) {
+  const float c = f[i];
+  const float zm = z > 0 ? f[i - g.nx] : 0.0f;
+  const float zp = z < g.nz - 1 ? f[i + g.nx] : 0.0f;
+  const float xm = x > 0 ? f[i - 1] : 0.0f;
+  const float xp = x < g.nx - 1 ? f[i + 1] : 0.0f;
+  return lap_taps(zm, xm, c, xp, zp, ih2);
+}
+
 namespace tiles {
 
 // Tiles of the fused loops: a 256-thread block is 64 lanes along x by 4

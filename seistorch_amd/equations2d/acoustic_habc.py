@@ -1,0 +1,156 @@
+"""Constant-density acoustic equation with the HABC boundary, backed by HIP kernels.
+
+Drop-in for ``seistorch.equations2d.acoustic_habc``
+(seistorch/equations2d/acoustic_habc.py):
+
+    _time_step(vp, h1, h2, dt, h, b, habcs=masks)             -> (y, h1)
+    _time_step_backward(vp, h1, h2, dt, h, b, rings, source)   -> (y, h1)
+    _time_step_backward_multiple(...)                          -> (y, h1)
+
+``b`` is the HABC weight field (seistorch_amd.habc.generate_habc_coefficients_2d);
+``habcs`` are the strip masks the reference's cell passes (cell.py:20-27); only
+whether the top mask is None (free surface) is read from them, the kernel
+evaluates the trapezoids itself.  The step's gradient is the hand-written
+transpose kernel.  ``HabcLoop`` / ``propagate`` is the fused loop with the
+boundary-saving adjoint (ring at index 49, whose HABC weight is 0, so the
+interior reverses exactly as in the PML case).
+"""
+import torch
+
+from .. import _lib
+from .acoustic import (NPML, _check_acquisition, _f, default_shots_per_group,
+                       restore_boundaries, step as _undamped_step)
+
+
+def _multiple(habcs):
+    return habcs is not None and habcs[0] is None
+
+
+class _HabcStepFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vp, h1, h2, b, dt, h, multiple):
+        _lib.require_device(vp, h1, h2, b)
+        if h1.shape != h2.shape or h1.dim() != 3 or vp.shape != h1.shape[1:] or b.shape != vp.shape:
+            raise RuntimeError("habc step: shape mismatch")
+        B, nz, nx = h1.shape
+        y = torch.empty_like(h1)
+        _lib.call("seis_habc_step", _lib.ptr(vp), _lib.ptr(b), _lib.ptr(h1), _lib.ptr(h2),
+                  _lib.ptr(y), B, nz, nx, dt, h, int(multiple), _lib.stream_of(h1))
+        ctx.save_for_backward(vp, b, h1, h2)
+        ctx.meta = (dt, h, int(multiple))
+        return y
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, gy):
+        vp, b, h1, h2 = ctx.saved_tensors
+        dt, h, multiple = ctx.meta
+        B, nz, nx = h1.shape
+        gy = gy.contiguous()
+        gh1 = torch.empty_like(h1)
+        gh2 = torch.empty_like(h1)
+        gvp = torch.empty_like(vp) if ctx.needs_input_grad[0] else None
+        _lib.call("seis_habc_step_vjp", _lib.ptr(vp), _lib.ptr(b), _lib.ptr(h1), _lib.ptr(h2),
+                  _lib.ptr(gy), _lib.ptr(gh1), _lib.ptr(gh2), _lib.ptr(gvp), B, nz, nx, dt, h,
+                  multiple, _lib.stream_of(h1))
+        return gvp, gh1, gh2, None, None, None, None
+
+
+def _time_step(*args, **kwargs):
+    """acoustic_habc.py:206-221: undamped step + HABC strips, returns (y, h1)."""
+    c = args[0]
+    h1, h2 = args[1:3]
+    dt, h, b = args[3:6]
+    multiple = _multiple(kwargs.get("habcs"))
+    y = _HabcStepFn.apply(c, h1.contiguous(), h2.contiguous(), b, _f(dt), _f(h), multiple)
+    return y, h1
+
+
+def _backward_common(args, multiple):
+    vp = args[0]
+    h1, h2 = args[1:3]
+    dt, h, b = args[3:6]
+    h_bd, _ = args[-2]
+    src_type, src_func, src_values = args[-1]
+    y = _undamped_step(vp, h1, h2, torch.zeros_like(vp), dt, h)
+    with torch.no_grad():
+        y = restore_boundaries(y, h_bd, multiple=multiple)
+    y = src_func(y, src_values, 1)
+    return y, h1
+
+
+def _time_step_backward(*args, **kwargs):
+    """acoustic_habc.py:223-245: undamped reverse step, ring restore, source re-add."""
+    return _backward_common(args, multiple=False)
+
+
+def _time_step_backward_multiple(*args, **kwargs):
+    """acoustic_habc.py:247-268."""
+    return _backward_common(args, multiple=True)
+
+
+# ----------------------------------------------------------------- fused loop
+
+class HabcLoop(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vp, b, wavelet, src_zx, rec_flat, B, dt, h, multiple, shots_per_group,
+                snapshot_every):
+        from .acoustic import default_snapshot_every, ring_len
+        _lib.require_device(vp, b, wavelet, src_zx, rec_flat)
+        need_grad = ctx.needs_input_grad[0]
+        # the HABC weight is zero on the interior, which the adjoint's reversal needs
+        _check_acquisition(vp, b, wavelet, src_zx, rec_flat, B, need_grad)
+        nz, nx = vp.shape
+        nt = wavelet.shape[0]
+        R = rec_flat.shape[0]
+        S = int(shots_per_group or default_shots_per_group(B))
+        u = [torch.empty((B, nz, nx), device=vp.device) for _ in range(3)]
+        rec = torch.empty((nt, R), device=vp.device)
+        ring = snaps = None
+        K = 0
+        if need_grad:
+            if nt > 1:
+                ring = torch.empty((nt - 1, B, ring_len(nz, nx)), device=vp.device)
+            K = default_snapshot_every(nt, B, nz, nx, vp.device) if snapshot_every is None \
+                else int(snapshot_every)
+            n_snap = int(_lib.load().seis_snapshot_count(nt, K))
+            if n_snap > 0:
+                snaps = torch.empty((n_snap, 2, B, nz, nx), device=vp.device)
+        _lib.call("seis_habc_forward", _lib.ptr(vp), _lib.ptr(b), _lib.ptr(wavelet), nt,
+                  _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
+                  _lib.ptr(snaps), K, _lib.ptr(u[0]), _lib.ptr(u[1]), _lib.ptr(u[2]), B, nz, nx,
+                  dt, h, NPML, int(multiple), S, _lib.stream_of(vp))
+        if need_grad:
+            ctx.save_for_backward(vp, b, wavelet, src_zx, rec_flat)
+            ctx.state = (ring, snaps, u[0], u[1])
+            ctx.meta = (B, dt, h, S, K, int(multiple))
+        return rec
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, grad_rec):
+        vp, b, wavelet, src_zx, rec_flat = ctx.saved_tensors
+        ring, snaps, u_a, u_b = ctx.state
+        ctx.state = None
+        B, dt, h, S, K, multiple = ctx.meta
+        nz, nx = vp.shape
+        nt = wavelet.shape[0]
+        R = rec_flat.shape[0]
+        G = (B + S - 1) // S
+        lam = [torch.empty_like(u_a) for _ in range(3)]
+        gpart = torch.empty((G, nz, nx), device=vp.device)
+        grad = torch.empty_like(vp)
+        _lib.call("seis_habc_backward", _lib.ptr(vp), _lib.ptr(b), _lib.ptr(wavelet), nt,
+                  _lib.ptr(src_zx), _lib.ptr(rec_flat), R,
+                  _lib.ptr(grad_rec.contiguous().float()), _lib.ptr(ring), _lib.ptr(snaps),
+                  K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(lam[0]),
+                  _lib.ptr(lam[1]), _lib.ptr(lam[2]), _lib.ptr(gpart), _lib.ptr(grad), B, nz, nx,
+                  dt, h, NPML, multiple, S, _lib.stream_of(vp))
+        return (grad,) + (None,) * 10
+
+
+def propagate(vp, b, wavelet, src_zx, rec_flat, B, dt, h, multiple=False, shots_per_group=None,
+              snapshot_every=None):
+    """Records (nt, R) of the HABC loop; differentiable w.r.t. vp."""
+    return HabcLoop.apply(vp, b, wavelet.contiguous(), src_zx.contiguous(), rec_flat.contiguous(),
+                          int(B), _f(dt), _f(h), bool(multiple), shots_per_group, snapshot_every)

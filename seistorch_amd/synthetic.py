@@ -70,3 +70,24 @@ def rec_flat_index(recs, nz, nx):
         r = np.asarray(r, np.int64)
         out.append(b * nz * nx + r[:, 0] * nx + r[:, 1])
     return np.concatenate(out)
+
+
+def elastic_models(nz, nx, vmin=1800.0, vmax=4500.0, water=20, pert=0.03, sigma=10.0,
+                   seed=DEFAULT_SEED):
+    """(vp, vs, rho): smooth layered vp with a seeded smooth perturbation,
+    vs = vp/1.73 (0 in the top `water` rows), Gardner rho = 310 vp^0.25, each
+    with its own seeded smooth perturbation."""
+    rng = np.random.default_rng(seed)
+    z = np.linspace(0.0, 1.0, nz)[:, None]
+    vp = vmin + (vmax - vmin) * z + 0.0 * np.arange(nx)[None, :]
+    vp = gaussian_filter(vp + np.floor(z * 5) / 5 * 0.1 * (vmax - vmin), 2.0)
+
+    def smooth_noise():
+        n = gaussian_filter(rng.standard_normal((nz, nx)), sigma)
+        return n / max(np.abs(n).max(), 1e-30)
+
+    vp = vp * (1.0 + pert * smooth_noise())
+    vs = vp / 1.73 * (1.0 + pert * smooth_noise())
+    vs[:water] = 0.0
+    rho = 310.0 * vp ** 0.25 * (1.0 + pert * smooth_noise())
+    return vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32)
