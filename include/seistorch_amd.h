@@ -37,6 +37,8 @@ int32_t seis_abi_version(void);
 #define SEIS_KERNEL_ELASTIC_BWD 3  /* both adjoint stages of one step */
 #define SEIS_KERNEL_HABC_FWD 4
 #define SEIS_KERNEL_HABC_BWD 5
+#define SEIS_KERNEL_BATCH_FWD 6  /* k_fwd_batch, direct launches only */
+#define SEIS_KERNEL_BATCH_BWD 7  /* k_bwd_batch, direct launches only */
 #define SEIS_PROFILE_KERNELS 8
 int seis_profile_enable(int32_t on);
 int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
@@ -109,6 +111,50 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            int64_t nz, int64_t nx, float dt, float h, int32_t npml,
                            int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                            void* stream);
+
+/* ---------------- acoustic PML, uneven multi-model batch ------------------ */
+/* n independent acoustic PML problems advanced together: every time step is
+ * one launch over the tiles of all instances still running (the reference
+ * runs each model through its own rnn.py:178-212 loop and backward,
+ * seistorch/checkpoint_new.py:146-217; these two functions are
+ * seis_acoustic_forward / seis_acoustic_backward applied to every instance at
+ * once, bit-identical per instance to the single-instance calls with the same
+ * launch shape and snap_every).
+ *
+ * shape (host, n x 5 int64): nz, nx, B, nt, R of each instance;
+ * dt_h (host, n x 2 float): dt, h.  Every device arena is the concatenation
+ * over q = 0..n-1 (the caller's order) of the instance's array:
+ *   vp, d, grad          (nz, nx)
+ *   wavelet              (nt,)
+ *   src_zx               (B, 2) int32
+ *   rec_flat             (R,) int64, flat indices into the instance's (B, nz, nx)
+ *   rec_out, rec_grad    (nt, R)
+ *   ring                 (nt-1, B, seis_ring_len(nz, nx, npml))
+ *   snaps                (seis_snapshot_count(nt, snap_every), 2, B, nz, nx)
+ *   u_a, u_b, mu_a, mu_b (B, nz, nx)
+ *   grad_part            (ceil(B/shots_per_group), nz, nx)
+ * table is device scratch of n x SEIS_BATCH_COLS int64 (written here).
+ * vec_x must divide every nx.  use_graph != 0 records the whole time loop
+ * into one hipGraph on an internal capture stream and launches it on
+ * `stream` (waits for it to finish before returning); 0 launches the
+ * kernels directly on `stream`. */
+#define SEIS_BATCH_COLS 22
+int seis_acoustic_forward_batch(int64_t n, const int64_t* shape, const float* dt_h,
+                                const float* vp, const float* d, const float* wavelet,
+                                const int32_t* src_zx, const int64_t* rec_flat, float* rec_out,
+                                float* ring, float* snaps, int32_t snap_every, float* u_a,
+                                float* u_b, int64_t* table, int32_t npml,
+                                int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
+                                int32_t use_graph, void* stream);
+
+int seis_acoustic_backward_batch(int64_t n, const int64_t* shape, const float* dt_h,
+                                 const float* vp, const float* d, const float* wavelet,
+                                 const int32_t* src_zx, const int64_t* rec_flat,
+                                 const float* rec_grad, const float* ring, const float* snaps,
+                                 int32_t snap_every, float* u_a, float* u_b, float* mu_a,
+                                 float* mu_b, float* grad_part, float* grad, int64_t* table,
+                                 int32_t npml, int32_t shots_per_group, int32_t rows_per_thread,
+                                 int32_t vec_x, int32_t use_graph, void* stream);
 
 /* ---------------- acoustic, constant density, HABC ----------------------- */
 /* b is the (nz, nx) HABC weight field (seistorch/habc.py:252-291); the strips

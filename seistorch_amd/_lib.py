@@ -36,6 +36,12 @@ SIGNATURES = {
     "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
                                               _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
                                               _i32, _i32, _i32, _i32, _p]),
+    "seis_acoustic_forward_batch": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                                   _p, _i32, _p, _p, _p, _i32, _i32, _i32, _i32,
+                                                   _i32, _p]),
+    "seis_acoustic_backward_batch": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                                    _p, _i32, _p, _p, _p, _p, _p, _p, _p, _i32,
+                                                    _i32, _i32, _i32, _i32, _p]),
     "seis_habc_step": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32, _p]),
     "seis_habc_step_vjp": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f,
                                           _f, _i32, _p]),
@@ -60,6 +66,7 @@ SIGNATURES = {
 }
 
 ABI_VERSION = 1
+BATCH_COLS = 22  # SEIS_BATCH_COLS: int64 columns per instance of the batch table
 
 _lock = threading.Lock()
 _lib = None

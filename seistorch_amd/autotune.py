@@ -72,3 +72,49 @@ def tune_acoustic(vp, d, wavelet, src_zx, rec_flat, B, dt, h, nt_probe=24, repea
     shape = min(results, key=results.get)
     record("acoustic", B, nz, nx, shape)
     return shape, results
+
+
+def _batch_key(sizes):
+    import hashlib
+    blob = json.dumps([list(map(int, s)) for s in sizes]).encode()
+    return "acoustic_batch:" + hashlib.sha1(blob).hexdigest()[:16]
+
+
+def lookup_batch(sizes):
+    return _TABLE.get(_batch_key(sizes))
+
+
+def tune_batch(insts, nt_probe=16, repeats=2, candidates_s=CANDIDATES_S,
+               candidates_rz=CANDIDATES_RZ, candidates_vx=CANDIDATES_VX, verbose=False):
+    """Time one batched forward+adjoint of every instance over nt_probe steps
+    (all instances active, the dominant regime) for each launch shape."""
+    from .equations2d import acoustic_batch as ab
+    sizes = [(int(it.vp.shape[0]), int(it.vp.shape[1]), int(it.B), int(it.wavelet.shape[0]),
+              int(it.rec_flat.shape[0])) for it in insts]
+    probe = [ab.BatchInstance(it.vp.detach(), it.d, it.wavelet[:nt_probe].contiguous(),
+                              it.src_zx, it.rec_flat, it.B, it.dt, it.h) for it in insts]
+    bmin = min(s[2] for s in sizes)
+    results = {}
+    for S, rz, vx in itertools.product(candidates_s, candidates_rz, candidates_vx):
+        if S > bmin or any(s[1] % vx for s in sizes):
+            continue
+        best = float("inf")
+        for _ in range(repeats):
+            for it in probe:
+                it.vp = it.vp.detach().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            recs = ab.propagate_batch(probe, shots_per_group=S, rows_per_thread=rz, vec_x=vx,
+                                      snapshot_every=0, use_graph=False)
+            torch.autograd.backward(recs, [torch.ones_like(r) for r in recs])
+            t1.record()
+            torch.cuda.synchronize()
+            best = min(best, t0.elapsed_time(t1))
+        results[(S, rz, vx)] = best
+        if verbose:
+            print(f"[autotune] batch S={S} rz={rz} vx={vx}: {best:.2f} ms", flush=True)
+    shape = min(results, key=results.get)
+    _TABLE[_batch_key(sizes)] = shape
+    return shape, results

@@ -22,6 +22,11 @@
 #include "seistorch_amd.h"
 #include "tiles.h"
 
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
 using seis::Grid;
 using seis::lap_taps;
 using seis::acoustic_update;
@@ -113,12 +118,10 @@ __global__ __launch_bounds__(256) void k_step_vjp_model(
 // over u^{n-1} (each cell of u^{n-1} is read only by the thread that
 // overwrites it).  Same Laplacian and update helpers as the per-step kernel.
 template <int S, int RZ, int VX>
-__global__ __launch_bounds__(kBlock) void k_fwd(
-    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ u,
-    float* __restrict__ up, const int32_t* __restrict__ src, const float* __restrict__ wn, int B,
-    Grid g, seis::TimeConsts k, TileMap tm) {
-  int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;
+__device__ __forceinline__ void fwd_tile(
+    int tx, int tzi, int grp, const float* __restrict__ vp, const float* __restrict__ d,
+    const float* __restrict__ u, float* __restrict__ up, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int B, Grid g, const seis::TimeConsts& k) {
   const int x0 = (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
   const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
   if (x0 >= g.nx || zb >= g.nz) return;
@@ -208,13 +211,11 @@ __global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
 // shortcut 2u^{k-1} - u^k + c^2 dt^2 L assumes A*B == 1, which FP32 misses by
 // ~1 ulp and which then drifts coherently over the reversed steps.
 template <int S, int RZ, int VX>
-__global__ __launch_bounds__(kBlock) void k_bwd(
-    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ um,
-    float* __restrict__ uc, const float* __restrict__ m1, float* __restrict__ m2,
-    float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
-    int recon, int B, Grid g, seis::TimeConsts k, TileMap tm) {
-  int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;
+__device__ __forceinline__ void bwd_tile(
+    int tx, int tzi, int grp, const float* __restrict__ vp, const float* __restrict__ d,
+    const float* __restrict__ um, float* __restrict__ uc, const float* __restrict__ m1,
+    float* __restrict__ m2, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int recon, int B, Grid g, const seis::TimeConsts& k) {
   const int x0 = (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
   const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
   if (x0 >= g.nx || zb >= g.nz) return;
@@ -337,6 +338,27 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
   }
 }
 
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_fwd(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ u,
+    float* __restrict__ up, const int32_t* __restrict__ src, const float* __restrict__ wn, int B,
+    Grid g, seis::TimeConsts k, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  fwd_tile<S, RZ, VX>(tx, tzi, grp, vp, d, u, up, src, wn, B, g, k);
+}
+
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_bwd(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ um,
+    float* __restrict__ uc, const float* __restrict__ m1, float* __restrict__ m2,
+    float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
+    int recon, int B, Grid g, seis::TimeConsts k, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  bwd_tile<S, RZ, VX>(tx, tzi, grp, vp, d, um, uc, m1, m2, gpart, src, wn, recon, B, g, k);
+}
+
 // After k_bwd: inject the adjoint source A*s^k at the receivers of mu^k (and
 // its gradient share 2c*L(u^{k-1})*A*s^k), restore the ring of u^{k-2}.
 __global__ __launch_bounds__(256) void k_bwd_post(
@@ -400,18 +422,192 @@ inline int copy2(const float* a, float* ao, const float* b, float* bo, int64_t n
   return 0;
 }
 
+// ------------------------------------------------- uneven multi-instance batch
+//
+// Many independent problems (own grid, shots, nt, dt, h, receivers) advance in
+// one launch per time step.  Instance q is one row of an int64 table on the
+// device (columns below).  The rows are sorted by nt, longest first, and every
+// instance starts at global step 0 (the reverse loop at its own last step), so
+// the instances still running at global step j are a prefix of the table and
+// each launch covers exactly their tiles: global tile t belongs to the last
+// row whose first tile is <= t.  Offsets index the concatenated arenas.
+enum BatchCol {
+  kCNz = 0, kCNx, kCB, kCNt, kCR, kCNtx, kCNtz,
+  kCTile0,     // first global tile of the row (prefix over the sorted rows)
+  kCAux0,      // first global index of its receiver + ring work
+  kCField,     // (B, nz, nx) fields in the field arenas
+  kCModel,     // (nz, nx) in the model arenas
+  kCSrc,       // shot index into the (sum B, 2) source arena
+  kCRec,       // receiver index into the rec arena (flat into its own fields)
+  kCOut,       // (nt, R) records
+  kCWav,       // wavelet samples
+  kCRing,      // (nt-1, B, ring_len) rings
+  kCRingLen,
+  kCGpart,     // (groups, nz, nx) gradient planes
+  kCIdt2, kCIdt, kCTwoIdt2, kCIh2,  // FP32 time constants as bit patterns
+  kBatchCols
+};
+static_assert(kBatchCols == SEIS_BATCH_COLS, "batch table width");
+
+// Last row of the active prefix [0, n_active) whose column `col` is <= t.
+__device__ __forceinline__ int find_row(const int64_t* __restrict__ tab, int n_active, int col,
+                                        int64_t t) {
+  int lo = 0, hi = n_active - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[(int64_t)mid * kBatchCols + col] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ Grid row_grid(const int64_t* r, int npml) {
+  return Grid{(int)r[kCNz], (int)r[kCNx], npml, r[kCNz] * r[kCNx]};
+}
+
+__device__ __forceinline__ seis::TimeConsts row_consts(const int64_t* r) {
+  seis::TimeConsts c;
+  c.idt2 = __int_as_float((int)r[kCIdt2]);
+  c.idt = __int_as_float((int)r[kCIdt]);
+  c.two_idt2 = __int_as_float((int)r[kCTwoIdt2]);
+  c.ih2 = __int_as_float((int)r[kCIh2]);
+  c.dt2 = 0.0f;
+  return c;
+}
+
+// u^m of an instance lives in field arena a iff m + nt is even (so u^nt ends
+// in a, as in the single-instance loop); mu^m lives in a iff m + nt + 1 is even.
+__device__ __forceinline__ bool in_a(int64_t m, int64_t nt) { return ((m + nt) & 1) == 0; }
+
+// Row of global tile t (XCD-aware order over the active tiles) and its local
+// tile coordinates.
+__device__ __forceinline__ const int64_t* batch_tile(const int64_t* __restrict__ tab,
+                                                     int n_active, int64_t tiles, int per,
+                                                     int& tx, int& tzi, int& grp) {
+  const int w = blockIdx.x;
+  const int64_t t = (int64_t)(w % kXcds) * per + w / kXcds;
+  if (t >= tiles) return nullptr;
+  const int64_t* r = tab + (int64_t)find_row(tab, n_active, kCTile0, t) * kBatchCols;
+  const int64_t local = t - r[kCTile0];
+  const int64_t ntx = r[kCNtx], ntz = r[kCNtz];
+  tx = (int)(local % ntx);
+  tzi = (int)((local / ntx) % ntz);
+  grp = (int)(local / (ntx * ntz));
+  return r;
+}
+
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_fwd_batch(
+    const int64_t* __restrict__ tab, int n_active, int64_t tiles, int per, int npml,
+    const float* __restrict__ vp, const float* __restrict__ d, float* __restrict__ fa,
+    float* __restrict__ fb, const int32_t* __restrict__ src, const float* __restrict__ wav,
+    int64_t n) {
+  int tx, tzi, grp;
+  const int64_t* r = batch_tile(tab, n_active, tiles, per, tx, tzi, grp);
+  if (!r) return;
+  const int64_t fo = r[kCField];
+  const bool a = in_a(n, r[kCNt]);
+  fwd_tile<S, RZ, VX>(tx, tzi, grp, vp + r[kCModel], d + r[kCModel], (a ? fa : fb) + fo,
+                      (a ? fb : fa) + fo, src + 2 * r[kCSrc], wav + r[kCWav] + n, (int)r[kCB],
+                      row_grid(r, npml), row_consts(r));
+}
+
+// Receivers of u^{n+1} into records row n, and its ring into slot n.
+__global__ __launch_bounds__(256) void k_record_batch(
+    const int64_t* __restrict__ tab, int n_active, int64_t work, int npml,
+    const float* __restrict__ fa, const float* __restrict__ fb, const int64_t* __restrict__ rec,
+    float* __restrict__ out, float* __restrict__ ring, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= work) return;
+  const int64_t* r = tab + (int64_t)find_row(tab, n_active, kCAux0, t) * kBatchCols;
+  const int64_t local = t - r[kCAux0];
+  const int64_t R = r[kCR], nt = r[kCNt];
+  const float* u = (in_a(n + 1, nt) ? fa : fb) + r[kCField];
+  if (local < R) {
+    out[r[kCOut] + n * R + local] = u[rec[r[kCRec] + local]];
+    return;
+  }
+  const int64_t j0 = local - R;
+  const int64_t L = r[kCRingLen];
+  if (ring && n + 1 < nt && j0 < r[kCB] * L) {
+    const int64_t b = j0 / L;
+    const Grid g = row_grid(r, npml);
+    ring[r[kCRing] + n * r[kCB] * L + j0] = u[b * g.plane + seis::ring_cell(j0 - b * L, g)];
+  }
+}
+
+// Reverse global iteration j: each active row runs its own iteration kk = nt - j.
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_bwd_batch(
+    const int64_t* __restrict__ tab, int n_active, int64_t tiles, int per, int npml,
+    const float* __restrict__ vp, const float* __restrict__ d, float* __restrict__ fa,
+    float* __restrict__ fb, float* __restrict__ ma, float* __restrict__ mb,
+    float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wav,
+    int64_t j) {
+  int tx, tzi, grp;
+  const int64_t* r = batch_tile(tab, n_active, tiles, per, tx, tzi, grp);
+  if (!r) return;
+  const int64_t nt = r[kCNt], kk = nt - j, fo = r[kCField];
+  const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
+  bwd_tile<S, RZ, VX>(tx, tzi, grp, vp + r[kCModel], d + r[kCModel],
+                      (in_a(kk - 1, nt) ? fa : fb) + fo, (in_a(kk, nt) ? fa : fb) + fo,
+                      (in_a(kk + 2, nt) ? ma : mb) + fo, (in_a(kk + 3, nt) ? ma : mb) + fo,
+                      gpart + r[kCGpart], src + 2 * r[kCSrc], wav + r[kCWav] + (kk - 1), recon,
+                      (int)r[kCB], row_grid(r, npml), row_consts(r));
+}
+
+// k_bwd_post for every active row.
+__global__ __launch_bounds__(256) void k_post_batch(
+    const int64_t* __restrict__ tab, int n_active, int64_t work, int npml, int S,
+    const float* __restrict__ vp, const float* __restrict__ d, float* __restrict__ fa,
+    float* __restrict__ fb, float* __restrict__ ma, float* __restrict__ mb,
+    float* __restrict__ gpart, const int64_t* __restrict__ rec, const float* __restrict__ rgrad,
+    const float* __restrict__ ring, int64_t j) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= work) return;
+  const int64_t* r = tab + (int64_t)find_row(tab, n_active, kCAux0, t) * kBatchCols;
+  const int64_t local = t - r[kCAux0];
+  const int64_t R = r[kCR], nt = r[kCNt], kk = nt - j, fo = r[kCField], mo = r[kCModel];
+  const Grid g = row_grid(r, npml);
+  if (local < R) {
+    const seis::TimeConsts k = row_consts(r);
+    const float* X = (in_a(kk - 1, nt) ? fa : fb) + fo;
+    float* M2 = (in_a(kk + 3, nt) ? ma : mb) + fo;
+    const int64_t f = rec[r[kCRec] + local];
+    const int64_t b = f / g.plane;
+    const int i = (int)(f - b * g.plane);
+    const int z = i / g.nx, x = i - z * g.nx;
+    const float A = 1.0f / (k.idt2 + d[mo + i] * k.idt);
+    const float as = A * rgrad[r[kCOut] + (kk - 1) * R + local];
+    atomicAdd(M2 + f, as);
+    if (seis::interior(z, x, g)) {
+      const float Lu = lap5(X + b * g.plane, z, x, i, g, k.ih2);
+      atomicAdd(gpart + r[kCGpart] + (b / S) * g.plane + i, 2.0f * vp[mo + i] * Lu * as);
+    }
+    return;
+  }
+  const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
+  const int64_t j0 = local - R;
+  const int64_t L = r[kCRingLen];
+  if (recon != 0 && j0 < r[kCB] * L) {
+    float* Y = (in_a(kk, nt) ? fa : fb) + fo;
+    const int64_t b = j0 / L;
+    Y[b * g.plane + seis::ring_cell(j0 - b * L, g)] =
+        recon == 1 ? ring[r[kCRing] + (kk - 3) * r[kCB] * L + j0] : 0.0f;
+  }
+}
+
 // Instantiated launch shapes: shots per thread S, rows per thread RZ, cells
 // per lane VX.
 #define SEIS_RZ_VX(K, S_)                                                              \
-  case S_ * 256 + 1 * 16 + 1: K<S_, 1, 1><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 2 * 16 + 1: K<S_, 2, 1><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 4 * 16 + 1: K<S_, 4, 1><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 1 * 16 + 2: K<S_, 1, 2><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 2 * 16 + 2: K<S_, 2, 2><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 4 * 16 + 2: K<S_, 4, 2><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 1 * 16 + 4: K<S_, 1, 4><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 2 * 16 + 4: K<S_, 2, 4><<<grid, kBlock, 0, s>>>(a..., tm); break;  \
-  case S_ * 256 + 4 * 16 + 4: K<S_, 4, 4><<<grid, kBlock, 0, s>>>(a..., tm); break;
+  case S_ * 256 + 1 * 16 + 1: K<S_, 1, 1><<<grid, kBlock, 0, s>>>(a...); break;        \
+  case S_ * 256 + 2 * 16 + 1: K<S_, 2, 1><<<grid, kBlock, 0, s>>>(a...); break;        \
+  case S_ * 256 + 4 * 16 + 1: K<S_, 4, 1><<<grid, kBlock, 0, s>>>(a...); break;        \
+  case S_ * 256 + 1 * 16 + 2: K<S_, 1, 2><<<grid, kBlock, 0, s>>>(a...); break;        \
+  case S_ * 256 + 2 * 16 + 2: K<S_, 2, 2><<<grid, kBlock, 0, s>>>(a...); break;        \
+  case S_ * 256 + 4 * 16 + 2: K<S_, 4, 2><<<grid, kBlock, 0, s>>>(a...); break;        \
+  case S_ * 256 + 1 * 16 + 4: K<S_, 1, 4><<<grid, kBlock, 0, s>>>(a...); break;        \
+  case S_ * 256 + 2 * 16 + 4: K<S_, 2, 4><<<grid, kBlock, 0, s>>>(a...); break;        \
+  case S_ * 256 + 4 * 16 + 4: K<S_, 4, 4><<<grid, kBlock, 0, s>>>(a...); break;      
 
 inline bool valid_shape(int S, int RZ, int VX) {
   return (S == 1 || S == 2 || S == 4 || S == 8) && (RZ == 1 || RZ == 2 || RZ == 4) &&
@@ -420,8 +616,8 @@ inline bool valid_shape(int S, int RZ, int VX) {
 
 #define SEIS_DEFINE_LAUNCH(NAME, K)                                                   \
   template <typename... Args>                                                        \
-  int NAME(int S, int RZ, int VX, const TileMap& tm, hipStream_t s, Args... a) {     \
-    const dim3 grid(tm.per * kXcds);                                                 \
+  int NAME(int S, int RZ, int VX, int64_t blocks, hipStream_t s, Args... a) {      \
+    const dim3 grid((unsigned)blocks);                                               \
     switch (S * 256 + RZ * 16 + VX) {                                                \
       SEIS_RZ_VX(K, 1)                                                               \
       SEIS_RZ_VX(K, 2)                                                               \
@@ -437,6 +633,8 @@ inline bool valid_shape(int S, int RZ, int VX) {
 
 SEIS_DEFINE_LAUNCH(launch_fwd, k_fwd)
 SEIS_DEFINE_LAUNCH(launch_bwd, k_bwd)
+SEIS_DEFINE_LAUNCH(launch_fwd_batch, k_fwd_batch)
+SEIS_DEFINE_LAUNCH(launch_bwd_batch, k_bwd_batch)
 
 inline dim3 tile_grid(const Grid& g, int zdim) {
   return dim3((g.nx + kTX - 1) / kTX, (g.nz + kTZ - 1) / kTZ, zdim);
@@ -460,6 +658,250 @@ int check_grid(int64_t B, int64_t nz, int64_t nx, float dt, float h) {
                  (long long)nx);
   SEIS_CHECK_ARG(dt > 0.0f && h > 0.0f, "dt and h must be positive (dt=%g, h=%g)", dt, h);
   return 0;
+}
+
+
+// ------------------------------------------------- batch plan (host side)
+
+// Float bit pattern as a table entry.
+inline int64_t fbits(float v) {
+  int32_t b;
+  std::memcpy(&b, &v, sizeof b);
+  return (int64_t)b;
+}
+
+struct BatchPlan {
+  int n = 0;
+  std::vector<int64_t> tab;        // sorted rows, kBatchCols each
+  std::vector<int64_t> nt_sorted;  // nt of the sorted rows
+  std::vector<int64_t> tile_end;   // tiles of the first k sorted rows, k = 0..n
+  std::vector<int64_t> aux_end;    // receiver + ring work of the first k sorted rows
+  std::vector<int64_t> snap_off;   // per sorted row: offset into the snapshot arena
+  std::vector<int64_t> groups;     // per sorted row: shot groups
+  int64_t nt_max = 0, fields = 0, gparts = 0;
+};
+
+// Validates every instance and lays out the arenas in the caller's order;
+// rows are then sorted by nt (longest first, stable).
+int make_plan(BatchPlan& p, int64_t n, const int64_t* shape, const float* dt_h, int npml, int S,
+              int RZ, int VX, int32_t snap_every, bool need_ring) {
+  SEIS_CHECK_ARG(n >= 1 && n <= (1 << 20), "instance count %lld outside [1, 2^20]",
+                 (long long)n);
+  SEIS_CHECK_ARG(shape && dt_h, "null host pointer argument");
+  SEIS_CHECK_ARG(valid_shape(S, RZ, VX),
+                 "launch shape (shots_per_group %d, rows_per_thread %d, vec_x %d) not in "
+                 "{1,2,4,8} x {1,2,4} x {1,2,4}",
+                 S, RZ, VX);
+  p.n = (int)n;
+  std::vector<int64_t> rows((size_t)n * kBatchCols, 0);
+  int64_t o_model = 0, o_wav = 0, o_src = 0, o_rec = 0, o_out = 0, o_ring = 0, o_snap = 0,
+          o_field = 0, o_gpart = 0;
+  std::vector<int64_t> snap_off(n), groups(n);
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t nz = shape[5 * q], nx = shape[5 * q + 1], B = shape[5 * q + 2],
+                  nt = shape[5 * q + 3], R = shape[5 * q + 4];
+    const float dt = dt_h[2 * q], h = dt_h[2 * q + 1];
+    if (int e = check_grid(B, nz, nx, dt, h)) {
+      const std::string msg = seis_last_error();
+      seis::set_error("instance %lld: %s", (long long)q, msg.c_str());
+      return e;
+    }
+    SEIS_CHECK_ARG(nt >= 1 && R >= 0, "instance %lld: nt %lld, R %lld", (long long)q,
+                   (long long)nt, (long long)R);
+    SEIS_CHECK_ARG(nx % VX == 0, "instance %lld: vec_x %d does not divide nx = %lld",
+                   (long long)q, VX, (long long)nx);
+    const int64_t L = seis_ring_len(nz, nx, npml);
+    SEIS_CHECK_ARG(!need_ring || L > 0, "instance %lld: grid %lld x %lld has no interior "
+                   "inside npml=%d", (long long)q, (long long)nz, (long long)nx, npml);
+    const int64_t plane = nz * nx;
+    const int64_t G = (B + S - 1) / S;
+    const Grid g{(int)nz, (int)nx, npml, plane};
+    const TileMap tm = make_tiles(g, VX, RZ, (int)G);
+    const seis::TimeConsts k = seis::make_consts(dt, h);
+    int64_t* r = rows.data() + q * kBatchCols;
+    r[kCNz] = nz; r[kCNx] = nx; r[kCB] = B; r[kCNt] = nt; r[kCR] = R;
+    r[kCNtx] = tm.ntx; r[kCNtz] = tm.ntz;
+    r[kCField] = o_field; r[kCModel] = o_model; r[kCSrc] = o_src; r[kCRec] = o_rec;
+    r[kCOut] = o_out; r[kCWav] = o_wav; r[kCRing] = o_ring; r[kCRingLen] = need_ring && L > 0 ? L : 0;
+    r[kCGpart] = o_gpart;
+    r[kCIdt2] = fbits(k.idt2); r[kCIdt] = fbits(k.idt); r[kCTwoIdt2] = fbits(k.two_idt2);
+    r[kCIh2] = fbits(k.ih2);
+    snap_off[q] = o_snap;
+    groups[q] = G;
+    o_model += plane; o_wav += nt; o_src += B; o_rec += R; o_out += nt * R;
+    if (need_ring && nt > 1) o_ring += (nt - 1) * B * L;
+    o_snap += seis_snapshot_count(nt, snap_every) * 2 * B * plane;
+    o_field += B * plane; o_gpart += G * plane;
+  }
+  std::vector<int> order(n);
+  for (int q = 0; q < (int)n; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return rows[(size_t)a * kBatchCols + kCNt] > rows[(size_t)b * kBatchCols + kCNt];
+  });
+  p.tab.assign((size_t)n * kBatchCols, 0);
+  p.nt_sorted.resize(n); p.snap_off.resize(n); p.groups.resize(n);
+  p.tile_end.assign(n + 1, 0); p.aux_end.assign(n + 1, 0);
+  for (int k = 0; k < (int)n; ++k) {
+    int64_t* r = p.tab.data() + (size_t)k * kBatchCols;
+    std::memcpy(r, rows.data() + (size_t)order[k] * kBatchCols, sizeof(int64_t) * kBatchCols);
+    r[kCTile0] = p.tile_end[k];
+    r[kCAux0] = p.aux_end[k];
+    p.tile_end[k + 1] = p.tile_end[k] + r[kCNtx] * r[kCNtz] * groups[order[k]];
+    p.aux_end[k + 1] = p.aux_end[k] + r[kCR] + r[kCB] * r[kCRingLen];
+    p.nt_sorted[k] = r[kCNt];
+    p.snap_off[k] = snap_off[order[k]];
+    p.groups[k] = groups[order[k]];
+  }
+  SEIS_CHECK_ARG(p.tile_end[n] / kXcds < (int64_t)1 << 30, "too many tiles in one batch");
+  p.nt_max = p.nt_sorted[0];
+  p.fields = o_field;
+  p.gparts = o_gpart;
+  return 0;
+}
+
+int upload_plan(const BatchPlan& p, int64_t* table, hipStream_t s) {
+  SEIS_CHECK_HIP(hipMemcpyAsync(table, p.tab.data(), p.tab.size() * sizeof(int64_t),
+                                hipMemcpyHostToDevice, s));
+  // the host rows are released when the caller's plan goes out of scope
+  SEIS_CHECK_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Number of sorted rows still running at global step j (nt > j).
+inline int active_rows(const BatchPlan& p, int64_t j) {
+  return (int)(std::partition_point(p.nt_sorted.begin(), p.nt_sorted.end(),
+                                    [j](int64_t nt) { return nt > j; }) -
+               p.nt_sorted.begin());
+}
+
+struct BatchArgs {
+  const float *vp, *d, *wavelet;
+  const int32_t* src;
+  const int64_t* rec;
+  float* out;           // forward records
+  const float* rgrad;   // backward record cotangents
+  float* ring;
+  float* snaps;
+  int32_t K;
+  float *ua, *ub, *ma, *mb, *gpart, *grad;
+  const int64_t* table;
+  int npml, S, RZ, VX;
+};
+
+int enqueue_forward_batch(const BatchPlan& p, const BatchArgs& a, hipStream_t s, bool prof) {
+  if (int e = fill(a.ua, p.fields, 0.0f, s)) return e;
+  if (int e = fill(a.ub, p.fields, 0.0f, s)) return e;
+  for (int64_t n = 0; n < p.nt_max; ++n) {
+    const int na = active_rows(p, n);
+    const int64_t tiles = p.tile_end[na];
+    const int64_t per = (tiles + kXcds - 1) / kXcds;
+    if (prof) SEIS_PROF(SEIS_KERNEL_BATCH_FWD, s);
+    if (int e = launch_fwd_batch(a.S, a.RZ, a.VX, per * kXcds, s, a.table, na, tiles, (int)per,
+                                 a.npml, a.vp, a.d, a.ua, a.ub, a.src, a.wavelet, n))
+      return e;
+    if (prof) SEIS_PROF(SEIS_KERNEL_BATCH_FWD, s);
+    const int64_t work = p.aux_end[na];
+    if (work > 0) {
+      k_record_batch<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(
+          a.table, na, work, a.npml, a.ua, a.ub, a.rec, a.out, a.ring, n);
+      SEIS_CHECK_LAUNCH();
+    }
+    if (a.snaps && n >= a.K && n % a.K == 0) {
+      for (int k = 0; k < na; ++k) {
+        const int64_t* r = p.tab.data() + (size_t)k * kBatchCols;
+        const int64_t nt = r[kCNt];
+        if (n > nt - 2) continue;
+        const int64_t sz = r[kCB] * r[kCNz] * r[kCNx];
+        float* cur = (((n + nt) & 1) == 0 ? a.ua : a.ub) + r[kCField];
+        float* nxt = (((n + nt) & 1) == 0 ? a.ub : a.ua) + r[kCField];
+        float* slot = a.snaps + p.snap_off[k] + (n / a.K - 1) * 2 * sz;
+        if (int e = copy2(cur, slot, nxt, slot + sz, sz, s)) return e;
+      }
+    }
+  }
+  return 0;
+}
+
+int enqueue_backward_batch(const BatchPlan& p, const BatchArgs& a, hipStream_t s, bool prof) {
+  if (int e = fill(a.ma, p.fields, 0.0f, s)) return e;
+  if (int e = fill(a.mb, p.fields, 0.0f, s)) return e;
+  if (int e = fill(a.gpart, p.gparts, 0.0f, s)) return e;
+  for (int64_t j = 0; j < p.nt_max; ++j) {
+    const int na = active_rows(p, j);
+    const int64_t tiles = p.tile_end[na];
+    const int64_t per = (tiles + kXcds - 1) / kXcds;
+    if (prof) SEIS_PROF(SEIS_KERNEL_BATCH_BWD, s);
+    if (int e = launch_bwd_batch(a.S, a.RZ, a.VX, per * kXcds, s, a.table, na, tiles, (int)per,
+                                 a.npml, a.vp, a.d, a.ua, a.ub, a.ma, a.mb, a.gpart, a.src,
+                                 a.wavelet, j))
+      return e;
+    if (prof) SEIS_PROF(SEIS_KERNEL_BATCH_BWD, s);
+    const int64_t work = p.aux_end[na];
+    if (work > 0) {
+      k_post_batch<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(
+          a.table, na, work, a.npml, a.S, a.vp, a.d, a.ua, a.ub, a.ma, a.mb, a.gpart, a.rec,
+          a.rgrad, a.ring, j);
+      SEIS_CHECK_LAUNCH();
+    }
+    if (a.snaps) {
+      for (int k = 0; k < na; ++k) {
+        const int64_t* r = p.tab.data() + (size_t)k * kBatchCols;
+        const int64_t nt = r[kCNt], m = nt - j - 2;
+        if (m < a.K || m % a.K != 0 || m > nt - 2) continue;
+        const int64_t sz = r[kCB] * r[kCNz] * r[kCNx];
+        // (u^m, u^{m+1}) -> the buffers of u^{kk-2} = u^m and u^{kk-1}
+        float* um = (((m + nt) & 1) == 0 ? a.ua : a.ub) + r[kCField];
+        float* um1 = (((m + nt) & 1) == 0 ? a.ub : a.ua) + r[kCField];
+        const float* slot = a.snaps + p.snap_off[k] + (m / a.K - 1) * 2 * sz;
+        if (int e = copy2(slot, um, slot + sz, um1, sz, s)) return e;
+      }
+    }
+  }
+  for (int k = 0; k < p.n; ++k) {
+    const int64_t* r = p.tab.data() + (size_t)k * kBatchCols;
+    const int64_t plane = r[kCNz] * r[kCNx];
+    k_sum_groups<<<(unsigned)((plane + 255) / 256), 256, 0, s>>>(
+        a.gpart + r[kCGpart], (int)p.groups[k], a.grad + r[kCModel], plane);
+    SEIS_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+// Runs `enqueue` directly on s, or records it into a graph on a private
+// capture stream and launches that graph on s (then waits for it).
+template <typename F>
+int run_maybe_graph(bool use_graph, hipStream_t s, F enqueue) {
+  if (!use_graph) return enqueue(s, true);
+  hipStream_t cs = nullptr;
+  SEIS_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  int rc = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipError_t he = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (he != hipSuccess) {
+    seis::set_error("hipStreamBeginCapture failed: %s", hipGetErrorString(he));
+    rc = 2;
+  } else {
+    rc = enqueue(cs, false);
+    he = hipStreamEndCapture(cs, &graph);
+    if (rc == 0 && he != hipSuccess) {
+      seis::set_error("hipStreamEndCapture failed: %s", hipGetErrorString(he));
+      rc = 2;
+    }
+  }
+  if (rc == 0) {
+    he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (he == hipSuccess) he = hipGraphLaunch(exec, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) {
+      seis::set_error("batch graph failed: %s", hipGetErrorString(he));
+      rc = 2;
+    }
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipStreamDestroy(cs);
+  return rc;
 }
 
 }  // namespace
@@ -549,8 +991,8 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   const int64_t aux = R + ((ring && nt > 1) ? B * ring_len : 0);
   for (int64_t n = 0; n < nt; ++n) {
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
-    if (int e = launch_fwd(S, rows_per_thread, vec_x, tm, s, vp, d, cur, prev, src_zx,
-                           wavelet + n, (int)B, g, k))
+    if (int e = launch_fwd(S, rows_per_thread, vec_x, (int64_t)tm.per * kXcds, s, vp, d, cur,
+                           prev, src_zx, wavelet + n, (int)B, g, k, tm))
       return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
@@ -615,8 +1057,8 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   for (int64_t kk = nt; kk >= 1; --kk) {
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
-    if (int e = launch_bwd(S, rows_per_thread, vec_x, tm, s, vp, d, X, Y, M1, M2, grad_part, src_zx,
-                           wavelet + (kk - 1), recon, (int)B, g, k))
+    if (int e = launch_bwd(S, rows_per_thread, vec_x, (int64_t)tm.per * kXcds, s, vp, d, X, Y, M1,
+                           M2, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, g, k, tm))
       return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
@@ -639,6 +1081,71 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   k_sum_groups<<<(unsigned)((g.plane + 255) / 256), 256, 0, s>>>(grad_part, G, grad, g.plane);
   SEIS_CHECK_LAUNCH();
   return 0;
+}
+
+int seis_acoustic_forward_batch(int64_t n, const int64_t* shape, const float* dt_h,
+                                const float* vp, const float* d, const float* wavelet,
+                                const int32_t* src_zx, const int64_t* rec_flat, float* rec_out,
+                                float* ring, float* snaps, int32_t snap_every, float* u_a,
+                                float* u_b, int64_t* table, int32_t npml,
+                                int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
+                                int32_t use_graph, void* stream) {
+  BatchPlan p;
+  if (int e = make_plan(p, n, shape, dt_h, npml, shots_per_group, rows_per_thread, vec_x,
+                        snaps ? snap_every : 0, ring != nullptr))
+    return e;
+  SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b && table, "null pointer argument");
+  bool any_r = false;
+  for (int k = 0; k < p.n; ++k) any_r = any_r || p.tab[(size_t)k * kBatchCols + kCR] > 0;
+  SEIS_CHECK_ARG(!any_r || (rec_flat && rec_out), "receivers need rec_flat and rec_out");
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  SEIS_CHECK_ARG(aligned(vp, 4 * vec_x) && aligned(d, 4 * vec_x) && aligned(u_a, 4 * vec_x) &&
+                     aligned(u_b, 4 * vec_x),
+                 "field and model pointers must be %d-byte aligned for vec_x %d", 4 * vec_x,
+                 vec_x);
+  hipStream_t s = (hipStream_t)stream;
+  if (int e = upload_plan(p, table, s)) return e;
+  BatchArgs a{vp, d, wavelet, src_zx, rec_flat, rec_out, nullptr, ring, snaps, snap_every,
+              u_a, u_b, nullptr, nullptr, nullptr, nullptr, table, npml, shots_per_group,
+              rows_per_thread, vec_x};
+  return run_maybe_graph(use_graph != 0, s, [&](hipStream_t st, bool prof) {
+    return enqueue_forward_batch(p, a, st, prof);
+  });
+}
+
+int seis_acoustic_backward_batch(int64_t n, const int64_t* shape, const float* dt_h,
+                                 const float* vp, const float* d, const float* wavelet,
+                                 const int32_t* src_zx, const int64_t* rec_flat,
+                                 const float* rec_grad, const float* ring, const float* snaps,
+                                 int32_t snap_every, float* u_a, float* u_b, float* mu_a,
+                                 float* mu_b, float* grad_part, float* grad, int64_t* table,
+                                 int32_t npml, int32_t shots_per_group, int32_t rows_per_thread,
+                                 int32_t vec_x, int32_t use_graph, void* stream) {
+  BatchPlan p;
+  if (int e = make_plan(p, n, shape, dt_h, npml, shots_per_group, rows_per_thread, vec_x,
+                        snaps ? snap_every : 0, true))
+    return e;
+  SEIS_CHECK_ARG(vp && d && wavelet && src_zx && u_a && u_b && mu_a && mu_b && grad_part &&
+                     grad && table,
+                 "null pointer argument");
+  SEIS_CHECK_ARG(ring || p.nt_max <= 2, "ring storage is required for nt > 2");
+  bool any_r = false;
+  for (int k = 0; k < p.n; ++k) any_r = any_r || p.tab[(size_t)k * kBatchCols + kCR] > 0;
+  SEIS_CHECK_ARG(!any_r || (rec_flat && rec_grad), "receivers need rec_flat and rec_grad");
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  SEIS_CHECK_ARG(aligned(vp, 4 * vec_x) && aligned(d, 4 * vec_x) && aligned(u_a, 4 * vec_x) &&
+                     aligned(u_b, 4 * vec_x) && aligned(mu_a, 4 * vec_x) &&
+                     aligned(mu_b, 4 * vec_x) && aligned(grad_part, 4 * vec_x),
+                 "field and model pointers must be %d-byte aligned for vec_x %d", 4 * vec_x,
+                 vec_x);
+  hipStream_t s = (hipStream_t)stream;
+  if (int e = upload_plan(p, table, s)) return e;
+  BatchArgs a{vp, d, wavelet, src_zx, rec_flat, nullptr, rec_grad, const_cast<float*>(ring),
+              const_cast<float*>(snaps), snap_every, u_a, u_b, mu_a, mu_b, grad_part, grad,
+              table, npml, shots_per_group, rows_per_thread, vec_x};
+  return run_maybe_graph(use_graph != 0, s, [&](hipStream_t st, bool prof) {
+    return enqueue_backward_batch(p, a, st, prof);
+  });
 }
 
 }  // extern "C"
