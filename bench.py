@@ -24,8 +24,10 @@ METRIC = "Gcell-updates/s fwd+adjoint, % HBM roofline; wall-clock to target misf
 HBM_PEAK = 8.0e12  # B/s, MI355X
 # algorithmic bytes per cell-update of each timed kernel (fields touched once
 # per step, damping/coefficients read once per cell)
-BYTES_PER_CELL = {"k_fwd": 16.0, "k_bwd": 36.0, "k_el_fwd": 52.0, "k_el_bwd": 116.0}
-KERNEL_IDS = {"acoustic": (("k_fwd", 0), ("k_bwd", 1)), "elastic": (("k_el_fwd", 2), ("k_el_bwd", 3))}
+BYTES_PER_CELL = {"k_fwd": 16.0, "k_bwd": 36.0, "k_el_fwd": 52.0, "k_el_bwd": 116.0,
+                  "k_fwd_batch": 16.0, "k_bwd_batch": 36.0}
+KERNEL_IDS = {"acoustic": (("k_fwd", 0), ("k_bwd", 1)), "elastic": (("k_el_fwd", 2), ("k_el_bwd", 3)),
+              "generator": (("k_fwd_batch", 6), ("k_bwd_batch", 7))}
 
 
 def parse():
@@ -38,9 +40,17 @@ def parse():
                    help="time steps of the bounded CPU sample (1 shot, full grid)")
     p.add_argument("--nt", type=int, default=None,
                    help="shorter time axis for profiler runs (the metric is quoted at full nt)")
-    p.add_argument("--workload", choices=["acoustic", "elastic"], default="acoustic",
+    p.add_argument("--workload", choices=["acoustic", "elastic", "generator"],
+                   default="acoustic",
                    help="acoustic: 64-shot 800x2400 FWI gradient (the headline); elastic: "
-                        "32-shot 600x1800 vp/vs/rho gradient")
+                        "32-shot 600x1800 vp/vs/rho gradient; generator: 256 uneven seeded "
+                        "models, records + misfit gradient in one batch")
+    p.add_argument("--models", type=int, default=256, help="generator: number of models")
+    p.add_argument("--graph", type=int, default=0,
+                   help="generator: 1 records each batched loop into one hipGraph (kernel "
+                        "timing events are then unavailable)")
+    p.add_argument("--snapshot-every", type=int, default=500,
+                   help="generator: field-pair snapshot spacing of the batched adjoint")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a plain 1 GiB device copy (HBM-counter calibration)")
     return p.parse_args()
@@ -139,6 +149,99 @@ def cpu_baseline(P, steps):
                       f"{el:.1f} s"}
 
 
+def run_generator(args, dev):
+    """Uneven batch: 256 seeded models (grids 64..512 x 64..768, 4-16 shots,
+    nt = 1.2 x diagonal traveltime).  One step = records of every true model
+    (batched forward) + the L2-misfit gradient of every start model (batched
+    forward + adjoint).  value = fwd+adjoint cell-updates of the gradient pass
+    / its wall time, as for the single-model workloads."""
+    import ctypes
+    import torch
+    from seistorch_amd import _lib, autotune, generator
+    t0 = time.perf_counter()
+    spec = generator.GenSpec(n_models=args.models)
+    host = generator.make_instances(spec=spec)
+    if args.nt:
+        for it in host:
+            it["nt"] = min(it["nt"], args.nt)
+            it["wavelet"] = it["wavelet"][:it["nt"]]
+    insts = generator.to_device(host, dev)
+    cells = generator.cell_updates(host)
+    print(f"[bench] generator: {len(host)} models, {cells:.3e} cell-updates per propagation, "
+          f"built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    shape, tune = autotune.tune_batch(generator.batch_instances(insts, "vp_start"))
+    launch = dict(shots_per_group=shape[0], rows_per_thread=shape[1], vec_x=shape[2])
+    print(f"[bench] autotuned batch launch shape {shape}", file=sys.stderr, flush=True)
+    use_graph = bool(args.graph)
+    lib = _lib.load()
+
+    def step():
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        obs = generator.observed(insts, use_graph=use_graph, **launch)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        loss, grads = generator.misfit_gradients(insts, obs, use_graph=use_graph,
+                                                 snapshot_every=args.snapshot_every, **launch)
+        torch.cuda.synchronize()
+        c = time.perf_counter()
+        return b - a, c - b, loss
+
+    for i in range(args.warmup):
+        step()
+        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    lib.seis_profile_enable(0 if use_graph else 1)
+    t_fwd = t_grad = 0.0
+    for i in range(args.steps):
+        f, g, loss = step()
+        t_fwd += f
+        t_grad += g
+        print(f"[bench] step {i + 1}/{args.steps}: records {f:.2f} s, gradient {g:.2f} s",
+              file=sys.stderr, flush=True)
+    lib.seis_profile_enable(0)
+    kern = {}
+    for name, kid in KERNEL_IDS["generator"]:
+        ms = ctypes.c_double(0.0)
+        cnt = ctypes.c_int64(0)
+        _lib.call("seis_profile_collect", kid, ctypes.byref(ms), ctypes.byref(cnt))
+        kern[name] = (ms.value, cnt.value)
+    value = cells * args.steps / t_grad / 1e9
+    roof = None
+    if not use_graph:
+        # per launch the batched kernels cover every running instance; the
+        # algorithmic bytes of all timed launches over their summed duration
+        tot_ms, n_launch = kern["k_bwd_batch"]
+        achieved = BYTES_PER_CELL["k_bwd_batch"] * cells * args.steps / (tot_ms / 1e3)
+        roof = {"bound": "hbm", "kernel": "k_bwd_batch", "achieved": round(achieved / 1e9, 1),
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                "traffic": None, "avg_launch_us": round(tot_ms / max(n_launch, 1) * 1e3, 2),
+                "launches": n_launch, "bytes_per_cell_update": BYTES_PER_CELL["k_bwd_batch"]}
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round((t_fwd + t_grad) / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded layered/faulted/anomaly models, records from the true models)",
+        "config": {"workload": "uneven batch: records + L2-misfit gradient of every model, "
+                               "PML 50, one launch per step over all running models",
+                   "models": len(host), "cell_updates_per_propagation": cells,
+                   "nt_range": [min(i["nt"] for i in host), max(i["nt"] for i in host)],
+                   "shots_range": [min(i["B"] for i in host), max(i["B"] for i in host)],
+                   "hipgraph": use_graph, "snapshot_every": args.snapshot_every},
+        "records_gcells_per_s": round(cells * args.steps / t_fwd / 1e9, 3),
+        "gradient_s_per_step": round(t_grad / args.steps, 3),
+        "records_s_per_step": round(t_fwd / args.steps, 3),
+        "roofline": roof,
+        "kernels_ms_total": {k: round(v[0], 2) for k, v in kern.items()},
+        "launch_shape": {"shots_per_group": shape[0], "rows_per_thread": shape[1],
+                         "vec_x": shape[2],
+                         "autotune_ms": {f"S{a}_rz{b}_vx{c}": round(v, 2)
+                                         for (a, b, c), v in tune.items()}},
+        "loss_sum": float(loss.sum()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -148,6 +251,8 @@ def main():
     import torch
     from seistorch_amd import _lib
     dev = torch.device("cuda:0")
+    if args.workload == "generator":
+        return run_generator(args, dev)
     elastic = args.workload == "elastic"
     if elastic:
         P = build_elastic_problem(dev, nt_override=args.nt)

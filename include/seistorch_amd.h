@@ -265,6 +265,61 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
                           int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
                           void* stream);
 
+/* ---------------- learned gradient preconditioner (U-Net) ----------------- */
+/* No reference counterpart: a small U-Net maps (gradient, model) to an
+ * update direction inside FWI; the reference's plain-gradient step
+ * (seistorch/fwi.py) is its fallback.  Activations are NHWC; bf16 tensors are
+ * raw uint16 bit patterns (16-byte aligned); accumulation is FP32 on the
+ * matrix cores. */
+
+/* 3x3 convolution, zero padding 1: x (N,H,W,Cin) bf16, Cin a multiple of 8;
+ * w (Cout, 9*Cin) bf16 with k = tap*Cin + ci, tap = 3*(dy+1) + (dx+1);
+ * bias (Cout) FP32 or NULL.  Either y (bf16, = act(acc + bias), act 1 =
+ * GELU(erf), optional z = acc + bias in bf16) or y_f32 (= acc + bias). */
+int seis_unet_conv3x3(const uint16_t* x, int32_t N, int32_t H, int32_t W, int32_t Cin,
+                      const uint16_t* w, const float* bias, int32_t Cout, int32_t act,
+                      uint16_t* z, uint16_t* y, float* y_f32, void* stream);
+/* wt (Cin, 9*Cout) = w with taps flipped and channel roles swapped: the data
+ * gradient of a convolution is seis_unet_conv3x3 of its output gradient with wt. */
+int seis_unet_flip_weights(const uint16_t* w, int32_t Cout, int32_t Cin, uint16_t* wt,
+                           void* stream);
+/* dw (Cout, 9*Cin) += weight gradient; db (Cout) += bias gradient (may be NULL). */
+int seis_unet_conv3x3_wgrad(const uint16_t* x, const uint16_t* dz, int32_t N, int32_t H,
+                            int32_t W, int32_t Cin, int32_t Cout, float* dw, float* db,
+                            void* stream);
+/* dz = bf16(dy * GELU'(z)). */
+int seis_unet_gelu_bwd(const float* dy, const uint16_t* z, int64_t n, uint16_t* dz, void* stream);
+int seis_unet_to_bf16(const float* x, int64_t n, uint16_t* y, void* stream);
+/* 2x2 average pooling and its gradient (dx = or += 0.25*dy[pixel/2]). */
+int seis_unet_avgpool2(const uint16_t* x, int32_t N, int32_t H, int32_t W, int32_t C, uint16_t* y,
+                       void* stream);
+int seis_unet_avgpool2_bwd(const float* dy, int32_t N, int32_t H, int32_t W, int32_t C, float* dx,
+                           int32_t accumulate, void* stream);
+/* out (N,H,W,C1+C2) = [nearest 2x upsampling of low (N,H/2,W/2,C1), skip (N,H,W,C2)]
+ * and its gradient (dlow =, dskip = or +=). */
+int seis_unet_up_concat(const uint16_t* low, const uint16_t* skip, int32_t N, int32_t H,
+                        int32_t W, int32_t C1, int32_t C2, uint16_t* out, void* stream);
+int seis_unet_up_concat_bwd(const float* dout, int32_t N, int32_t H, int32_t W, int32_t C1,
+                            int32_t C2, float* dlow, float* dskip, int32_t accumulate_skip,
+                            void* stream);
+/* Network input (N,H,W,8) bf16: channel 0 = grad*scales[3n], channel 1 =
+ * (model - scales[3n+2])*scales[3n+1], channels 2..7 zero. */
+int seis_unet_pack_input(const float* grad, const float* model, int32_t N, int32_t H, int32_t W,
+                         const float* scales, uint16_t* out, void* stream);
+/* 1x1 head to one FP32 channel and its gradient (dw, db accumulate). */
+int seis_unet_head(const uint16_t* x, int64_t P, int32_t C, const float* w, const float* b,
+                   float* out, void* stream);
+int seis_unet_head_bwd(const uint16_t* x, const float* dout, int64_t P, int32_t C, const float* w,
+                       float* dx, float* dw, float* db, void* stream);
+/* loss[0] = sum(mask*(pred-target))^2 * inv_count; dpred = its gradient. */
+int seis_unet_mse(const float* pred, const float* target, const float* mask, int64_t n,
+                  float inv_count, float* loss, float* dpred, void* stream);
+/* AdamW step `step` (>= 1) on FP32 master weights, refreshing param_bf16 if not NULL. */
+int seis_unet_adamw(float* param, const float* grad, float* m, float* v, int64_t n, float lr,
+                    float beta1, float beta2, float eps, float weight_decay, int32_t step,
+                    uint16_t* param_bf16, void* stream);
+int seis_unet_fill(float* p, int64_t n, float v, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,24 @@ SIGNATURES = {
     "seis_elastic_backward": (ctypes.c_int, [_p, _p, _i64, _p, _i32, _p, _i64, _p, _i32, _p, _p,
                                              _p, _i32, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
                                              _i32, _i32, _p]),
+    "seis_unet_conv3x3": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _p, _p, _i32, _i32, _p, _p,
+                                         _p, _p]),
+    "seis_unet_flip_weights": (ctypes.c_int, [_p, _i32, _i32, _p, _p]),
+    "seis_unet_conv3x3_wgrad": (ctypes.c_int, [_p, _p, _i32, _i32, _i32, _i32, _i32, _p, _p,
+                                               _p]),
+    "seis_unet_gelu_bwd": (ctypes.c_int, [_p, _p, _i64, _p, _p]),
+    "seis_unet_to_bf16": (ctypes.c_int, [_p, _i64, _p, _p]),
+    "seis_unet_avgpool2": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _p, _p]),
+    "seis_unet_avgpool2_bwd": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _p, _i32, _p]),
+    "seis_unet_up_concat": (ctypes.c_int, [_p, _p, _i32, _i32, _i32, _i32, _i32, _p, _p]),
+    "seis_unet_up_concat_bwd": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _i32, _p, _p, _i32,
+                                               _p]),
+    "seis_unet_pack_input": (ctypes.c_int, [_p, _p, _i32, _i32, _i32, _p, _p, _p]),
+    "seis_unet_head": (ctypes.c_int, [_p, _i64, _i32, _p, _p, _p, _p]),
+    "seis_unet_head_bwd": (ctypes.c_int, [_p, _p, _i64, _i32, _p, _p, _p, _p, _p]),
+    "seis_unet_mse": (ctypes.c_int, [_p, _p, _p, _i64, _f, _p, _p, _p]),
+    "seis_unet_adamw": (ctypes.c_int, [_p, _p, _p, _p, _i64, _f, _f, _f, _f, _f, _i32, _p, _p]),
+    "seis_unet_fill": (ctypes.c_int, [_p, _i64, _f, _p]),
 }
 
 ABI_VERSION = 1

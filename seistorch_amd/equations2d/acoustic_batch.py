@@ -154,11 +154,14 @@ class _BatchLoop(torch.autograd.Function):
             ctx.save_for_backward(vp_arena, d_arena, wav_arena, src_arena, rec_arena)
             ctx.state = (ring, snaps, u_a, u_b, table)
             ctx.meta = (sizes, dts, launch, K if snaps is not None else 0, use_graph, lay)
-        return rec
+        # one output per instance, so the backward receives each cotangent once
+        return tuple(rec[o:o + nt * R].view(nt, R)
+                     for o, (_, _, _, nt, R) in zip(lay.off["out"], sizes))
 
     @staticmethod
     @torch.autograd.function.once_differentiable
-    def backward(ctx, grad_rec):
+    def backward(ctx, *grad_recs):
+        grad_rec = torch.cat([g.reshape(-1) for g in grad_recs])
         vp_arena, d_arena, wav_arena, src_arena, rec_arena = ctx.saved_tensors
         ring, snaps, u_a, u_b, table = ctx.state
         ctx.state = None  # the backward consumes the final forward state
@@ -212,10 +215,5 @@ def propagate_batch(insts, shots_per_group=None, rows_per_thread=None, vec_x=Non
         wav_arena = torch.cat([it.wavelet.reshape(-1) for it in insts])
         src_arena = torch.cat([it.src_zx.reshape(-1) for it in insts])
         rec_arena = torch.cat([it.rec_flat.reshape(-1) for it in insts])
-    rec = _BatchLoop.apply(vp_arena, d_arena, wav_arena, src_arena, rec_arena, sizes, dts,
-                           launch, snapshot_every, bool(use_graph))
-    out, o = [], 0
-    for nz, nx, B, nt, R in sizes:
-        out.append(rec[o:o + nt * R].view(nt, R))
-        o += nt * R
-    return out
+    return list(_BatchLoop.apply(vp_arena, d_arena, wav_arena, src_arena, rec_arena, sizes,
+                                 dts, launch, snapshot_every, bool(use_graph)))

@@ -1,0 +1,173 @@
+"""Single-GPU acoustic FWI driver: plain-gradient iterations and iterations
+preconditioned by the learned U-Net, compared at equal wall-clock time.
+
+The plain method follows the reference's FWI loop (seistorch/fwi.py and
+seistorch_dist.py:166-303: forward, L2 misfit of loss.py:409-421, backward,
+model update) with a steepest-descent step normalised to a velocity change
+and accepted only if the misfit decreases (otherwise halved).  The gradient
+is muted within `mute` cells of the surface, as the reference's seabed cut
+(seistorch/process.py:43-47) does, and only the unpadded model is updated;
+the boundary strips are re-padded from its edges (geom.py:213-240).
+
+The preconditioned method feeds (gradient, model) to the network, which
+predicts the model perturbation; the step along it is accepted if the
+misfit decreases (tried at full and half length), otherwise the plain step
+of the same iteration is taken (the fallback).
+"""
+import time
+from dataclasses import dataclass, field
+
+import torch
+import torch.nn.functional as F
+
+from .equations2d import acoustic as ac
+
+PAD = 50
+
+
+@dataclass
+class Problem:
+    vp_true: torch.Tensor   # padded (nz, nx)
+    vp_init: torch.Tensor   # padded (nz, nx)
+    d: torch.Tensor
+    wavelet: torch.Tensor
+    src: torch.Tensor
+    rec: torch.Tensor
+    B: int
+    dt: float
+    h: float
+    obs: torch.Tensor = None
+    mute: int = 5
+    vmin: float = 1400.0
+    vmax: float = 5000.0
+
+    def interior(self, v):
+        return v[PAD:v.shape[0] - PAD, PAD:v.shape[1] - PAD]
+
+    def pad(self, vi):
+        return F.pad(vi[None, None], (PAD, PAD, PAD, PAD), mode="replicate")[0, 0].contiguous()
+
+    def records(self, vp):
+        with torch.no_grad():
+            return ac.propagate(vp, self.d, self.wavelet, self.src, self.rec, self.B, self.dt,
+                                self.h)
+
+    def misfit(self, vp):
+        with torch.no_grad():
+            return float(torch.sum((self.records(vp) - self.obs) ** 2))
+
+    def gradient(self, vp):
+        v = vp.detach().clone().requires_grad_(True)
+        rec = ac.propagate(v, self.d, self.wavelet, self.src, self.rec, self.B, self.dt, self.h)
+        loss = torch.sum((rec - self.obs) ** 2)
+        loss.backward()
+        g = self.interior(v.grad).clone()
+        g[:self.mute] = 0.0
+        return float(loss), g
+
+    def rmse(self, vp):
+        return float(torch.sqrt(torch.mean((self.interior(vp) - self.interior(self.vp_true)) ** 2)))
+
+
+@dataclass
+class History:
+    t: list = field(default_factory=list)
+    misfit: list = field(default_factory=list)
+    rmse: list = field(default_factory=list)
+    kind: list = field(default_factory=list)
+
+    def add(self, t, misfit, rmse, kind):
+        self.t.append(t); self.misfit.append(misfit); self.rmse.append(rmse); self.kind.append(kind)
+
+
+def _clip(p, vi):
+    return vi.clamp(p.vmin, p.vmax)
+
+
+def plain_step(p, vp, loss, g, step):
+    """Steepest descent: dv = -step * g / max|g|; halve until the misfit drops
+    (at most 3 tries).  Returns (vp, misfit, step, accepted)."""
+    gmax = float(g.abs().max())
+    if gmax == 0.0:
+        return vp, loss, step, False
+    vi = p.interior(vp)
+    for _ in range(3):
+        trial = p.pad(_clip(p, vi - step * g / gmax))
+        m = p.misfit(trial)
+        if m < loss:
+            return trial, m, step, True
+        step *= 0.5
+    return vp, loss, step, False
+
+
+def run_plain(p, iters, step0=40.0, time_budget=None, log=None):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vp = p.vp_init.clone()
+    hist = History()
+    hist.add(0.0, p.misfit(vp), p.rmse(vp), "start")
+    step = step0
+    for it in range(iters):
+        loss, g = p.gradient(vp)
+        vp, m, step, ok = plain_step(p, vp, loss, g, step)
+        if ok:
+            step *= 1.25
+        torch.cuda.synchronize()
+        hist.add(time.perf_counter() - t0, m, p.rmse(vp), "plain" if ok else "none")
+        if log:
+            log(f"[plain] it {it + 1}: misfit {m:.4e} rmse {hist.rmse[-1]:.1f} step {step:.1f}")
+        if time_budget and hist.t[-1] >= time_budget:
+            break
+    return vp, hist
+
+
+def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, time_budget=None, log=None):
+    from . import precond
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vp = p.vp_init.clone()
+    hist = History()
+    hist.add(0.0, p.misfit(vp), p.rmse(vp), "start")
+    step = step0
+    for it in range(iters):
+        loss, g = p.gradient(vp)
+        vi = p.interior(vp).contiguous()
+        sc = precond.input_scales(g[None], vi[None])
+        d = net.forward(g[None].contiguous(), vi[None], sc)[0] * vscale
+        d[:p.mute] = 0.0
+        kind, m = "none", loss
+        for a in (1.0, 0.5):
+            trial = p.pad(_clip(p, vi + a * d))
+            mt = p.misfit(trial)
+            if mt < loss:
+                vp, m, kind = trial, mt, f"net{a:g}"
+                break
+        if kind == "none":  # fallback: the plain step of this iteration
+            vp, m, step, ok = plain_step(p, vp, loss, g, step)
+            kind = "plain" if ok else "none"
+            if ok:
+                step *= 1.25
+        torch.cuda.synchronize()
+        hist.add(time.perf_counter() - t0, m, p.rmse(vp), kind)
+        if log:
+            log(f"[precond] it {it + 1}: {kind} misfit {m:.4e} rmse {hist.rmse[-1]:.1f}")
+        if time_budget and hist.t[-1] >= time_budget:
+            break
+    return vp, hist
+
+
+def misfit_at(hist, t):
+    """Best misfit reached by wall-clock time t."""
+    best = None
+    for ti, m in zip(hist.t, hist.misfit):
+        if ti <= t:
+            best = m if best is None else min(best, m)
+    return best
+
+
+def time_to(hist, target):
+    """Wall-clock time at which the misfit first reaches `target` (None if never)."""
+    for ti, m in zip(hist.t, hist.misfit):
+        if m <= target:
+            return ti
+    return None

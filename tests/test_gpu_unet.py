@@ -1,0 +1,170 @@
+"""Parity of the U-Net kernels (BF16 MFMA) with the FP32 CPU oracle.
+
+Tolerances: single convolutions compare FP32 accumulations of the same bf16
+products (rel-L2 1e-5; a bf16-stored output adds its rounding, <= 2^-8
+relative per element).  The whole network stores every activation in bf16 on
+both sides, so the forward output agrees to REL_FWD; its parameter
+gradients differ by the bf16 rounding of the activation gradients of the HIP
+path (the oracle keeps them in FP32), bounded by REL_GRAD.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import unet as ou
+from tests.cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+REL_FWD = 1e-2
+REL_GRAD = 5e-2
+
+
+def bf(t):
+    """FP32 tensor -> bf16 bit patterns (int16), round to nearest even."""
+    return t.to(torch.bfloat16).view(torch.int16)
+
+
+def unbf(t):
+    return t.view(torch.bfloat16).float()
+
+
+def _call(name, *a):
+    from seistorch_amd import _lib
+    _lib.call(name, *a)
+
+
+@pytest.mark.parametrize("N,H,W,cin,cout", [(2, 5, 70, 8, 32), (1, 9, 64, 32, 64),
+                                            (1, 6, 33, 96, 32), (2, 4, 130, 64, 96)])
+def test_conv3x3_forward(N, H, W, cin, cout):
+    from seistorch_amd import _lib
+    g = torch.Generator().manual_seed(N * 1000 + cin + cout)
+    x = torch.randn((N, H, W, cin), generator=g)
+    w = torch.randn((cout, 9 * cin), generator=g) * 0.1
+    b = torch.randn(cout, generator=g)
+    xb, wb = bf(x), bf(w)
+    ref = torch.nn.functional.conv2d(unbf(xb).permute(0, 3, 1, 2),
+                                     unbf(wb).reshape(cout, 3, 3, cin).permute(0, 3, 1, 2), b,
+                                     padding=1).permute(0, 2, 3, 1)
+    d = {k: v.cuda() for k, v in dict(x=xb, w=wb, b=b).items()}
+    z = torch.empty((N, H, W, cout), dtype=torch.int16, device="cuda")
+    y = torch.empty_like(z)
+    yf = torch.empty((N, H, W, cout), device="cuda")
+    s = _lib.stream_of(d["b"])
+    _call("seis_unet_conv3x3", _lib.ptr(d["x"]), N, H, W, cin, _lib.ptr(d["w"]), _lib.ptr(d["b"]),
+          cout, 1, _lib.ptr(z), _lib.ptr(y), None, s)
+    _call("seis_unet_conv3x3", _lib.ptr(d["x"]), N, H, W, cin, _lib.ptr(d["w"]), _lib.ptr(d["b"]),
+          cout, 0, None, None, _lib.ptr(yf), s)
+    torch.cuda.synchronize()
+    e = rel_l2(yf.cpu().numpy(), ref.numpy())
+    print(f"conv fp32 out rel-L2 {e:.2e}")
+    assert e <= 1e-5
+    zc = unbf(z.cpu())
+    assert torch.all((zc - ref).abs() <= 2 ** -8 * ref.abs() + 1e-6)
+    yref = torch.nn.functional.gelu(ref)
+    assert torch.all((unbf(y.cpu()) - yref).abs() <= 2 ** -8 * yref.abs() + 1e-5)
+
+
+@pytest.mark.parametrize("N,H,W,cin,cout", [(2, 5, 70, 8, 32), (1, 8, 64, 32, 64),
+                                            (2, 6, 40, 96, 32), (1, 4, 96, 64, 128)])
+def test_conv3x3_weight_and_data_gradients(N, H, W, cin, cout):
+    from seistorch_amd import _lib
+    g = torch.Generator().manual_seed(7 + cin + cout)
+    x = unbf(bf(torch.randn((N, H, W, cin), generator=g)))
+    w = unbf(bf(torch.randn((cout, 9 * cin), generator=g) * 0.1))
+    dz = unbf(bf(torch.randn((N, H, W, cout), generator=g)))
+    xt = x.permute(0, 3, 1, 2).clone().requires_grad_(True)
+    wt = w.reshape(cout, 3, 3, cin).permute(0, 3, 1, 2).clone().requires_grad_(True)
+    out = torch.nn.functional.conv2d(xt, wt, None, padding=1)
+    out.backward(dz.permute(0, 3, 1, 2))
+    dw_ref = wt.grad.permute(0, 2, 3, 1).reshape(cout, 9 * cin)
+    dx_ref = xt.grad.permute(0, 2, 3, 1)
+    db_ref = dz.sum((0, 1, 2))
+    xd, wd, dzd = bf(x).cuda(), bf(w).cuda(), bf(dz).cuda()
+    dw = torch.zeros((cout, 9 * cin), device="cuda")
+    db = torch.zeros(cout, device="cuda")
+    s = _lib.stream_of(dw)
+    _call("seis_unet_conv3x3_wgrad", _lib.ptr(xd), _lib.ptr(dzd), N, H, W, cin, cout,
+          _lib.ptr(dw), _lib.ptr(db), s)
+    wtd = torch.empty((cin, 9 * cout), dtype=torch.int16, device="cuda")
+    _call("seis_unet_flip_weights", _lib.ptr(wd), cout, cin, _lib.ptr(wtd), s)
+    dx = torch.empty((N, H, W, cin), device="cuda")
+    _call("seis_unet_conv3x3", _lib.ptr(dzd), N, H, W, cout, _lib.ptr(wtd), None, cin, 0, None,
+          None, _lib.ptr(dx), s)
+    torch.cuda.synchronize()
+    for name, a, r in (("dw", dw, dw_ref), ("db", db, db_ref), ("dx", dx, dx_ref)):
+        e = rel_l2(a.cpu().numpy(), r.detach().numpy())
+        print(f"{name} rel-L2 {e:.2e}")
+        assert e <= 1e-5, (name, e)
+
+
+def _net(base=32, seed=0):
+    from seistorch_amd import precond
+    p = precond.Params("cuda", base=base, seed=seed)
+    return p, precond.UNet(p)
+
+
+def _inputs(N, H, W, seed):
+    g = torch.Generator().manual_seed(seed)
+    grad = torch.randn((N, H, W), generator=g) * 1e-3
+    model = 3000 + 500 * torch.randn((N, H, W), generator=g)
+    tgt = torch.randn((N, H, W), generator=g) * 0.1
+    return grad, model, tgt
+
+
+def test_network_forward_and_gradients_match_oracle():
+    from seistorch_amd import precond
+    N, H, W = 2, 32, 48
+    p, net = _net()
+    grad, model, tgt = _inputs(N, H, W, 3)
+    scales = precond.input_scales(grad.cuda(), model.cuda())
+    out = net.forward(grad.cuda(), model.cuda(), scales, keep=True)
+    loss, dout = precond.mse(out, tgt.cuda())
+    gpu_grad = torch.zeros_like(p.master)
+    net.backward(dout, gpu_grad)
+    torch.cuda.synchronize()
+    master = p.master.detach().cpu().clone().requires_grad_(True)
+    ref = ou.forward(master, p.specs, p.off, p.base, grad, model, scales.cpu())
+    ((ref - tgt) ** 2).mean().backward()
+    e_out = rel_l2(out.cpu().numpy(), ref.detach().numpy())
+    e_loss = abs(float(loss) / float(((ref - tgt) ** 2).mean()) - 1)
+    e_grad = rel_l2(gpu_grad.cpu().numpy(), master.grad.numpy())
+    print(f"U-Net output rel-L2 {e_out:.2e}, loss rel {e_loss:.2e}, gradient rel-L2 {e_grad:.2e}")
+    assert e_out <= REL_FWD
+    assert e_loss <= REL_FWD
+    assert e_grad <= REL_GRAD
+    for name, cin, cout in p.specs:
+        w0, b0 = p.off[name]
+        e = rel_l2(gpu_grad[w0:b0].cpu().numpy(), master.grad[w0:b0].numpy())
+        assert e <= 2 * REL_GRAD, (name, e)
+
+
+def test_adamw_matches_torch():
+    from seistorch_amd import precond
+    p, _ = _net(base=8)
+    g = torch.Generator().manual_seed(5)
+    opt = precond.AdamW(p, lr=1e-3, weight_decay=1e-2)
+    ref = torch.nn.Parameter(p.master.detach().cpu().clone())
+    topt = torch.optim.AdamW([ref], lr=1e-3, weight_decay=1e-2)
+    for _ in range(3):
+        gr = torch.randn(p.n, generator=g)
+        opt.step(gr.cuda())
+        ref.grad = gr
+        topt.step()
+    torch.cuda.synchronize()
+    assert rel_l2(p.master.cpu().numpy(), ref.detach().numpy()) <= 1e-6
+    assert torch.equal(p.bf16.cpu(), bf(p.master.cpu()))
+
+
+def test_training_reduces_loss_on_a_fixed_batch():
+    from seistorch_amd import precond
+    p, net = _net(seed=1)
+    opt = precond.AdamW(p, lr=1e-3)
+    N, H, W = 4, 32, 32
+    grad, model, _ = _inputs(N, H, W, 11)
+    tgt = torch.tanh(grad * 2e3) * 0.5  # a learnable map of the input
+    grad, model, tgt = grad.cuda(), model.cuda(), tgt.cuda()
+    scales = precond.input_scales(grad, model)
+    losses = [float(precond.train_step(net, opt, grad, model, scales, tgt)) for _ in range(40)]
+    print(f"training loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.3 * losses[0]
