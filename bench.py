@@ -40,17 +40,23 @@ def parse():
                    help="time steps of the bounded CPU sample (1 shot, full grid)")
     p.add_argument("--nt", type=int, default=None,
                    help="shorter time axis for profiler runs (the metric is quoted at full nt)")
-    p.add_argument("--workload", choices=["acoustic", "elastic", "generator"],
+    p.add_argument("--workload", choices=["acoustic", "elastic", "generator", "precond"],
                    default="acoustic",
                    help="acoustic: 64-shot 800x2400 FWI gradient (the headline); elastic: "
                         "32-shot 600x1800 vp/vs/rho gradient; generator: 256 uneven seeded "
-                        "models, records + misfit gradient in one batch")
+                        "models, records + misfit gradient in one batch; precond: train the "
+                        "U-Net preconditioner on generator pairs, then plain vs preconditioned "
+                        "FWI on the 800x2400 problem")
     p.add_argument("--models", type=int, default=256, help="generator: number of models")
     p.add_argument("--graph", type=int, default=0,
                    help="generator: 1 records each batched loop into one hipGraph (kernel "
                         "timing events are then unavailable)")
     p.add_argument("--snapshot-every", type=int, default=500,
                    help="generator: field-pair snapshot spacing of the batched adjoint")
+    p.add_argument("--pair-models", type=int, default=512,
+                   help="precond: generator models whose gradients make the training pairs")
+    p.add_argument("--train-steps", type=int, default=1500, help="precond: AdamW steps")
+    p.add_argument("--fwi-iters", type=int, default=30, help="precond: FWI iterations per method")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a plain 1 GiB device copy (HBM-counter calibration)")
     return p.parse_args()
@@ -242,6 +248,65 @@ def run_generator(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_precond(args, dev):
+    """Learned preconditioner: pairs from the generator, AdamW training with
+    the hand-written kernels, then plain and preconditioned FWI on the
+    800x2400 64-shot problem; value = wall-clock of the preconditioned run to
+    reach the plain run's final misfit / the plain run's wall-clock."""
+    import torch
+    from seistorch_amd import fwi, learned, precond
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    pairs = learned.make_pairs(args.pair_models, dev, log=log)
+    t_pairs = time.perf_counter() - t0
+    n = pairs[0].shape[0]
+    n_val = max(1, n // 10)
+    val = tuple(x[:n_val] for x in pairs)
+    trn = tuple(x[n_val:] for x in pairs)
+    p = precond.Params(dev, seed=0)
+    net = precond.UNet(p)
+    opt = precond.AdamW(p, lr=1e-3)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist = learned.train(net, opt, trn, args.train_steps, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    val_loss, val_zero = learned.evaluate(net, val)
+    log(f"[bench] pairs {n} ({t_pairs:.1f} s), training {t_train:.1f} s, validation loss "
+        f"{val_loss:.4f} vs zero direction {val_zero:.4f}")
+    P = build_problem(dev, nt_override=args.nt)
+    prob = fwi.Problem(P["vp_true"], P["vp_init"], P["d"], P["w"], P["src"], P["rec"], P["B"],
+                       P["dt"], P["h"])
+    prob.obs = prob.records(prob.vp_true)
+    _, hp = fwi.run_plain(prob, args.fwi_iters, log=log)
+    _, hn = fwi.run_preconditioned(prob, net, args.fwi_iters, log=log)
+    target = hp.misfit[-1]
+    t_hit = fwi.time_to(hn, target)
+    ratio = (t_hit / hp.t[-1]) if t_hit is not None else None
+    out = {
+        "metric": METRIC, "value": None if ratio is None else round(ratio, 3),
+        "unit": "preconditioned wall-clock to the plain run's final misfit / plain wall-clock",
+        "n_gpus": 1, "steps": args.fwi_iters, "warmup": 0,
+        "ms_per_step": round(hn.t[-1] / max(len(hn.t) - 1, 1) * 1e3, 1),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
+        "data": "synthetic (generator pairs for training; seeded anomaly model for FWI)",
+        "config": {"workload": "U-Net preconditioned FWI vs plain FWI, 800x2400, 64 shots",
+                   "pair_models": args.pair_models, "pairs": n, "train_steps": args.train_steps,
+                   "fwi_iters": args.fwi_iters, "nt": P["nt"]},
+        "training": {"pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1),
+                     "loss_history": hist, "val_loss": val_loss, "val_zero_loss": val_zero},
+        "plain": {"t": [round(x, 2) for x in hp.t], "misfit": hp.misfit, "rmse": hp.rmse},
+        "preconditioned": {"t": [round(x, 2) for x in hn.t], "misfit": hn.misfit,
+                           "rmse": hn.rmse, "kind": hn.kind},
+        "misfit_at_plain_end": {"plain": target,
+                                "preconditioned": fwi.misfit_at(hn, hp.t[-1])},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -253,6 +318,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "generator":
         return run_generator(args, dev)
+    if args.workload == "precond":
+        return run_precond(args, dev)
     elastic = args.workload == "elastic"
     if elastic:
         P = build_elastic_problem(dev, nt_override=args.nt)

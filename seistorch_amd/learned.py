@@ -19,9 +19,10 @@ PAD = generator.PAD
 
 
 def make_pairs(n_models, device, seed=20230503, crop=128, crops_per_model=4, mute=5,
-               chunk=256, vscale=500.0, log=None):
+               chunk=256, vscale=500.0, spec=None, log=None):
     """(grad, model, target) crops as (M, crop, crop) FP32 device tensors."""
-    spec = generator.GenSpec(n_models=n_models)
+    spec = spec or generator.GenSpec()
+    spec = generator.GenSpec(**{**spec.__dict__, "n_models": n_models})
     host = generator.make_instances(seed=seed, spec=spec)
     rng = np.random.default_rng(seed + 1)
     G, Mo, T = [], [], []
@@ -42,8 +43,6 @@ def make_pairs(n_models, device, seed=20230503, crop=128, crops_per_model=4, mut
                 z0 = int(rng.integers(0, nz - crop + 1))
                 x0 = int(rng.integers(0, nx - crop + 1))
                 sl = (slice(z0, z0 + crop), slice(x0, x0 + crop))
-                # a crop below the surface has no muted rows: keep the mute
-                # only where the crop touches the surface
                 G.append(gi[sl]); Mo.append(vi[sl]); T.append(ti[sl])
         del part, obs, grads
         if log:
