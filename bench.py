@@ -155,6 +155,20 @@ def cpu_baseline(P, steps):
                       f"{el:.1f} s"}
 
 
+def copy_calibration(dev):
+    """A plain 1 GiB device copy: the reference dispatch for scaling the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
+    import torch
+    n = 1 << 28  # 1 GiB of float32
+    a = torch.ones(n, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    return 4 * n
+
+
 def run_generator(args, dev):
     """Uneven batch: 256 seeded models (grids 64..512 x 64..768, 4-16 shots,
     nt = 1.2 x diagonal traveltime).  One step = records of every true model
@@ -245,6 +259,8 @@ def run_generator(args, dev):
                                          for (a, b, c), v in tune.items()}},
         "loss_sum": float(loss.sum()),
     }
+    if args.copy_calibration:
+        out["copy_calibration_bytes"] = copy_calibration(dev)
     print(json.dumps(out), flush=True)
 
 
@@ -262,10 +278,9 @@ def run_precond(args, dev):
     t0 = time.perf_counter()
     pairs = learned.make_pairs(args.pair_models, dev, log=log)
     t_pairs = time.perf_counter() - t0
-    n = pairs[0].shape[0]
+    n = len(pairs)
     n_val = max(1, n // 10)
-    val = tuple(x[:n_val] for x in pairs)
-    trn = tuple(x[n_val:] for x in pairs)
+    val, trn = pairs[:n_val], pairs[n_val:]
     p = precond.Params(dev, seed=0)
     net = precond.UNet(p)
     opt = precond.AdamW(p, lr=1e-3)
@@ -277,6 +292,12 @@ def run_precond(args, dev):
     val_loss, val_zero = learned.evaluate(net, val)
     log(f"[bench] pairs {n} ({t_pairs:.1f} s), training {t_train:.1f} s, validation loss "
         f"{val_loss:.4f} vs zero direction {val_zero:.4f}")
+    if args.fwi_iters == 0:  # training only (profiling runs)
+        print(json.dumps({"training": {"pairs": n, "pairs_s": round(t_pairs, 1),
+                                       "train_s": round(t_train, 1), "loss_history": hist,
+                                       "val_loss": val_loss, "val_zero_loss": val_zero}}),
+              flush=True)
+        return
     P = build_problem(dev, nt_override=args.nt)
     prob = fwi.Problem(P["vp_true"], P["vp_init"], P["d"], P["w"], P["src"], P["rec"], P["B"],
                        P["dt"], P["h"])
@@ -397,14 +418,7 @@ def main():
         "loss": float(loss),
     }
     if args.copy_calibration:
-        n = 1 << 28  # 1 GiB of float32
-        a = torch.ones(n, device=dev)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = 4 * n
-        del a, b
+        out["copy_calibration_bytes"] = copy_calibration(dev)
     if not args.no_cpu_baseline and not elastic:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, args.cpu_steps)

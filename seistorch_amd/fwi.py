@@ -10,9 +10,10 @@ is muted within `mute` cells of the surface, as the reference's seabed cut
 the boundary strips are re-padded from its edges (geom.py:213-240).
 
 The preconditioned method feeds (gradient, model) to the network, which
-predicts the model perturbation; the step along it is accepted if the
-misfit decreases (tried at full and half length), otherwise the plain step
-of the same iteration is taken (the fallback).
+predicts the model perturbation; that step is accepted if the misfit
+decreases, else the network's direction scaled to the current plain step
+length is tried, and otherwise the plain step of the same iteration is
+taken (the fallback).
 """
 import time
 from dataclasses import dataclass, field
@@ -136,11 +137,18 @@ def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, time_budget=None
         d = net.forward(g[None].contiguous(), vi[None], sc)[0] * vscale
         d[:p.mute] = 0.0
         kind, m = "none", loss
-        for a in (1.0, 0.5):
-            trial = p.pad(_clip(p, vi + a * d))
+        dmax = float(d.abs().max())
+        # the network's own step, then its direction at the plain step length
+        trials = [("net", d)]
+        if dmax > 0.0:
+            trials.append(("netdir", d * (step / dmax)))
+        for name, dv in trials:
+            trial = p.pad(_clip(p, vi + dv))
             mt = p.misfit(trial)
             if mt < loss:
-                vp, m, kind = trial, mt, f"net{a:g}"
+                vp, m, kind = trial, mt, name
+                if name == "netdir":
+                    step *= 1.25
                 break
         if kind == "none":  # fallback: the plain step of this iteration
             vp, m, step, ok = plain_step(p, vp, loss, g, step)

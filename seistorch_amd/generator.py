@@ -9,7 +9,8 @@ Each instance (seeded from one integer):
 * 3-8 dipping layers with velocities uniform in [1500, 4500] m/s (sorted with
   depth), 0-2 normal/reverse faults offsetting the layers, 1-3 Gaussian
   anomalies of +-5-10 %, clipped to [1500, 4500];
-* start model: the true model Gaussian-smoothed (sigma = 10 cells);
+* start model: the true model Gaussian-smoothed (sigma = 20 cells, as the
+  800x2400 FWI problem's start model);
 * 4-16 shots evenly spaced at depth 2 cells, a receiver at every surface cell;
 * Ricker 5-15 Hz (delay 1.2/f), dt = 1 ms, nt = ceil(1.2 * diagonal / 1500 m/s / dt).
 The reference has no generator of its own; its per-model pieces are the
@@ -46,7 +47,7 @@ class GenSpec:
     dx: float = 10.0
     nt_factor: float = 1.2
     v_ref: float = 1500.0
-    smooth: float = 10.0
+    smooth: float = 20.0
     nx_step: int = 4
 
 

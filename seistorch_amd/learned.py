@@ -2,67 +2,83 @@
 
 Pairs come from the uneven-batch generator (seistorch_amd.generator): for
 every seeded model, the L2-misfit gradient at the smoothed start model
-(computed by the batched solver) and the perturbation true - start.  Each
-pair is cut into fixed-size crops of the unpadded grid (network input:
-gradient with the top `mute` rows zeroed and the start model; target:
-(true - start) / vscale), so that crops of models of different sizes batch
-together.
+(computed by the batched solver) and the perturbation true - start.  A pair
+is the whole unpadded model cut to multiples of 16 cells from its surface
+corner, so that the network sees what it sees inside FWI (the surface
+acquisition footprint and a per-image gradient normalisation): input =
+(gradient with the top `mute` rows zeroed, start model), target =
+(true - start) / vscale.  Mini-batches pad their members to a common size
+(gradient and target zero, model edge-replicated) and mask the padding out of
+the loss.
 """
 import time
 
 import numpy as np
 import torch
+import torch.nn.functional as F
 
 from . import generator, precond
 
 PAD = generator.PAD
 
 
-def make_pairs(n_models, device, seed=20230503, crop=128, crops_per_model=4, mute=5,
-               chunk=256, vscale=500.0, spec=None, log=None):
-    """(grad, model, target) crops as (M, crop, crop) FP32 device tensors."""
+def make_pairs(n_models, device, seed=20230503, mute=5, chunk=256, vscale=500.0, spec=None,
+               log=None):
+    """[(grad, model, target)] per model, each (h, w) FP32 on the device."""
     spec = spec or generator.GenSpec()
     spec = generator.GenSpec(**{**spec.__dict__, "n_models": n_models})
     host = generator.make_instances(seed=seed, spec=spec)
-    rng = np.random.default_rng(seed + 1)
-    G, Mo, T = [], [], []
+    out = []
     t0 = time.perf_counter()
     for c0 in range(0, len(host), chunk):
         part = generator.to_device(host[c0:c0 + chunk], device)
         obs = generator.observed(part)
         _, grads = generator.misfit_gradients(part, obs, snapshot_every=500)
         for it, g in zip(part, grads):
-            nz, nx = it["nz_i"], it["nx_i"]
-            if nz < crop or nx < crop:
-                continue
-            gi = g[PAD:PAD + nz, PAD:PAD + nx].clone()
+            h, w = it["nz_i"] // 16 * 16, it["nx_i"] // 16 * 16
+            sl = (slice(PAD, PAD + h), slice(PAD, PAD + w))
+            gi = g[sl].clone()
             gi[:mute] = 0.0
-            vi = it["vp_start"][PAD:PAD + nz, PAD:PAD + nx]
-            ti = (it["vp_true"][PAD:PAD + nz, PAD:PAD + nx] - vi) / vscale
-            for _ in range(crops_per_model):
-                z0 = int(rng.integers(0, nz - crop + 1))
-                x0 = int(rng.integers(0, nx - crop + 1))
-                sl = (slice(z0, z0 + crop), slice(x0, x0 + crop))
-                G.append(gi[sl]); Mo.append(vi[sl]); T.append(ti[sl])
+            vi = it["vp_start"][sl].clone()
+            ti = (it["vp_true"][sl] - vi) / vscale
+            out.append((gi, vi, ti))
         del part, obs, grads
         if log:
             log(f"[pairs] {min(c0 + chunk, len(host))}/{len(host)} models, "
-                f"{len(G)} crops, {time.perf_counter() - t0:.1f} s")
-    return (torch.stack(G).contiguous(), torch.stack(Mo).contiguous(),
-            torch.stack(T).contiguous())
+                f"{time.perf_counter() - t0:.1f} s")
+    return out
 
 
-def train(net, opt, pairs, steps, batch=16, seed=0, log=None, log_every=100):
-    """AdamW steps on random mini-batches of the pairs; returns the loss history."""
-    G, Mo, T = pairs
-    n = G.shape[0]
+def collate(pairs):
+    """Pad a list of pairs to their largest (h, w): (grad, model, target, mask)."""
+    H = max(p[0].shape[0] for p in pairs)
+    W = max(p[0].shape[1] for p in pairs)
+    n = len(pairs)
+    dev = pairs[0][0].device
+    G = torch.zeros((n, H, W), device=dev)
+    T = torch.zeros((n, H, W), device=dev)
+    Mk = torch.zeros((n, H, W), device=dev)
+    Mo = torch.empty((n, H, W), device=dev)
+    for i, (g, m, t) in enumerate(pairs):
+        h, w = g.shape
+        G[i, :h, :w] = g
+        T[i, :h, :w] = t
+        Mk[i, :h, :w] = 1.0
+        Mo[i] = F.pad(m[None, None], (0, W - w, 0, H - h), mode="replicate")[0, 0]
+    return G, Mo, T, Mk
+
+
+def train(net, opt, pairs, steps, batch=8, seed=0, log=None, log_every=100):
+    """AdamW steps on random mini-batches of whole-model pairs; returns the
+    (step, loss) history."""
     gen = torch.Generator().manual_seed(seed)
+    n = len(pairs)
     losses = []
     for s in range(steps):
-        idx = torch.randint(0, n, (min(batch, n),), generator=gen).to(G.device)
-        g, m, t = G[idx].contiguous(), Mo[idx].contiguous(), T[idx].contiguous()
+        idx = torch.randint(0, n, (min(batch, n),), generator=gen).tolist()
+        g, m, t, mk = collate([pairs[i] for i in idx])
         sc = precond.input_scales(g, m)
-        loss = precond.train_step(net, opt, g, m, sc, t)
+        loss = precond.train_step(net, opt, g, m, sc, t, mk)
         if (s + 1) % log_every == 0 or s == steps - 1:
             losses.append((s + 1, float(loss)))
             if log:
@@ -70,13 +86,14 @@ def train(net, opt, pairs, steps, batch=16, seed=0, log=None, log_every=100):
     return losses
 
 
-def evaluate(net, pairs, batch=32):
-    """Mean loss of the network and of the zero direction over the pairs."""
-    G, Mo, T = pairs
-    tot = zero = 0.0
-    for i in range(0, G.shape[0], batch):
-        g, m, t = G[i:i + batch], Mo[i:i + batch], T[i:i + batch]
-        out = net.forward(g, m, precond.input_scales(g, m))
+def evaluate(net, pairs):
+    """Mean squared error of the network and of the zero direction over the
+    pairs (each model on its own, no padding)."""
+    tot = zero = cells = 0.0
+    for g, m, t in pairs:
+        out = net.forward(g[None].contiguous(), m[None].contiguous(),
+                          precond.input_scales(g[None], m[None]))[0]
         tot += float(((out - t) ** 2).sum())
         zero += float((t ** 2).sum())
-    return tot / G.numel(), zero / G.numel()
+        cells += t.numel()
+    return tot / cells, zero / cells

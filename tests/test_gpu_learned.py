@@ -37,14 +37,14 @@ def _small_problem():
 def test_training_and_preconditioned_fwi_small():
     from seistorch_amd import fwi, generator, learned, precond
     spec = generator.GenSpec(nz=(64, 160), nx=(64, 192))
-    pairs = learned.make_pairs(24, "cuda", crop=64, crops_per_model=4, spec=spec)
-    n = pairs[0].shape[0]
-    assert n >= 32
+    pairs = learned.make_pairs(24, "cuda", spec=spec)
+    n = len(pairs)
+    assert n == 24
     p = precond.Params("cuda", seed=0)
     net = precond.UNet(p)
     opt = precond.AdamW(p, lr=1e-3)
     loss0, zero = learned.evaluate(net, pairs)
-    learned.train(net, opt, pairs, 300, batch=16)
+    learned.train(net, opt, pairs, 300, batch=8)
     loss1, zero = learned.evaluate(net, pairs)
     print(f"pairs {n}: loss {loss0:.4f} -> {loss1:.4f}, zero direction {zero:.4f}")
     assert loss1 < zero
