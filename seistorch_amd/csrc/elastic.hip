@@ -27,6 +27,8 @@
 #include "seistorch_amd.h"
 #include "tiles.h"
 
+#include <initializer_list>
+
 using seis::Grid;
 using namespace seis::tiles;
 
@@ -51,16 +53,6 @@ struct CFields {
   const float* f[5];
 };
 
-// One cell per thread (64 x 4 tiles), S shots per thread, XCD-aware order.
-__device__ __forceinline__ bool cell_of(const TileMap tm, const Grid g, int& z, int& x,
-                                        int& grp) {
-  int tx, tzi;
-  if (!tile_of(tm, tx, tzi, grp)) return false;
-  x = tx * kLanes + (int)(threadIdx.x % kLanes);
-  z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
-  return x < g.nx && z < g.nz;
-}
-
 __global__ __launch_bounds__(256) void k_el_coef(const float* __restrict__ vp,
                                                  const float* __restrict__ vs,
                                                  const float* __restrict__ rho,
@@ -81,60 +73,124 @@ __global__ __launch_bounds__(256) void k_el_coef(const float* __restrict__ vp,
   cf[kBm * plane + i] = 1.0f - c;                    // (1-c)
 }
 
-// Stage T in place over the region: stresses from velocities.
-template <int S>
+// Lane cells of the vectorised stage kernels: VX consecutive x cells of one
+// row (float/float2/float4 access), S shots per thread, XCD-aware order.
+template <int VX>
+__device__ __forceinline__ bool vcell_of(const TileMap tm, const Grid g, int& z, int& x0,
+                                         int& grp) {
+  int tx, tzi;
+  if (!tile_of(tm, tx, tzi, grp)) return false;
+  x0 = (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
+  z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  return x0 < g.nx && z < g.nz;
+}
+
+// Stage T in place over the region: stresses from velocities.  Each lane
+// updates VX cells; cells outside the region keep their values.  The
+// arithmetic per cell is the scalar expression order of elastic.py:16-24.
+template <int S, int VX>
 __global__ __launch_bounds__(256) void k_el_T(const float* __restrict__ cf, float sdti,
                                               float sgn, Region rg, Fields F, int B, Grid g,
                                               TileMap tm) {
-  int z, x, grp;
-  if (!cell_of(tm, g, z, x, grp) || !in_region(z, x, rg)) return;
-  const int i = z * g.nx + x;
+  int z, x0, grp;
+  if (!vcell_of<VX>(tm, g, z, x0, grp)) return;
+  if (z < rg.z0 || z >= rg.z1 || x0 + VX <= rg.x0 || x0 >= rg.x1) return;
+  const int i = z * g.nx + x0;
   const int64_t P = g.plane;
-  const float l2m = cf[kL2M * P + i], lam = cf[kLam * P + i];
-  const float smc = sgn * cf[kMc * P + i];
-  const float a = cf[kAinv * P + i], bm = cf[kBm * P + i];
+  float l2m[VX], lam[VX], mc[VX], a[VX], bm[VX];
+  vload<VX>(cf + kL2M * P + i, l2m);
+  vload<VX>(cf + kLam * P + i, lam);
+  vload<VX>(cf + kMc * P + i, mc);
+  vload<VX>(cf + kAinv * P + i, a);
+  vload<VX>(cf + kBm * P + i, bm);
+  bool in[VX];
+#pragma unroll
+  for (int e = 0; e < VX; ++e) in[e] = x0 + e >= rg.x0 && x0 + e < rg.x1;
+  const bool zl = z > 0, zr = z < g.nz - 1;
   const int b0 = grp * S, nb = min(S, B - b0);
   for (int s = 0; s < nb; ++s) {
     const int64_t off = (int64_t)(b0 + s) * P + i;
     const float* vx = F.f[0] + off;
     const float* vz = F.f[1] + off;
-    const float vx_x = x > 0 ? vx[0] - vx[-1] : 0.0f;
-    const float vz_z = z < g.nz - 1 ? vz[g.nx] - vz[0] : 0.0f;
-    const float vx_z = z > 0 ? vx[0] - vx[-g.nx] : 0.0f;
-    const float vz_x = x < g.nx - 1 ? vz[1] - vz[0] : 0.0f;
-    float* txx = F.f[2] + off;
-    float* tzz = F.f[3] + off;
-    float* txz = F.f[4] + off;
-    *txx = a * (sdti * (l2m * vx_x + lam * vz_z) + bm * *txx);
-    *tzz = a * (sdti * (l2m * vz_z + lam * vx_x) + bm * *tzz);
-    *txz = a * (smc * (vz_x + vx_z) + bm * *txz);
+    float vxc[VX], vzc[VX], vxu[VX], vzd[VX];
+    vload<VX>(vx, vxc);
+    vload<VX>(vz, vzc);
+    if (zl) vload<VX>(vx - g.nx, vxu); else vzero<VX>(vxu);
+    if (zr) vload<VX>(vz + g.nx, vzd); else vzero<VX>(vzd);
+    const float vxl = x0 > 0 ? vx[-1] : 0.0f;
+    const float vzr = x0 + VX < g.nx ? vz[VX] : 0.0f;
+    float txx[VX], tzz[VX], txz[VX];
+    vload<VX>(F.f[2] + off, txx);
+    vload<VX>(F.f[3] + off, tzz);
+    vload<VX>(F.f[4] + off, txz);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      const int x = x0 + e;
+      const float vx_x = x > 0 ? vxc[e] - (e == 0 ? vxl : vxc[e - 1]) : 0.0f;
+      const float vz_z = zr ? vzd[e] - vzc[e] : 0.0f;
+      const float vx_z = zl ? vxc[e] - vxu[e] : 0.0f;
+      const float vz_x = x < g.nx - 1 ? (e == VX - 1 ? vzr : vzc[e + 1]) - vzc[e] : 0.0f;
+      if (in[e]) {
+        txx[e] = a[e] * (sdti * (l2m[e] * vx_x + lam[e] * vz_z) + bm[e] * txx[e]);
+        tzz[e] = a[e] * (sdti * (l2m[e] * vz_z + lam[e] * vx_x) + bm[e] * tzz[e]);
+        txz[e] = a[e] * ((sgn * mc[e]) * (vz_x + vx_z) + bm[e] * txz[e]);
+      }
+    }
+    vstore<VX>(F.f[2] + off, txx);
+    vstore<VX>(F.f[3] + off, tzz);
+    vstore<VX>(F.f[4] + off, txz);
   }
 }
 
-// Stage V in place over the region: velocities from stresses.
-template <int S>
+// Stage V in place over the region: velocities from stresses (elastic.py:26-35).
+template <int S, int VX>
 __global__ __launch_bounds__(256) void k_el_V(const float* __restrict__ cf, float sgn, Region rg,
                                               Fields F, int B, Grid g, TileMap tm) {
-  int z, x, grp;
-  if (!cell_of(tm, g, z, x, grp) || !in_region(z, x, rg)) return;
-  const int i = z * g.nx + x;
+  int z, x0, grp;
+  if (!vcell_of<VX>(tm, g, z, x0, grp)) return;
+  if (z < rg.z0 || z >= rg.z1 || x0 + VX <= rg.x0 || x0 >= rg.x1) return;
+  const int i = z * g.nx + x0;
   const int64_t P = g.plane;
-  const float src = sgn * cf[kRc * P + i];
-  const float a = cf[kAinv * P + i], bm = cf[kBm * P + i];
+  float rc[VX], a[VX], bm[VX];
+  vload<VX>(cf + kRc * P + i, rc);
+  vload<VX>(cf + kAinv * P + i, a);
+  vload<VX>(cf + kBm * P + i, bm);
+  bool in[VX];
+#pragma unroll
+  for (int e = 0; e < VX; ++e) in[e] = x0 + e >= rg.x0 && x0 + e < rg.x1;
+  const bool zl = z > 0, zr = z < g.nz - 1;
   const int b0 = grp * S, nb = min(S, B - b0);
   for (int s = 0; s < nb; ++s) {
     const int64_t off = (int64_t)(b0 + s) * P + i;
     const float* txx = F.f[2] + off;
     const float* tzz = F.f[3] + off;
     const float* txz = F.f[4] + off;
-    const float txx_x = x < g.nx - 1 ? txx[1] - txx[0] : 0.0f;
-    const float txz_z = z < g.nz - 1 ? txz[g.nx] - txz[0] : 0.0f;
-    const float tzz_z = z > 0 ? tzz[0] - tzz[-g.nx] : 0.0f;
-    const float txz_x = x > 0 ? txz[0] - txz[-1] : 0.0f;
-    float* vx = F.f[0] + off;
-    float* vz = F.f[1] + off;
-    *vx = a * (src * (txx_x + txz_z) + bm * *vx);
-    *vz = a * (src * (txz_x + tzz_z) + bm * *vz);
+    float txxc[VX], tzzc[VX], txzc[VX], tzzu[VX], txzd[VX];
+    vload<VX>(txx, txxc);
+    vload<VX>(tzz, tzzc);
+    vload<VX>(txz, txzc);
+    if (zl) vload<VX>(tzz - g.nx, tzzu); else vzero<VX>(tzzu);
+    if (zr) vload<VX>(txz + g.nx, txzd); else vzero<VX>(txzd);
+    const float txxr = x0 + VX < g.nx ? txx[VX] : 0.0f;
+    const float txzl = x0 > 0 ? txz[-1] : 0.0f;
+    float vx[VX], vz[VX];
+    vload<VX>(F.f[0] + off, vx);
+    vload<VX>(F.f[1] + off, vz);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      const int x = x0 + e;
+      const float txx_x = x < g.nx - 1 ? (e == VX - 1 ? txxr : txxc[e + 1]) - txxc[e] : 0.0f;
+      const float txz_z = zr ? txzd[e] - txzc[e] : 0.0f;
+      const float tzz_z = zl ? tzzc[e] - tzzu[e] : 0.0f;
+      const float txz_x = x > 0 ? txzc[e] - (e == 0 ? txzl : txzc[e - 1]) : 0.0f;
+      if (in[e]) {
+        const float src = sgn * rc[e];
+        vx[e] = a[e] * (src * (txx_x + txz_z) + bm[e] * vx[e]);
+        vz[e] = a[e] * (src * (txz_x + tzz_z) + bm[e] * vz[e]);
+      }
+    }
+    vstore<VX>(F.f[0] + off, vx);
+    vstore<VX>(F.f[1] + off, vz);
   }
 }
 
@@ -142,84 +198,141 @@ __global__ __launch_bounds__(256) void k_el_V(const float* __restrict__ cf, floa
 // inputs.  fw: the stage's input fields (stresses used for the Rc gradient);
 // gmode 0 none, 1 every region cell, 2 interior cells only; with recon the
 // velocities in fw are rebuilt in place on the interior (the reverse of V).
-template <int S>
+// Each lane handles VX consecutive cells of a row; every cell's arithmetic is
+// the scalar expression order.
+template <int S, int VX>
 __global__ __launch_bounds__(256) void k_el_adjV(const float* __restrict__ cf, float sgn,
                                                  Region rg, CFields in, Fields out, Fields fw,
                                                  float* __restrict__ gRc, int gmode, int recon,
                                                  int B, Grid g, TileMap tm) {
-  int z, x, grp;
-  if (!cell_of(tm, g, z, x, grp)) return;
-  const int i = z * g.nx + x;
+  int z, x0, grp;
+  if (!vcell_of<VX>(tm, g, z, x0, grp)) return;
+  const int i = z * g.nx + x0;
   const int64_t P = g.plane;
   const float* A = cf + kAinv * P;
   const float* R = cf + kRc * P;
-  // s*a*Rc at the cell and the neighbours whose t_x / t_z reach it (0 outside)
+  // s*a*Rc at (zz, xx), 0 outside the region
   auto aR = [&](int zz, int xx) {
     return in_region(zz, xx, rg) ? sgn * A[zz * g.nx + xx] * R[zz * g.nx + xx] : 0.0f;
   };
-  const bool reg = in_region(z, x, rg);
-  const float c0 = aR(z, x);
-  const float cxm = x > 0 ? aR(z, x - 1) : 0.0f;
-  const float czm = z > 0 ? aR(z - 1, x) : 0.0f;
-  const float cxp = x < g.nx - 1 ? aR(z, x + 1) : 0.0f;
-  const float czp = z < g.nz - 1 ? aR(z + 1, x) : 0.0f;
-  const float own = reg ? A[i] * cf[kBm * P + i] : 1.0f;
-  const bool grad = gRc && reg && (gmode == 1 || interior(z, x, g));
-  const bool rebuild = recon && interior(z, x, g);
-  const float rc = R[i];
-  float gacc = 0.0f;
+  const bool zl = z >= 1, zr = z <= g.nz - 2;
+  // c at x0-1 .. x0+VX of row z (index e+1), and rows z-1, z+1
+  float crow[VX + 2], cup[VX], cdn[VX], own[VX], rc[VX], ai[VX];
+  bool grad[VX], rebuild[VX];
+  bool anyg = false;
+#pragma unroll
+  for (int e = -1; e <= VX; ++e) {
+    const int x = x0 + e;
+    crow[e + 1] = (x >= 0 && x < g.nx) ? aR(z, x) : 0.0f;
+  }
+#pragma unroll
+  for (int e = 0; e < VX; ++e) {
+    const int x = x0 + e;
+    cup[e] = zl ? aR(z - 1, x) : 0.0f;
+    cdn[e] = zr ? aR(z + 1, x) : 0.0f;
+    const bool reg = in_region(z, x, rg);
+    own[e] = reg ? A[i + e] * cf[kBm * P + i + e] : 1.0f;
+    rc[e] = R[i + e];
+    ai[e] = A[i + e];
+    grad[e] = gRc && reg && (gmode == 1 || seis::interior(z, x, g));
+    rebuild[e] = recon && seis::interior(z, x, g);
+    anyg = anyg || grad[e] || rebuild[e];
+  }
+  float gacc[VX];
+  vzero<VX>(gacc);
   const int b0 = grp * S, nb = min(S, B - b0);
   for (int s = 0; s < nb; ++s) {
     const int64_t off = (int64_t)(b0 + s) * P + i;
     const float* avx = in.f[0] + off;
     const float* avz = in.f[1] + off;
-    // t_x = s*a*Rc*avx, t_z likewise
-    const float tx0 = c0 * avx[0];
-    const float tz0 = c0 * avz[0];
-    const float txm = x > 0 ? cxm * avx[-1] : 0.0f;
-    const float txzm = z > 0 ? czm * avx[-g.nx] : 0.0f;
-    const float tzxp = x < g.nx - 1 ? cxp * avz[1] : 0.0f;
-    const float tzzp = z < g.nz - 1 ? czp * avz[g.nx] : 0.0f;
-    const bool xl = x >= 1, xr = x <= g.nx - 2, zl = z >= 1, zr = z <= g.nz - 2;
-    // Dx+^T t_x, Dz+^T t_x + Dx-^T t_z, Dz-^T t_z
-    out.f[2][off] = in.f[2][off] + ((xl ? txm : 0.0f) - (xr ? tx0 : 0.0f));
-    out.f[4][off] = in.f[4][off] + ((zl ? txzm : 0.0f) - (zr ? tx0 : 0.0f)) +
-                    ((xl ? tz0 : 0.0f) - (xr ? tzxp : 0.0f));
-    out.f[3][off] = in.f[3][off] + ((zl ? tz0 : 0.0f) - (zr ? tzzp : 0.0f));
-    out.f[0][off] = own * avx[0];
-    out.f[1][off] = own * avz[0];
-    if (grad || rebuild) {
+    float vxc[VX], vzc[VX], vxu[VX], vzd[VX];
+    vload<VX>(avx, vxc);
+    vload<VX>(avz, vzc);
+    if (zl) vload<VX>(avx - g.nx, vxu); else vzero<VX>(vxu);
+    if (zr) vload<VX>(avz + g.nx, vzd); else vzero<VX>(vzd);
+    const float vxl = x0 > 0 ? avx[-1] : 0.0f;
+    const float vzr = x0 + VX < g.nx ? avz[VX] : 0.0f;
+    float i2[VX], i3[VX], i4[VX], o0[VX], o1[VX];
+    vload<VX>(in.f[2] + off, i2);
+    vload<VX>(in.f[3] + off, i3);
+    vload<VX>(in.f[4] + off, i4);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      const int x = x0 + e;
+      const bool xl = x >= 1, xr = x <= g.nx - 2;
+      const float c0 = crow[e + 1];
+      const float tx0 = c0 * vxc[e];
+      const float tz0 = c0 * vzc[e];
+      const float txm = xl ? crow[e] * (e == 0 ? vxl : vxc[e - 1]) : 0.0f;
+      const float txzm = zl ? cup[e] * vxu[e] : 0.0f;
+      const float tzxp = xr ? crow[e + 2] * (e == VX - 1 ? vzr : vzc[e + 1]) : 0.0f;
+      const float tzzp = zr ? cdn[e] * vzd[e] : 0.0f;
+      i2[e] = i2[e] + ((xl ? txm : 0.0f) - (xr ? tx0 : 0.0f));
+      i4[e] = i4[e] + ((zl ? txzm : 0.0f) - (zr ? tx0 : 0.0f)) +
+              ((xl ? tz0 : 0.0f) - (xr ? tzxp : 0.0f));
+      i3[e] = i3[e] + ((zl ? tz0 : 0.0f) - (zr ? tzzp : 0.0f));
+      o0[e] = own[e] * vxc[e];
+      o1[e] = own[e] * vzc[e];
+    }
+    vstore<VX>(out.f[2] + off, i2);
+    vstore<VX>(out.f[4] + off, i4);
+    vstore<VX>(out.f[3] + off, i3);
+    vstore<VX>(out.f[0] + off, o0);
+    vstore<VX>(out.f[1] + off, o1);
+    if (anyg) {
       const float* txx = fw.f[2] + off;
       const float* tzz = fw.f[3] + off;
       const float* txz = fw.f[4] + off;
-      const float sx = (x < g.nx - 1 ? txx[1] - txx[0] : 0.0f) +
-                       (z < g.nz - 1 ? txz[g.nx] - txz[0] : 0.0f);
-      const float sz = (x > 0 ? txz[0] - txz[-1] : 0.0f) + (z > 0 ? tzz[0] - tzz[-g.nx] : 0.0f);
-      if (grad) gacc += A[i] * sgn * (sx * avx[0] + sz * avz[0]);
-      if (rebuild) {
-        // forward vx' = a*(Rc*sx + b*vx) with a = b = 1 on the interior
-        float* vx = fw.f[0] + off;
-        float* vz = fw.f[1] + off;
-        *vx = *vx - rc * sx;
-        *vz = *vz - rc * sz;
+      float txxc[VX], tzzc[VX], txzc[VX], tzzu[VX], txzd[VX];
+      vload<VX>(txx, txxc);
+      vload<VX>(tzz, tzzc);
+      vload<VX>(txz, txzc);
+      if (zl) vload<VX>(tzz - g.nx, tzzu); else vzero<VX>(tzzu);
+      if (zr) vload<VX>(txz + g.nx, txzd); else vzero<VX>(txzd);
+      const float txxr = x0 + VX < g.nx ? txx[VX] : 0.0f;
+      const float txzl = x0 > 0 ? txz[-1] : 0.0f;
+      float fvx[VX], fvz[VX];
+      vload<VX>(fw.f[0] + off, fvx);
+      vload<VX>(fw.f[1] + off, fvz);
+      bool anyr = false;
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const int x = x0 + e;
+        const float sx = (x < g.nx - 1 ? (e == VX - 1 ? txxr : txxc[e + 1]) - txxc[e] : 0.0f) +
+                         (zr ? txzd[e] - txzc[e] : 0.0f);
+        const float sz = (x > 0 ? txzc[e] - (e == 0 ? txzl : txzc[e - 1]) : 0.0f) +
+                         (zl ? tzzc[e] - tzzu[e] : 0.0f);
+        if (grad[e]) gacc[e] += ai[e] * sgn * (sx * vxc[e] + sz * vzc[e]);
+        if (rebuild[e]) {
+          // forward vx' = a*(Rc*sx + b*vx) with a = b = 1 on the interior
+          fvx[e] = fvx[e] - rc[e] * sx;
+          fvz[e] = fvz[e] - rc[e] * sz;
+          anyr = true;
+        }
+      }
+      if (anyr) {
+        vstore<VX>(fw.f[0] + off, fvx);
+        vstore<VX>(fw.f[1] + off, fvz);
       }
     }
   }
-  if (grad) gRc[(int64_t)grp * 4 * P + i] += gacc;
+#pragma unroll
+  for (int e = 0; e < VX; ++e)
+    if (grad[e]) gRc[(int64_t)grp * 4 * P + i + e] += gacc[e];
 }
 
 // Transpose of stage T (see k_el_adjV).  fw: the stage's input velocities
 // (for the coefficient gradients) and, with recon, the stresses to rebuild in
 // place on the interior (the reverse of T).  gradients: planes 0..2 of the
 // group's 4-plane block (lam+2mu, lam, Mc).
-template <int S>
+template <int S, int VX>
 __global__ __launch_bounds__(256) void k_el_adjT(const float* __restrict__ cf, float sdti,
                                                  float sgn, Region rg, CFields in, Fields out,
                                                  Fields fw, float* __restrict__ gco, int gmode,
                                                  int recon, int B, Grid g, TileMap tm) {
-  int z, x, grp;
-  if (!cell_of(tm, g, z, x, grp)) return;
-  const int i = z * g.nx + x;
+  int z, x0, grp;
+  if (!vcell_of<VX>(tm, g, z, x0, grp)) return;
+  const int i = z * g.nx + x0;
   const int64_t P = g.plane;
   const float* A = cf + kAinv * P;
   // per-neighbour coefficients of q1 = sdti*(l2m*u1 + lam*u2), q2 = sdti*(lam*u1
@@ -227,78 +340,144 @@ __global__ __launch_bounds__(256) void k_el_adjT(const float* __restrict__ cf, f
   struct C3 { float a, l2m, lam, mc; };
   auto co = [&](int zz, int xx) {
     C3 r{0.0f, 0.0f, 0.0f, 0.0f};
-    if (in_region(zz, xx, rg)) {
+    if (zz >= 0 && zz < g.nz && xx >= 0 && xx < g.nx && in_region(zz, xx, rg)) {
       const int j = zz * g.nx + xx;
       r = C3{A[j], cf[kL2M * P + j], cf[kLam * P + j], cf[kMc * P + j]};
     }
     return r;
   };
-  const bool reg = in_region(z, x, rg);
-  const C3 k0 = co(z, x);
-  const C3 kxp = x < g.nx - 1 ? co(z, x + 1) : C3{0, 0, 0, 0};
-  const C3 kzp = z < g.nz - 1 ? co(z + 1, x) : C3{0, 0, 0, 0};
-  const C3 kxm = x > 0 ? co(z, x - 1) : C3{0, 0, 0, 0};
-  const C3 kzm = z > 0 ? co(z - 1, x) : C3{0, 0, 0, 0};
-  const float bm = reg ? cf[kBm * P + i] : 1.0f;
-  const bool grad = gco && reg && (gmode == 1 || interior(z, x, g));
-  const bool rebuild = recon && interior(z, x, g);
-  const bool xl = x >= 1, xr = x <= g.nx - 2, zl = z >= 1, zr = z <= g.nz - 2;
-  float g_l2m = 0.0f, g_lam = 0.0f, g_mc = 0.0f;
+  const bool zl = z >= 1, zr = z <= g.nz - 2;
+  C3 krow[VX + 2], kup[VX], kdn[VX];
+#pragma unroll
+  for (int e = -1; e <= VX; ++e) krow[e + 1] = co(z, x0 + e);
+  float bm[VX];
+  bool reg[VX], grad[VX], rebuild[VX];
+  bool anyg = false;
+#pragma unroll
+  for (int e = 0; e < VX; ++e) {
+    const int x = x0 + e;
+    kup[e] = co(z - 1, x);
+    kdn[e] = co(z + 1, x);
+    reg[e] = in_region(z, x, rg);
+    bm[e] = reg[e] ? cf[kBm * P + i + e] : 1.0f;
+    grad[e] = gco && reg[e] && (gmode == 1 || seis::interior(z, x, g));
+    rebuild[e] = recon && seis::interior(z, x, g);
+    anyg = anyg || grad[e] || rebuild[e];
+  }
+  float g_l2m[VX], g_lam[VX], g_mc[VX];
+  vzero<VX>(g_l2m);
+  vzero<VX>(g_lam);
+  vzero<VX>(g_mc);
   const int b0 = grp * S, nb = min(S, B - b0);
   for (int s = 0; s < nb; ++s) {
     const int64_t off = (int64_t)(b0 + s) * P + i;
     const float* axx = in.f[2] + off;
     const float* azz = in.f[3] + off;
     const float* axz = in.f[4] + off;
-    const float u1 = k0.a * axx[0], u2 = k0.a * azz[0], u3 = k0.a * axz[0];
-    const float q1 = sdti * (k0.l2m * u1 + k0.lam * u2);
-    const float q2 = sdti * (k0.lam * u1 + k0.l2m * u2);
-    const float q3 = sgn * k0.mc * u3;
-    // q1 at x+1, q3 at z+1 (Dx-^T, Dz-^T); q2 at z-1, q3 at x-1 (Dz+^T, Dx+^T)
-    float q1xp = 0.0f, q3zp = 0.0f, q2zm = 0.0f, q3xm = 0.0f;
-    if (xr) q1xp = sdti * (kxp.l2m * (kxp.a * axx[1]) + kxp.lam * (kxp.a * azz[1]));
-    if (zr) q3zp = sgn * kzp.mc * (kzp.a * axz[g.nx]);
-    if (zl) q2zm = sdti * (kzm.lam * (kzm.a * axx[-g.nx]) + kzm.l2m * (kzm.a * azz[-g.nx]));
-    if (xl) q3xm = sgn * kxm.mc * (kxm.a * axz[-1]);
-    out.f[0][off] = in.f[0][off] + ((xl ? q1 : 0.0f) - q1xp) + ((zl ? q3 : 0.0f) - q3zp);
-    out.f[1][off] = in.f[1][off] + (q2zm - (zr ? q2 : 0.0f)) + (q3xm - (xr ? q3 : 0.0f));
-    if (reg) {
-      out.f[2][off] = bm * u1;
-      out.f[3][off] = bm * u2;
-      out.f[4][off] = bm * u3;
-    } else {
-      out.f[2][off] = axx[0];
-      out.f[3][off] = azz[0];
-      out.f[4][off] = axz[0];
+    float xxc[VX], zzc[VX], xzc[VX], xxu[VX], zzu[VX], xzd[VX];
+    vload<VX>(axx, xxc);
+    vload<VX>(azz, zzc);
+    vload<VX>(axz, xzc);
+    if (zl) { vload<VX>(axx - g.nx, xxu); vload<VX>(azz - g.nx, zzu); }
+    else { vzero<VX>(xxu); vzero<VX>(zzu); }
+    if (zr) vload<VX>(axz + g.nx, xzd); else vzero<VX>(xzd);
+    const bool right = x0 + VX < g.nx;
+    const float xxr = right ? axx[VX] : 0.0f;
+    const float zzr = right ? azz[VX] : 0.0f;
+    const float xzl = x0 > 0 ? axz[-1] : 0.0f;
+    float i0[VX], i1[VX], o2[VX], o3[VX], o4[VX], u1[VX], u2[VX], u3[VX];
+    vload<VX>(in.f[0] + off, i0);
+    vload<VX>(in.f[1] + off, i1);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      const int x = x0 + e;
+      const bool xl = x >= 1, xr = x <= g.nx - 2;
+      const C3 k0 = krow[e + 1], kxp = krow[e + 2], kxm = krow[e];
+      u1[e] = k0.a * xxc[e];
+      u2[e] = k0.a * zzc[e];
+      u3[e] = k0.a * xzc[e];
+      const float q1 = sdti * (k0.l2m * u1[e] + k0.lam * u2[e]);
+      const float q2 = sdti * (k0.lam * u1[e] + k0.l2m * u2[e]);
+      const float q3 = sgn * k0.mc * u3[e];
+      float q1xp = 0.0f, q3zp = 0.0f, q2zm = 0.0f, q3xm = 0.0f;
+      if (xr) {
+        const float ax1 = e == VX - 1 ? xxr : xxc[e + 1];
+        const float az1 = e == VX - 1 ? zzr : zzc[e + 1];
+        q1xp = sdti * (kxp.l2m * (kxp.a * ax1) + kxp.lam * (kxp.a * az1));
+      }
+      if (zr) q3zp = sgn * kdn[e].mc * (kdn[e].a * xzd[e]);
+      if (zl) q2zm = sdti * (kup[e].lam * (kup[e].a * xxu[e]) + kup[e].l2m * (kup[e].a * zzu[e]));
+      if (xl) q3xm = sgn * kxm.mc * (kxm.a * (e == 0 ? xzl : xzc[e - 1]));
+      i0[e] = i0[e] + ((xl ? q1 : 0.0f) - q1xp) + ((zl ? q3 : 0.0f) - q3zp);
+      i1[e] = i1[e] + (q2zm - (zr ? q2 : 0.0f)) + (q3xm - (xr ? q3 : 0.0f));
+      if (reg[e]) {
+        o2[e] = bm[e] * u1[e];
+        o3[e] = bm[e] * u2[e];
+        o4[e] = bm[e] * u3[e];
+      } else {
+        o2[e] = xxc[e];
+        o3[e] = zzc[e];
+        o4[e] = xzc[e];
+      }
     }
-    if (grad || rebuild) {
+    vstore<VX>(out.f[0] + off, i0);
+    vstore<VX>(out.f[1] + off, i1);
+    vstore<VX>(out.f[2] + off, o2);
+    vstore<VX>(out.f[3] + off, o3);
+    vstore<VX>(out.f[4] + off, o4);
+    if (anyg) {
       const float* vx = fw.f[0] + off;
       const float* vz = fw.f[1] + off;
-      const float vx_x = x > 0 ? vx[0] - vx[-1] : 0.0f;
-      const float vz_z = z < g.nz - 1 ? vz[g.nx] - vz[0] : 0.0f;
-      const float vx_z = z > 0 ? vx[0] - vx[-g.nx] : 0.0f;
-      const float vz_x = x < g.nx - 1 ? vz[1] - vz[0] : 0.0f;
-      if (grad) {
-        g_l2m += sdti * (vx_x * u1 + vz_z * u2);
-        g_lam += sdti * (vz_z * u1 + vx_x * u2);
-        g_mc += sgn * (vz_x + vx_z) * u3;
+      float vxc[VX], vzc[VX], vxu[VX], vzd[VX];
+      vload<VX>(vx, vxc);
+      vload<VX>(vz, vzc);
+      if (zl) vload<VX>(vx - g.nx, vxu); else vzero<VX>(vxu);
+      if (zr) vload<VX>(vz + g.nx, vzd); else vzero<VX>(vzd);
+      const float vxl = x0 > 0 ? vx[-1] : 0.0f;
+      const float vzr = x0 + VX < g.nx ? vz[VX] : 0.0f;
+      float txx[VX], tzz[VX], txz[VX];
+      bool anyr = false;
+#pragma unroll
+      for (int e = 0; e < VX; ++e) anyr = anyr || rebuild[e];
+      if (anyr) {
+        vload<VX>(fw.f[2] + off, txx);
+        vload<VX>(fw.f[3] + off, tzz);
+        vload<VX>(fw.f[4] + off, txz);
       }
-      if (rebuild) {
-        // forward txx' = a*(dti*(...) + b*txx) with a = b = 1 on the interior
-        float* txx = fw.f[2] + off;
-        float* tzz = fw.f[3] + off;
-        float* txz = fw.f[4] + off;
-        *txx = *txx - sdti * (k0.l2m * vx_x + k0.lam * vz_z);
-        *tzz = *tzz - sdti * (k0.l2m * vz_z + k0.lam * vx_x);
-        *txz = *txz - sgn * k0.mc * (vz_x + vx_z);
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const int x = x0 + e;
+        const float vx_x = x > 0 ? vxc[e] - (e == 0 ? vxl : vxc[e - 1]) : 0.0f;
+        const float vz_z = zr ? vzd[e] - vzc[e] : 0.0f;
+        const float vx_z = zl ? vxc[e] - vxu[e] : 0.0f;
+        const float vz_x = x < g.nx - 1 ? (e == VX - 1 ? vzr : vzc[e + 1]) - vzc[e] : 0.0f;
+        if (grad[e]) {
+          g_l2m[e] += sdti * (vx_x * u1[e] + vz_z * u2[e]);
+          g_lam[e] += sdti * (vz_z * u1[e] + vx_x * u2[e]);
+          g_mc[e] += sgn * (vz_x + vx_z) * u3[e];
+        }
+        if (rebuild[e]) {
+          // forward txx' = a*(dti*(...) + b*txx) with a = b = 1 on the interior
+          const C3 k0 = krow[e + 1];
+          txx[e] = txx[e] - sdti * (k0.l2m * vx_x + k0.lam * vz_z);
+          tzz[e] = tzz[e] - sdti * (k0.l2m * vz_z + k0.lam * vx_x);
+          txz[e] = txz[e] - sgn * k0.mc * (vz_x + vx_z);
+        }
+      }
+      if (anyr) {
+        vstore<VX>(fw.f[2] + off, txx);
+        vstore<VX>(fw.f[3] + off, tzz);
+        vstore<VX>(fw.f[4] + off, txz);
       }
     }
   }
-  if (grad) {
-    float* G = gco + (int64_t)grp * 4 * P + i;
-    G[0] += g_l2m;
-    G[P] += g_lam;
-    G[2 * P] += g_mc;
+#pragma unroll
+  for (int e = 0; e < VX; ++e) {
+    if (!grad[e]) continue;
+    float* G = gco + (int64_t)grp * 4 * P + i + e;
+    G[0] += g_l2m[e];
+    G[P] += g_lam[e];
+    G[2 * P] += g_mc[e];
   }
 }
 
@@ -411,16 +590,39 @@ inline int zero(float* p, int64_t n, hipStream_t s) {
 
 inline bool valid_s(int S) { return S == 1 || S == 2 || S == 4 || S == 8; }
 
-#define SEIS_EL_LAUNCH(K, S, grid, s, ...)                                  \
+#define SEIS_EL_VX(K, S_, VX_, grid, s, ...)                                  \
+  switch (VX_) {                                                               \
+    case 1: K<S_, 1><<<grid, 256, 0, s>>>(__VA_ARGS__); break;                \
+    case 2: K<S_, 2><<<grid, 256, 0, s>>>(__VA_ARGS__); break;                \
+    default: K<S_, 4><<<grid, 256, 0, s>>>(__VA_ARGS__); break;               \
+  }
+
+// Launch of a vectorised stage kernel K<S, VX>.
+#define SEIS_EL_LAUNCH_V(K, S, VX, grid, s, ...)                             \
   do {                                                                      \
     switch (S) {                                                            \
-      case 1: K<1><<<grid, 256, 0, s>>>(__VA_ARGS__); break;               \
-      case 2: K<2><<<grid, 256, 0, s>>>(__VA_ARGS__); break;               \
-      case 4: K<4><<<grid, 256, 0, s>>>(__VA_ARGS__); break;               \
-      default: K<8><<<grid, 256, 0, s>>>(__VA_ARGS__); break;              \
+      case 1: SEIS_EL_VX(K, 1, VX, grid, s, __VA_ARGS__) break;            \
+      case 2: SEIS_EL_VX(K, 2, VX, grid, s, __VA_ARGS__) break;            \
+      case 4: SEIS_EL_VX(K, 4, VX, grid, s, __VA_ARGS__) break;            \
+      default: SEIS_EL_VX(K, 8, VX, grid, s, __VA_ARGS__) break;           \
     }                                                                       \
     SEIS_CHECK_LAUNCH();                                                    \
   } while (0)
+
+// Widest lane vector (4, 2, 1 cells) that divides nx with every given
+// pointer (coefficients, field sets of five, an extra plane) aligned to it.
+int stage_vx(int64_t nx, const float* coef, std::initializer_list<float* const*> sets,
+             const float* extra = nullptr) {
+  for (int v : {4, 2}) {
+    if (nx % v) continue;
+    auto al = [v](const void* p) { return (reinterpret_cast<uintptr_t>(p) % (4 * v)) == 0; };
+    bool ok = al(coef) && al(extra);
+    for (float* const* f : sets)
+      for (int k = 0; k < 5; ++k) ok = ok && al(f[k]);
+    if (ok) return v;
+  }
+  return 1;
+}
 
 int check(int64_t B, int64_t nz, int64_t nx, float dt, float h, int S) {
   SEIS_CHECK_ARG(B >= 1 && B <= 65535, "batch size %lld outside [1, 65535]", (long long)B);
@@ -459,16 +661,17 @@ int seis_elastic_stage(int32_t stage, const float* coef, float* const* fields, f
   SEIS_CHECK_ARG(stage == 0 || stage == 1, "stage must be 0 (stresses) or 1 (velocities)");
   const Grid g{(int)nz, (int)nx, 0, nz * nx};
   const int S = shots_per_group;
-  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  const int VX = stage_vx(nx, coef, {fields});
+  const TileMap tm = make_tiles(g, VX, 1, (int)((B + S - 1) / S));
   const Region rg{z0, z1, x0, x1};
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid(tm.per * kXcds);
   const float dti = dt * (1.0f / h);
   if (stage == 0)
-    SEIS_EL_LAUNCH(k_el_T, S, grid, s, coef, sign * dti, sign, rg, fields_of(fields), (int)B, g,
-                   tm);
+    SEIS_EL_LAUNCH_V(k_el_T, S, VX, grid, s, coef, sign * dti, sign, rg, fields_of(fields),
+                     (int)B, g, tm);
   else
-    SEIS_EL_LAUNCH(k_el_V, S, grid, s, coef, sign, rg, fields_of(fields), (int)B, g, tm);
+    SEIS_EL_LAUNCH_V(k_el_V, S, VX, grid, s, coef, sign, rg, fields_of(fields), (int)B, g, tm);
   return 0;
 }
 
@@ -484,19 +687,20 @@ int seis_elastic_stage_vjp(int32_t stage, const float* coef, float* const* cot_i
   SEIS_CHECK_ARG(stage == 0 || stage == 1, "stage must be 0 (stresses) or 1 (velocities)");
   const Grid g{(int)nz, (int)nx, 0, nz * nx};
   const int S = shots_per_group;
-  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  const int VX = stage_vx(nx, coef, {cot_in, cot_out, fwd}, grad_coef);
+  const TileMap tm = make_tiles(g, VX, 1, (int)((B + S - 1) / S));
   const Region rg{z0, z1, x0, x1};
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid(tm.per * kXcds);
   const float dti = dt * (1.0f / h);
   const int gmode = grad_coef ? 1 : 0;
   if (stage == 0)
-    SEIS_EL_LAUNCH(k_el_adjT, S, grid, s, coef, sign * dti, sign, rg, cfields_of(cot_in),
-                   fields_of(cot_out), fields_of(fwd), grad_coef, gmode, 0, (int)B, g, tm);
+    SEIS_EL_LAUNCH_V(k_el_adjT, S, VX, grid, s, coef, sign * dti, sign, rg, cfields_of(cot_in),
+                     fields_of(cot_out), fields_of(fwd), grad_coef, gmode, 0, (int)B, g, tm);
   else
-    SEIS_EL_LAUNCH(k_el_adjV, S, grid, s, coef, sign, rg, cfields_of(cot_in), fields_of(cot_out),
-                   fields_of(fwd), grad_coef ? grad_coef + 3 * g.plane : nullptr, gmode, 0,
-                   (int)B, g, tm);
+    SEIS_EL_LAUNCH_V(k_el_adjV, S, VX, grid, s, coef, sign, rg, cfields_of(cot_in),
+                     fields_of(cot_out), fields_of(fwd),
+                     grad_coef ? grad_coef + 3 * g.plane : nullptr, gmode, 0, (int)B, g, tm);
   return 0;
 }
 
@@ -529,7 +733,8 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
   const Grid g{(int)nz, (int)nx, npml, nz * nx};
   const int S = shots_per_group;
-  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  const int VX = stage_vx(nx, coef, {fields});
+  const TileMap tm = make_tiles(g, VX, 1, (int)((B + S - 1) / S));
   const Region all{0, (int)nz, 0, (int)nx};
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid(tm.per * kXcds);
@@ -540,8 +745,8 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
     if (int e = zero(F.f[f], field, s)) return e;
   for (int64_t n = 0; n < nt; ++n) {
     SEIS_PROF(SEIS_KERNEL_ELASTIC_FWD, s);
-    SEIS_EL_LAUNCH(k_el_T, S, grid, s, coef, dti, 1.0f, all, F, (int)B, g, tm);
-    SEIS_EL_LAUNCH(k_el_V, S, grid, s, coef, 1.0f, all, F, (int)B, g, tm);
+    SEIS_EL_LAUNCH_V(k_el_T, S, VX, grid, s, coef, dti, 1.0f, all, F, (int)B, g, tm);
+    SEIS_EL_LAUNCH_V(k_el_V, S, VX, grid, s, coef, 1.0f, all, F, (int)B, g, tm);
     SEIS_PROF(SEIS_KERNEL_ELASTIC_FWD, s);
     k_el_source<<<blocks_for(B), 256, 0, s>>>(F, src_zx, wavelet + n, 1.0f, src_mask, (int)B, g);
     SEIS_CHECK_LAUNCH();
@@ -590,7 +795,8 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
   const Grid g{(int)nz, (int)nx, npml, nz * nx};
   const int S = shots_per_group;
   const int G = (int)((B + S - 1) / S);
-  const TileMap tm = make_tiles(g, 1, 1, G);
+  const int VX = stage_vx(nx, coef, {fields, adj_a, adj_b}, grad_coef);
+  const TileMap tm = make_tiles(g, VX, 1, G);
   const Region all{0, (int)nz, 0, (int)nx};
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid(tm.per * kXcds);
@@ -618,9 +824,9 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
     // X holds the cotangent of state n+1.  recon rebuilds state n.
     const int recon = n >= 1 ? 1 : 0;
     SEIS_PROF(SEIS_KERNEL_ELASTIC_BWD, s);
-    SEIS_EL_LAUNCH(k_el_adjV, S, grid, s, coef, 1.0f, all,
-                   CFields{{X.f[0], X.f[1], X.f[2], X.f[3], X.f[4]}}, Y, F,
-                   grad_coef + 3 * g.plane, 2, recon, (int)B, g, tm);
+    SEIS_EL_LAUNCH_V(k_el_adjV, S, VX, grid, s, coef, 1.0f, all,
+                     CFields{{X.f[0], X.f[1], X.f[2], X.f[3], X.f[4]}}, Y, F,
+                     grad_coef + 3 * g.plane, 2, recon, (int)B, g, tm);
     if (n == 0) {  // state 0 is zero: the stage-T gradient sees zero velocities
       if (int e = zero(F.f[0], field, s)) return e;
       if (int e = zero(F.f[1], field, s)) return e;
@@ -630,9 +836,9 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
                                                          0x3, ring_len, (int)B, g);
       SEIS_CHECK_LAUNCH();
     }
-    SEIS_EL_LAUNCH(k_el_adjT, S, grid, s, coef, dti, 1.0f, all,
-                   CFields{{Y.f[0], Y.f[1], Y.f[2], Y.f[3], Y.f[4]}}, X, F, grad_coef, 2, recon,
-                   (int)B, g, tm);
+    SEIS_EL_LAUNCH_V(k_el_adjT, S, VX, grid, s, coef, dti, 1.0f, all,
+                     CFields{{Y.f[0], Y.f[1], Y.f[2], Y.f[3], Y.f[4]}}, X, F, grad_coef, 2, recon,
+                     (int)B, g, tm);
     SEIS_PROF(SEIS_KERNEL_ELASTIC_BWD, s);
     if (n >= 1) {
       k_el_ring_restore<<<blocks_for(rwork), 256, 0, s>>>(F, ring + (n - 1) * 5 * B * ring_len,

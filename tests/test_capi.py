@@ -44,3 +44,29 @@ def test_ring_len_and_error_path():
                   0, None, None, 1, 120, 120, 1e-3, 10.0, 49, 3, 2, 4, None)
     with pytest.raises(RuntimeError, match="null pointer"):
         _lib.call("seis_acoustic_step", None, None, None, None, None, 1, 10, 10, 1e-3, 10.0, None)
+
+
+def test_batch_and_unet_argument_checks():
+    """Host-side validation of the batch planner and the U-Net entry points
+    (returns before any device call)."""
+    import ctypes
+    from seistorch_amd import _lib
+    shape = (ctypes.c_int64 * 10)(200, 300, 4, 500, 300, 220, 302, 4, 400, 302)
+    dth = (ctypes.c_float * 4)(1e-3, 10.0, 1e-3, 10.0)
+    args = [None] * 8 + [0, None, None, None, 49]  # vp .. snaps, snap_every, u_a, u_b, table, npml
+    with pytest.raises(RuntimeError, match="instance count"):
+        _lib.call("seis_acoustic_forward_batch", 0, shape, dth, *args, 4, 2, 4, 0, None)
+    with pytest.raises(RuntimeError, match="instance 1: vec_x 4 does not divide nx = 302"):
+        _lib.call("seis_acoustic_forward_batch", 2, shape, dth, *args, 4, 2, 4, 0, None)
+    with pytest.raises(RuntimeError, match="launch shape"):
+        _lib.call("seis_acoustic_forward_batch", 2, shape, dth, *args, 3, 2, 1, 0, None)
+    with pytest.raises(RuntimeError, match="null pointer"):
+        _lib.call("seis_acoustic_forward_batch", 2, shape, dth, *args, 4, 2, 2, 0, None)
+    bad_dt = (ctypes.c_float * 4)(1e-3, 10.0, 0.0, 10.0)
+    with pytest.raises(RuntimeError, match="instance 1: dt and h must be positive"):
+        _lib.call("seis_acoustic_forward_batch", 2, shape, bad_dt, *args, 4, 2, 2, 0, None)
+    with pytest.raises(RuntimeError, match="multiple of 8"):
+        _lib.call("seis_unet_conv3x3", None, 1, 16, 16, 12, None, None, 32, 1, None, None, None,
+                  None)
+    with pytest.raises(RuntimeError, match="even H, W"):
+        _lib.call("seis_unet_avgpool2", None, 1, 15, 16, 32, None, None)

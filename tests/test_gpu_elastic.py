@@ -167,3 +167,26 @@ def test_gradient_finite_difference_fp32():
             fd = (vals[0] - vals[1]) / (2 * step)
             print(f"elastic FD param {which} cell ({i},{j}): fd {fd:.6e} adjoint {g[i, j]:.6e}")
             assert abs(fd - g[i, j]) <= 2e-2 * max(abs(fd), 1e-2 * np.abs(g).max())
+
+
+@pytest.mark.parametrize("nx_i", [61, 62, 60])
+def test_lane_vector_widths(nx_i):
+    """Widths whose padded nx takes 1-, 2- and 4-cell lane vectors: records
+    bit-identical to the FP32 restatement, gradients as in the adjoint test."""
+    c = elastic_case(nx_i=nx_i, nt=150)
+    t = elastic_dev(c)
+    m = [t[k].clone().requires_grad_(True) for k in ("vp", "vs", "rho")]
+    rec = _records_fused(c, t, vp=m[0], vs=m[1], rho=m[2])
+    rr = torch.cat(ref_torch.elastic_forward(c.vp, c.vs, c.rho, c.d, c.dt, c.h, c.w, c.src,
+                                             c.rec), 1).numpy()
+    assert rel_l2(rec.detach().cpu().numpy(), rr) == 0.0
+    rng = np.random.default_rng(nx_i)
+    G = (rng.standard_normal(rr.shape) * np.abs(rr).max()).astype(np.float32)
+    rec.backward(torch.from_numpy(G).cuda())
+    _, gr, _ = ref_torch.elastic_gradient_ad(c.vp, c.vs, c.rho, c.d, c.dt, c.h, c.w, c.src,
+                                             c.rec, None, rec_cotangent=G)
+    I = (slice(50, -50), slice(50, -50))
+    for name, a, r in zip(("vp", "vs", "rho"), m, gr):
+        e = rel_l2(a.grad.cpu().numpy()[I], r.numpy()[I])
+        print(f"elastic nx={c.nx} d{name}: HIP vs ref_torch AD {e:.2e}")
+        assert e < 20 * TOL
