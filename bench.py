@@ -232,9 +232,18 @@ def run_generator(args, dev):
         # algorithmic bytes of all timed launches over their summed duration
         tot_ms, n_launch = kern["k_bwd_batch"]
         achieved = BYTES_PER_CELL["k_bwd_batch"] * cells * args.steps / (tot_ms / 1e3)
+        # measured HBM bytes per cell-update (rocprof counters) x the average
+        # cell-updates of one launch
+        traffic = None
+        tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                             "traffic.json")
+        per_cell = json.load(open(tpath)).get("k_bwd_batch_bytes_per_cell") \
+            if os.path.exists(tpath) else None
+        if per_cell and n_launch:
+            traffic = round(per_cell * cells * args.steps / n_launch)
         roof = {"bound": "hbm", "kernel": "k_bwd_batch", "achieved": round(achieved / 1e9, 1),
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                "traffic": None, "avg_launch_us": round(tot_ms / max(n_launch, 1) * 1e3, 2),
+                "traffic": traffic, "avg_launch_us": round(tot_ms / max(n_launch, 1) * 1e3, 2),
                 "launches": n_launch, "bytes_per_cell_update": BYTES_PER_CELL["k_bwd_batch"]}
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
