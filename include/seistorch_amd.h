@@ -39,7 +39,8 @@ int32_t seis_abi_version(void);
 #define SEIS_KERNEL_HABC_BWD 5
 #define SEIS_KERNEL_BATCH_FWD 6  /* k_fwd_batch, direct launches only */
 #define SEIS_KERNEL_BATCH_BWD 7  /* k_bwd_batch, direct launches only */
-#define SEIS_PROFILE_KERNELS 8
+#define SEIS_KERNEL_ACOUSTIC_FWD2 8  /* k_fwd2: two forward steps per launch */
+#define SEIS_PROFILE_KERNELS 9
 int seis_profile_enable(int32_t on);
 int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
 
@@ -87,12 +88,15 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
  * u^{nt} and u_b holds u^{nt-1}.  Launch shape: shots_per_group (1, 2, 4
  * or 8) shots share one thread (model reads amortised over them), each
  * thread owns rows_per_thread (1, 2 or 4) consecutive rows of vec_x (1, 2 or
- * 4; must divide nx, pointers vec_x*4-byte aligned) consecutive cells. */
+ * 4; must divide nx, pointers vec_x*4-byte aligned) consecutive cells.
+ * If u_c and u_d (two more (B, nz, nx) buffers, distinct from u_a, u_b) are
+ * given, two time steps run per launch (the intermediate field is staged in
+ * LDS), with the same results; NULL keeps one step per launch. */
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
-                          float* u_a, float* u_b, int64_t B, int64_t nz, int64_t nx, float dt,
-                          float h, int32_t npml, int32_t shots_per_group,
+                          float* u_a, float* u_b, float* u_c, float* u_d, int64_t B, int64_t nz,
+                          int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
                           int32_t rows_per_thread, int32_t vec_x, void* stream);
 
 /* Adjoint of seis_acoustic_forward (replaces the boundary-saving backward of

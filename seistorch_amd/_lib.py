@@ -31,8 +31,8 @@ SIGNATURES = {
     "seis_acoustic_step_vjp": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64,
                                               _f, _f, _p]),
     "seis_acoustic_forward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
-                                             _p, _p, _i64, _i64, _i64, _f, _f, _i32, _i32, _i32,
-                                             _i32, _p]),
+                                             _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32, _i32,
+                                             _i32, _i32, _p]),
     "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
                                               _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
                                               _i32, _i32, _i32, _i32, _p]),

@@ -179,6 +179,151 @@ __device__ __forceinline__ void fwd_tile(
   }
 }
 
+// u^{n+1} of one cell from u^n (f) and u^{n-1} (fp) at (z, x), with the zero
+// padding of the Laplacian and the source sample; the same operands and
+// helpers as fwd_tile, so the value is bit-identical to the one the cell's
+// own tile computes.
+__device__ __forceinline__ float step_cell(const float* __restrict__ vp,
+                                           const float* __restrict__ d,
+                                           const float* __restrict__ f,
+                                           const float* __restrict__ fp, int z, int x, Grid g,
+                                           const seis::TimeConsts& k, int sz, int sx, float w) {
+  const int i = z * g.nx + x;
+  const float A = 1.0f / (k.idt2 + d[i] * k.idt);
+  const float Bc = k.idt2 - d[i] * k.idt;
+  const float c = vp[i];
+  const float zm = z > 0 ? f[i - g.nx] : 0.0f;
+  const float zp = z < g.nz - 1 ? f[i + g.nx] : 0.0f;
+  const float xm = x > 0 ? f[i - 1] : 0.0f;
+  const float xp = x < g.nx - 1 ? f[i + 1] : 0.0f;
+  const float L = lap_taps(zm, xm, f[i], xp, zp, k.ih2);
+  float v = acoustic_update(A, Bc, k.two_idt2, f[i], fp[i], c * c, L);
+  if (z == sz && x == sx) v += w;
+  return v;
+}
+
+// Two forward steps per launch: u^{n+1} = step(u^n, u^{n-1}) + w[n] at the
+// source, u^{n+2} = step(u^{n+1}, u^n) + w[n+1].  Reads ua = u^n, ub =
+// u^{n-1}; writes uc = u^{n+1}, ud = u^{n+2} (four distinct buffers, since
+// neighbouring tiles still read u^n and u^{n-1} around this one).  A wave is
+// one row of 64 lanes; each lane computes u^{n+1} on its RZ rows plus the row
+// above and below (recomputed from u^n and u^{n-1} with the same operands as
+// the neighbouring thread row, so bit-identical) and takes its x-neighbours
+// from the adjacent lanes by cross-lane shuffles.  Tiles overlap by one lane
+// vector on each side: lanes 0 and 63 only supply u^{n+1} to lanes 1 and 62,
+// and the 62 inner lanes write both steps (kFwd2Out * VX columns per tile).
+// No LDS, no barriers; each field crosses HBM once per two steps.
+constexpr int kFwd2Out = kLanes - 2;
+
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_fwd2(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ ua,
+    const float* __restrict__ ub, float* __restrict__ uc, float* __restrict__ ud,
+    const int32_t* __restrict__ src, const float* __restrict__ wn, int B, Grid g,
+    seis::TimeConsts k, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
+  const int lane = (int)(threadIdx.x % kLanes);
+  const int x0 = (tx * kFwd2Out + lane - 1) * VX;  // lane 0 starts one vector left
+  const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
+  if (zb >= g.nz) return;  // uniform over the wave
+  const bool xin = x0 >= 0 && x0 < g.nx;  // nx is a multiple of VX
+  const bool inner = lane > 0 && lane < kLanes - 1;
+  const float w0 = wn[0], w1 = wn[1];
+  // coefficients on the rows zb-1 .. zb+RZ (index q = z - zb + 1)
+  float A[RZ + 2][VX], Bc[RZ + 2][VX], c2[RZ + 2][VX];
+#pragma unroll
+  for (int q = 0; q < RZ + 2; ++q) {
+    const int z = min(max(zb - 1 + q, 0), g.nz - 1);
+    float c[VX], dd[VX];
+    if (xin) {
+      vload<VX>(vp + z * g.nx + x0, c);
+      vload<VX>(d + z * g.nx + x0, dd);
+    } else {
+      vzero<VX>(c);
+      vzero<VX>(dd);
+    }
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      A[q][e] = 1.0f / (k.idt2 + dd[e] * k.idt);
+      Bc[q][e] = k.idt2 - dd[e] * k.idt;
+      c2[q][e] = c[e] * c[e];
+    }
+  }
+  const int b0 = grp * S;
+  const int nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int b = b0 + s;
+    const int64_t off = (int64_t)b * g.plane;
+    const float* f = ua + off;
+    const float* fp = ub + off;
+    const int sz = src[2 * b], sx = src[2 * b + 1];
+    // u^n on rows zb-2 .. zb+RZ+1 (index p = z - zb + 2), zero outside
+    float un[RZ + 4][VX];
+#pragma unroll
+    for (int p = 0; p < RZ + 4; ++p) {
+      const int z = zb - 2 + p;
+      if (xin && z >= 0 && z < g.nz) vload<VX>(f + z * g.nx + x0, un[p]);
+      else vzero<VX>(un[p]);
+    }
+    // step 1 on rows zb-1 .. zb+RZ
+    float u1[RZ + 2][VX];
+#pragma unroll
+    for (int q = 0; q < RZ + 2; ++q) {
+      const int z = zb - 1 + q;
+      const bool zok = z >= 0 && z < g.nz;
+      float xm = __shfl_up(un[q + 1][VX - 1], 1, kLanes);
+      float xp = __shfl_down(un[q + 1][0], 1, kLanes);
+      if (lane == 0) xm = (xin && zok && x0 > 0) ? f[z * g.nx + x0 - 1] : 0.0f;
+      if (lane == kLanes - 1) xp = (xin && zok && x0 + VX < g.nx) ? f[z * g.nx + x0 + VX] : 0.0f;
+      if (xin && zok) {
+        float prev[VX];
+        vload<VX>(fp + z * g.nx + x0, prev);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+          const float left = e == 0 ? xm : un[q + 1][e - 1];
+          const float right = e == VX - 1 ? xp : un[q + 1][e + 1];
+          const float L = lap_taps(un[q][e], left, un[q + 1][e], right, un[q + 2][e], k.ih2);
+          float v = acoustic_update(A[q][e], Bc[q][e], k.two_idt2, un[q + 1][e], prev[e],
+                                    c2[q][e], L);
+          if (z == sz && x0 + e == sx) v += w0;
+          u1[q][e] = v;
+        }
+      } else {
+        vzero<VX>(u1[q]);  // outside the grid: the Laplacian's zero padding
+      }
+    }
+    if (inner && xin) {
+#pragma unroll
+      for (int r = 0; r < RZ; ++r) {
+        const int z = zb + r;
+        if (z < g.nz) vstore<VX>(uc + off + z * g.nx + x0, u1[r + 1]);
+      }
+    }
+    // step 2 on rows zb .. zb+RZ-1 (inner lanes)
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int z = zb + r;
+      const int q = r + 1;
+      const float xm = __shfl_up(u1[q][VX - 1], 1, kLanes);
+      const float xp = __shfl_down(u1[q][0], 1, kLanes);
+      if (!inner || !xin || z >= g.nz) continue;
+      float out[VX];
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const float left = e == 0 ? xm : u1[q][e - 1];
+        const float right = e == VX - 1 ? xp : u1[q][e + 1];
+        const float L = lap_taps(u1[q - 1][e], left, u1[q][e], right, u1[q + 1][e], k.ih2);
+        float v = acoustic_update(A[q][e], Bc[q][e], k.two_idt2, u1[q][e], un[q + 1][e],
+                                  c2[q][e], L);
+        if (z == sz && x0 + e == sx) v += w1;
+        out[e] = v;
+      }
+      vstore<VX>(ud + off + z * g.nx + x0, out);
+    }
+  }
+}
+
 // Receivers of the new field into one record row, and its ring into one slot.
 __global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
                                                 const int64_t* __restrict__ rec, int64_t R,
@@ -633,6 +778,7 @@ inline bool valid_shape(int S, int RZ, int VX) {
 
 SEIS_DEFINE_LAUNCH(launch_fwd, k_fwd)
 SEIS_DEFINE_LAUNCH(launch_bwd, k_bwd)
+SEIS_DEFINE_LAUNCH(launch_fwd2, k_fwd2)
 SEIS_DEFINE_LAUNCH(launch_fwd_batch, k_fwd_batch)
 SEIS_DEFINE_LAUNCH(launch_bwd_batch, k_bwd_batch)
 
@@ -657,6 +803,85 @@ int check_grid(int64_t B, int64_t nz, int64_t nx, float dt, float h) {
   SEIS_CHECK_ARG(nz * nx < (int64_t)1 << 31, "grid %lld x %lld too large", (long long)nz,
                  (long long)nx);
   SEIS_CHECK_ARG(dt > 0.0f && h > 0.0f, "dt and h must be positive (dt=%g, h=%g)", dt, h);
+  return 0;
+}
+
+// Forward loop with two steps per launch (k_fwd2) over four rotating
+// buffers; a last single step (k_fwd) when nt is odd.  Records, rings and
+// snapshots are taken from the buffers that hold each time level, exactly as
+// in the one-step loop; u^{nt} and u^{nt-1} end in u_a and u_b.
+int forward_two_step(const float* vp, const float* d, const float* wavelet, int64_t nt,
+                     const int32_t* src_zx, const int64_t* rec_flat, int64_t R, float* rec_out,
+                     float* ring, float* snaps, int32_t snap_every, float* u_a, float* u_b,
+                     float* u_c, float* u_d, int B, const Grid& g, const seis::TimeConsts& k,
+                     int S, int RZ, int VX, const TileMap& tm, int64_t ring_len, int64_t aux,
+                     hipStream_t s) {
+  const int64_t field = (int64_t)B * g.plane;
+  // A launch maps (cur, prev, n1, n2) -> (n2, n1, cur, prev), a period-4
+  // rotation, and an odd tail swaps cur and prev; the starting assignment is
+  // chosen so that u^{nt} ends in u_a and u^{nt-1} in u_b without copies.
+  const int64_t q = nt / 2;
+  float* want_cur = (nt % 2 == 0) ? u_a : u_b;
+  float* want_prev = (nt % 2 == 0) ? u_b : u_a;
+  float* P[4];
+  switch (q % 4) {
+    case 0: P[0] = want_cur; P[1] = want_prev; P[2] = u_c; P[3] = u_d; break;
+    case 1: P[3] = want_cur; P[2] = want_prev; P[0] = u_c; P[1] = u_d; break;
+    case 2: P[1] = want_cur; P[0] = want_prev; P[2] = u_c; P[3] = u_d; break;
+    default: P[2] = want_cur; P[3] = want_prev; P[0] = u_c; P[1] = u_d; break;
+  }
+  float* cur = P[0];   // u^n
+  float* prev = P[1];  // u^{n-1}
+  float* n1 = P[2];    // u^{n+1}
+  float* n2 = P[3];    // u^{n+2}
+  if (int e = fill(cur, field, 0.0f, s)) return e;
+  if (int e = fill(prev, field, 0.0f, s)) return e;
+  // k_fwd2 tiles: kFwd2Out lane vectors of output per tile in x
+  TileMap tm2 = tm;
+  tm2.ntx = (g.nx + kFwd2Out * VX - 1) / (kFwd2Out * VX);
+  tm2.per = (int)(((int64_t)tm2.ntx * tm2.ntz * tm2.groups + kXcds - 1) / kXcds);
+  auto record = [&](const float* u, int64_t n) -> int {  // u = u^{n+1}
+    float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
+    const int64_t work = R + (slot ? B * ring_len : 0);
+    if (work > 0 && aux > 0) {
+      k_record<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(
+          u, rec_flat, R, R ? rec_out + n * R : nullptr, slot, ring_len, B, g);
+      SEIS_CHECK_LAUNCH();
+    }
+    return 0;
+  };
+  auto snapshot = [&](const float* um, const float* um1, int64_t m) -> int {
+    if (snaps && m >= snap_every && m % snap_every == 0 && m <= nt - 2) {
+      float* slot_p = snaps + (m / snap_every - 1) * 2 * field;
+      return copy2(um, slot_p, um1, slot_p + field, field, s);
+    }
+    return 0;
+  };
+  int64_t n = 0;
+  for (; n + 1 < nt; n += 2) {
+    SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD2, s);
+    if (int e = launch_fwd2(S, RZ, VX, (int64_t)tm2.per * kXcds, s, vp, d, cur, prev, n1, n2,
+                            src_zx, wavelet + n, B, g, k, tm2))
+      return e;
+    SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD2, s);
+    if (int e = record(n1, n)) return e;
+    if (int e = record(n2, n + 1)) return e;
+    if (int e = snapshot(cur, n1, n)) return e;
+    if (int e = snapshot(n1, n2, n + 1)) return e;
+    float* t0 = cur; float* t1 = prev;
+    cur = n2; prev = n1; n1 = t0; n2 = t1;
+  }
+  if (n < nt) {  // odd nt: one more single step, u^{nt} written over u^{nt-2}
+    SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
+    if (int e = launch_fwd(S, RZ, VX, (int64_t)tm.per * kXcds, s, vp, d, cur, prev, src_zx,
+                           wavelet + n, B, g, k, tm))
+      return e;
+    SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
+    if (int e = record(prev, n)) return e;
+    if (int e = snapshot(cur, prev, n)) return e;
+    float* t = cur; cur = prev; prev = t;
+  }
+  // by the choice of P, u^{nt} is now in u_a and u^{nt-1} in u_b
   return 0;
 }
 
@@ -955,8 +1180,8 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
-                          float* u_a, float* u_b, int64_t B, int64_t nz, int64_t nx, float dt,
-                          float h, int32_t npml, int32_t shots_per_group,
+                          float* u_a, float* u_b, float* u_c, float* u_d, int64_t B, int64_t nz,
+                          int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
                           int32_t rows_per_thread, int32_t vec_x, void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
   SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, vec_x),
@@ -984,11 +1209,20 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   // After nt steps u^{nt} must sit in u_a: step n writes into the buffer that
   // holds u^{n-1}, so u^{n} lives in the step-0 "current" buffer iff n is even.
   const TileMap tm = make_tiles(g, vec_x, rows_per_thread, G);
+  const int64_t aux = R + ((ring && nt > 1) ? B * ring_len : 0);
+  if (u_c && u_d) {
+    SEIS_CHECK_ARG(aligned(u_c, 4 * vec_x) && aligned(u_d, 4 * vec_x),
+                   "u_c and u_d must be %d-byte aligned for vec_x %d", 4 * vec_x, vec_x);
+    SEIS_CHECK_ARG(u_c != u_a && u_c != u_b && u_d != u_a && u_d != u_b && u_c != u_d,
+                   "the four field buffers must be distinct");
+    return forward_two_step(vp, d, wavelet, nt, src_zx, rec_flat, R, rec_out, ring, snaps,
+                            snap_every, u_a, u_b, u_c, u_d, (int)B, g, k, S, rows_per_thread,
+                            vec_x, tm, ring_len, aux, s);
+  }
   float* cur = (nt % 2 == 0) ? u_a : u_b;
   float* prev = (nt % 2 == 0) ? u_b : u_a;
   if (int e = fill(u_a, B * g.plane, 0.0f, s)) return e;
   if (int e = fill(u_b, B * g.plane, 0.0f, s)) return e;
-  const int64_t aux = R + ((ring && nt > 1) ? B * ring_len : 0);
   for (int64_t n = 0; n < nt; ++n) {
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
     if (int e = launch_fwd(S, rows_per_thread, vec_x, (int64_t)tm.per * kXcds, s, vp, d, cur,

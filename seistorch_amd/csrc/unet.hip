@@ -55,50 +55,98 @@ __device__ __forceinline__ float gelu_grad(float v) {
 
 // ---------------------------------------------------------------- conv 3x3
 
-// One block = 64 consecutive pixels of one image row x NT output channels;
-// wave w owns pixels 16w..16w+15 and NT/16 accumulator tiles.
+// One block = 64 consecutive pixels of one image row x NT output channels.
+// The K loop runs over chunks of 32 reduction indices: the block's 64 x 32
+// input slice (gathered from the 9 taps with zero padding) and NT x 32 weight
+// slice are loaded once, 16 bytes per thread, into a double-buffered LDS tile
+// (rows padded to 40 elements against bank conflicts); each wave then reads
+// its 16-pixel A fragment and NT/16 B fragments from LDS and issues NT/16
+// MFMAs, while the next chunk's global loads are in flight.
 // OUT 0: z = acc + bias (bf16, optional) and y = act(z) (bf16);
 // OUT 1: y = acc (+ bias) in FP32 (the data gradient of a convolution).
+constexpr int kCvPx = 64;
+constexpr int kCvK = 32;
+constexpr int kCvRow = kCvK + 8;  // padded LDS row (elements)
+
+template <int NT>
+__device__ __forceinline__ void conv_fetch(const uint16_t* __restrict__ x, int n, int H, int W,
+                                           int Cin, int yrow, int px0,
+                                           const uint16_t* __restrict__ w, int co0, int Cout,
+                                           int K, int kc, uint4& ra, uint4& rb) {
+  const int t = threadIdx.x;
+  // A: pixel t/4, 8 reduction indices starting at (t%4)*8
+  {
+    const int p = t / 4, ks = (t % 4) * 8;
+    const int kb = kc * kCvK + ks;
+    const int tap = kb / Cin;
+    const int ci = kb - tap * Cin;
+    ra = make_uint4(0u, 0u, 0u, 0u);
+    if (tap < 9) {
+      const int yy = yrow + tap / 3 - 1;
+      const int xx = px0 + p + tap % 3 - 1;
+      if (px0 + p < W && yy >= 0 && yy < H && xx >= 0 && xx < W)
+        ra = *reinterpret_cast<const uint4*>(x + (((int64_t)n * H + yy) * W + xx) * Cin + ci);
+    }
+  }
+  // B: output channel t/4 (threads beyond NT*4 load nothing)
+  rb = make_uint4(0u, 0u, 0u, 0u);
+  if (t < NT * 4) {
+    const int co = t / 4, ks = (t % 4) * 8;
+    const int kb = kc * kCvK + ks;
+    if (kb < K && co0 + co < Cout)
+      rb = *reinterpret_cast<const uint4*>(w + (int64_t)(co0 + co) * K + kb);
+  }
+}
+
 template <int NT, int OUT>
 __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ x, int N, int H,
                                                  int W, int Cin, const uint16_t* __restrict__ w,
                                                  const float* __restrict__ bias, int Cout, int act,
                                                  uint16_t* __restrict__ z, uint16_t* __restrict__ y,
                                                  float* __restrict__ yf) {
-  const int segs = (W + 63) / 64;
+  __shared__ __align__(16) uint16_t sA[2][kCvPx][kCvRow];
+  __shared__ __align__(16) uint16_t sB[2][NT][kCvRow];
+  const int segs = (W + kCvPx - 1) / kCvPx;
   const int seg = blockIdx.x % segs;
   const int row = blockIdx.x / segs;  // n*H + yrow
   const int yrow = row % H;
   const int n = row / H;
+  const int px0 = seg * kCvPx;
   const int co0 = blockIdx.y * NT;
   const int wave = threadIdx.x / kWave;
   const int l = threadIdx.x % kWave;
   const int m = l % 16, kq = l / 16;
-  const int px = seg * 64 + wave * 16 + m;  // pixel of this lane's A row
   const int K = 9 * Cin;
-  const int nk = (K + 31) / 32;
+  const int nk = (K + kCvK - 1) / kCvK;
+  const int t = threadIdx.x;
   floatx4 acc[NT / 16];
 #pragma unroll
   for (int j = 0; j < NT / 16; ++j) acc[j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  uint4 ra, rb;
+  conv_fetch<NT>(x, n, H, W, Cin, yrow, px0, w, co0, Cout, K, 0, ra, rb);
+  *reinterpret_cast<uint4*>(&sA[0][t / 4][(t % 4) * 8]) = ra;
+  if (t < NT * 4) *reinterpret_cast<uint4*>(&sB[0][t / 4][(t % 4) * 8]) = rb;
+  __syncthreads();
   for (int kc = 0; kc < nk; ++kc) {
-    const int kb = kc * 32 + kq * 8;
-    const int tap = kb / Cin;
-    const int ci = kb - tap * Cin;
-    bf16x8 a = zero8();
-    if (tap < 9) {
-      const int yy = yrow + tap / 3 - 1;
-      const int xx = px + tap % 3 - 1;
-      if (px < W && yy >= 0 && yy < H && xx >= 0 && xx < W)
-        a = load8(x + (((int64_t)n * H + yy) * W + xx) * Cin + ci);
-    }
+    const int buf = kc & 1;
+    const bool more = kc + 1 < nk;
+    if (more) conv_fetch<NT>(x, n, H, W, Cin, yrow, px0, w, co0, Cout, K, kc + 1, ra, rb);
+    const bf16x8 a = __builtin_bit_cast(
+        bf16x8, *reinterpret_cast<const uint4*>(&sA[buf][wave * 16 + m][kq * 8]));
 #pragma unroll
     for (int j = 0; j < NT / 16; ++j) {
-      const int co = co0 + j * 16 + m;
-      const bf16x8 b = (tap < 9 && co < Cout) ? load8(w + (int64_t)co * K + kb) : zero8();
+      const bf16x8 b = __builtin_bit_cast(
+          bf16x8, *reinterpret_cast<const uint4*>(&sB[buf][j * 16 + m][kq * 8]));
       acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
     }
+    if (more) {
+      // the other buffer was last read in iteration kc-1, before its barrier
+      *reinterpret_cast<uint4*>(&sA[buf ^ 1][t / 4][(t % 4) * 8]) = ra;
+      if (t < NT * 4) *reinterpret_cast<uint4*>(&sB[buf ^ 1][t / 4][(t % 4) * 8]) = rb;
+    }
+    __syncthreads();
   }
-  const int pbase = seg * 64 + wave * 16 + kq * 4;
+  const int pbase = px0 + wave * 16 + kq * 4;
 #pragma unroll
   for (int j = 0; j < NT / 16; ++j) {
     const int co = co0 + j * 16 + m;
@@ -169,23 +217,39 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t* __restric
     const int yrow = (int)(row % H);
     const int n = (int)(row / H);
     __syncthreads();  // the previous segment's fragments have been read
-    for (int e = threadIdx.x; e < kWgCOT * kWgSeg; e += 256) {
-      const int xo = e % kWgSeg, co = e / kWgSeg;
+    // dz: 32 pixels x 64 channels, 8 channels (16 bytes) per load, written
+    // transposed to [co][x]
+    {
+      const int xo = threadIdx.x / (kWgCOT / 8), c8 = (threadIdx.x % (kWgCOT / 8)) * 8;
       const int px = xs0 + xo;
-      uint16_t v = 0;
-      if (px < W && co0 + co < Cout) v = dz[(((int64_t)n * H + yrow) * W + px) * Cout + co0 + co];
-      sdz[co][xo] = v;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (px < W && co0 + c8 < Cout)
+        v = *reinterpret_cast<const uint4*>(dz + (((int64_t)n * H + yrow) * W + px) * Cout +
+                                            co0 + c8);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sdz[c8 + j][xo] = e[j];
     }
-    for (int e = threadIdx.x; e < 9 * kWgCIT * kWgSeg; e += 256) {
-      const int xo = e % kWgSeg;
-      const int ci = (e / kWgSeg) % kWgCIT;
-      const int tap = e / (kWgSeg * kWgCIT);
-      const int yy = yrow + tap / 3 - 1;
-      const int xx = xs0 + xo + tap % 3 - 1;
-      uint16_t v = 0;
-      if (xs0 + xo < W && yy >= 0 && yy < H && xx >= 0 && xx < W && ci0 + ci < Cin)
-        v = x[(((int64_t)n * H + yy) * W + xx) * Cin + ci0 + ci];
-      sx[tap][ci][xo] = v;
+    // input window: rows yrow-1..yrow+1, x from xs0-1 to xs0+32, 32 channels,
+    // 8 channels per load; each load feeds the three horizontal taps
+    for (int e = threadIdx.x; e < 3 * (kWgSeg + 2) * (kWgCIT / 8); e += 256) {
+      const int c8 = (e % (kWgCIT / 8)) * 8;
+      const int xw = (e / (kWgCIT / 8)) % (kWgSeg + 2);
+      const int dy = e / ((kWgCIT / 8) * (kWgSeg + 2)) - 1;
+      const int yy = yrow + dy, xx = xs0 + xw - 1;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (yy >= 0 && yy < H && xx >= 0 && xx < W && ci0 + c8 < Cin)
+        v = *reinterpret_cast<const uint4*>(x + (((int64_t)n * H + yy) * W + xx) * Cin + ci0 +
+                                            c8);
+      const uint16_t* ev = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xo = xw - dx - 1;
+        if (xo < 0 || xo >= kWgSeg) continue;
+        const int tap = 3 * (dy + 1) + (dx + 1);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sx[tap][c8 + j][xo] = ev[j];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -445,7 +509,7 @@ int seis_unet_conv3x3(const uint16_t* x, int32_t N, int32_t H, int32_t W, int32_
   SEIS_CHECK_ARG(y || (!z && !act), "z and act apply to the bf16 output only");
   SEIS_CHECK_ARG((int64_t)N * H <= (int64_t)1 << 24, "too many image rows");
   hipStream_t s = (hipStream_t)stream;
-  const int segs = (W + 63) / 64;
+  const int segs = (W + kCvPx - 1) / kCvPx;
   const bool wide = Cout > 32;
   const dim3 grid((unsigned)(segs * N * H), (unsigned)((Cout + (wide ? 63 : 31)) / (wide ? 64 : 32)));
   if (y) {
@@ -472,8 +536,11 @@ int seis_unet_conv3x3_wgrad(const uint16_t* x, const uint16_t* dz, int32_t N, in
                             int32_t W, int32_t Cin, int32_t Cout, float* dw, float* db,
                             void* stream) {
   SEIS_CHECK_ARG(N >= 1 && H >= 1 && W >= 1, "empty image batch");
-  SEIS_CHECK_ARG(Cin >= 1 && Cout >= 1 && Cin <= 4096 && Cout <= 4096, "bad channel counts");
+  SEIS_CHECK_ARG(Cin >= 8 && Cout >= 8 && Cin <= 4096 && Cout <= 4096 && Cin % 8 == 0 &&
+                     Cout % 8 == 0,
+                 "channel counts (%d, %d) must be multiples of 8", Cin, Cout);
   SEIS_CHECK_ARG(x && dz && dw, "null pointer argument");
+  SEIS_CHECK_ARG(aligned16(x) && aligned16(dz), "x and dz must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   const int tiles = ((Cin + kWgCIT - 1) / kWgCIT) * ((Cout + kWgCOT - 1) / kWgCOT);
   const int64_t nseg = (int64_t)N * H * ((W + kWgSeg - 1) / kWgSeg);

@@ -179,11 +179,13 @@ class AcousticLoop(torch.autograd.Function):
     Inputs: vp, d (nz, nx); wavelet (nt,); src_zx (B, 2) int32 padded (z, x);
     rec_flat (R,) int64 flat indices into (B, nz, nx); B; dt; h; launch shape
     (shots_per_group, rows_per_thread, vec_x), None entries automatic;
-    snapshot_every (None: automatic, 0: ring only as in the reference).
+    snapshot_every (None: automatic, 0: ring only as in the reference);
+    two_step: two time steps per forward launch (two more field buffers).
     """
 
     @staticmethod
-    def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shape, snapshot_every):
+    def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shape, snapshot_every,
+                two_step=True):
         _lib.require_device(vp, d, wavelet, src_zx, rec_flat)
         nz, nx = vp.shape
         nt = wavelet.shape[0]
@@ -208,10 +210,15 @@ class AcousticLoop(torch.autograd.Function):
             n_snap = int(_lib.load().seis_snapshot_count(nt, K))
             if n_snap > 0:
                 snaps = torch.empty((n_snap, 2, B, nz, nx), device=vp.device, dtype=torch.float32)
+        u_c = u_d = None
+        if two_step and nt >= 2:
+            u_c = torch.empty_like(u_a)
+            u_d = torch.empty_like(u_a)
         _lib.call("seis_acoustic_forward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
-                  _lib.ptr(snaps), K, _lib.ptr(u_a), _lib.ptr(u_b), B, nz, nx, dt, h, NPML, S,
-                  rz, vx, _lib.stream_of(vp))
+                  _lib.ptr(snaps), K, _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(u_c), _lib.ptr(u_d),
+                  B, nz, nx, dt, h, NPML, S, rz, vx, _lib.stream_of(vp))
+        del u_c, u_d  # scratch of the two-step launches
         if need_grad:
             ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
             ctx.state = (ring, snaps, u_a, u_b)
@@ -239,18 +246,20 @@ class AcousticLoop(torch.autograd.Function):
                   _lib.ptr(snaps), K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b),
                   _lib.ptr(mu_a), _lib.ptr(mu_b), _lib.ptr(gpart),
                   _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, rz, vx, _lib.stream_of(vp))
-        return grad, None, None, None, None, None, None, None, None, None
+        return grad, None, None, None, None, None, None, None, None, None, None
 
 
 def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
-              snapshot_every=None, rows_per_thread=None, vec_x=None):
+              snapshot_every=None, rows_per_thread=None, vec_x=None, two_step=True):
     """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint.
 
     The launch shape -- shots_per_group (1, 2, 4, 8), rows_per_thread
     (1, 2, 4), vec_x (cells per lane along x: 1, 2, 4, dividing nx) --
     defaults to the autotuned values for this shape (``seistorch_amd.autotune``),
-    else to fixed defaults.
+    else to fixed defaults.  two_step runs two forward time steps per launch
+    (same results, half the forward's field traffic).
     """
     return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
                               rec_flat.contiguous(), int(B), _f(dt), _f(h),
-                              (shots_per_group, rows_per_thread, vec_x), snapshot_every)
+                              (shots_per_group, rows_per_thread, vec_x), snapshot_every,
+                              bool(two_step))

@@ -13,7 +13,8 @@ The preconditioned method feeds (gradient, model) to the network, which
 predicts the model perturbation; that step is accepted if the misfit
 decreases, else the network's direction scaled to the current plain step
 length is tried, and otherwise the plain step of the same iteration is
-taken (the fallback).
+taken (the fallback).  After two iterations in a row without an accepted
+network step, the network is consulted every third iteration only.
 """
 import time
 from dataclasses import dataclass, field
@@ -130,18 +131,23 @@ def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, time_budget=None
     hist = History()
     hist.add(0.0, p.misfit(vp), p.rmse(vp), "start")
     step = step0
+    misses = 0  # consecutive iterations whose network trials were all rejected
     for it in range(iters):
         loss, g = p.gradient(vp)
         vi = p.interior(vp).contiguous()
-        sc = precond.input_scales(g[None], vi[None])
-        d = net.forward(g[None].contiguous(), vi[None], sc)[0] * vscale
-        d[:p.mute] = 0.0
+        trials = []
+        # after two misses in a row the network is consulted every third
+        # iteration only, so a direction that does not help costs little
+        if misses < 2 or it % 3 == 0:
+            sc = precond.input_scales(g[None], vi[None])
+            d = net.forward(g[None].contiguous(), vi[None], sc)[0] * vscale
+            d[:p.mute] = 0.0
+            dmax = float(d.abs().max())
+            # the network's own step, then its direction at the plain step length
+            trials = [("net", d)]
+            if dmax > 0.0:
+                trials.append(("netdir", d * (step / dmax)))
         kind, m = "none", loss
-        dmax = float(d.abs().max())
-        # the network's own step, then its direction at the plain step length
-        trials = [("net", d)]
-        if dmax > 0.0:
-            trials.append(("netdir", d * (step / dmax)))
         for name, dv in trials:
             trial = p.pad(_clip(p, vi + dv))
             mt = p.misfit(trial)
@@ -150,6 +156,8 @@ def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, time_budget=None
                 if name == "netdir":
                     step *= 1.25
                 break
+        if trials:
+            misses = 0 if kind != "none" else misses + 1
         if kind == "none":  # fallback: the plain step of this iteration
             vp, m, step, ok = plain_step(p, vp, loss, g, step)
             kind = "plain" if ok else "none"
