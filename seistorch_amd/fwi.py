@@ -10,11 +10,12 @@ is muted within `mute` cells of the surface, as the reference's seabed cut
 the boundary strips are re-padded from its edges (geom.py:213-240).
 
 The preconditioned method feeds (gradient, model) to the network, which
-predicts the model perturbation; that step is accepted if the misfit
-decreases, else the network's direction scaled to the current plain step
-length is tried, and otherwise the plain step of the same iteration is
-taken (the fallback).  After two iterations in a row without an accepted
-network step, the network is consulted every third iteration only.
+predicts the model perturbation true - start; in its first iteration(s) the
+trial update is that perturbation plus the plain step, accepted if the
+misfit decreases, otherwise the plain step of the same iteration is taken
+(the fallback); later iterations are plain.  Applying the network at every
+iteration was measured to drive the model away (it was trained on gradients
+at smoothed start models only).
 """
 import time
 from dataclasses import dataclass, field
@@ -123,7 +124,15 @@ def run_plain(p, iters, step0=40.0, time_budget=None, log=None):
     return vp, hist
 
 
-def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, time_budget=None, log=None):
+def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, net_iters=1, time_budget=None,
+                       log=None):
+    """FWI whose first `net_iters` iterations add the network's predicted
+    perturbation to the plain step (one trial, accepted if the misfit drops,
+    else the plain step alone), then plain iterations.
+
+    The network was trained on gradients at smoothed start models, so it is
+    consulted where its inputs look like its training data; afterwards the
+    plain steepest-descent iterations continue from the improved model."""
     from . import precond
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -131,34 +140,22 @@ def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, time_budget=None
     hist = History()
     hist.add(0.0, p.misfit(vp), p.rmse(vp), "start")
     step = step0
-    misses = 0  # consecutive iterations whose network trials were all rejected
     for it in range(iters):
         loss, g = p.gradient(vp)
-        vi = p.interior(vp).contiguous()
-        trials = []
-        # after two misses in a row the network is consulted every third
-        # iteration only, so a direction that does not help costs little
-        if misses < 2 or it % 3 == 0:
+        kind, m = "none", loss
+        if it < net_iters:
+            vi = p.interior(vp).contiguous()
             sc = precond.input_scales(g[None], vi[None])
             d = net.forward(g[None].contiguous(), vi[None], sc)[0] * vscale
             d[:p.mute] = 0.0
-            dmax = float(d.abs().max())
-            # the network's own step, then its direction at the plain step length
-            trials = [("net", d)]
-            if dmax > 0.0:
-                trials.append(("netdir", d * (step / dmax)))
-        kind, m = "none", loss
-        for name, dv in trials:
-            trial = p.pad(_clip(p, vi + dv))
+            gmax = float(g.abs().max())
+            plain_dv = -step * g / gmax if gmax > 0.0 else torch.zeros_like(g)
+            trial = p.pad(_clip(p, vi + d + plain_dv))
             mt = p.misfit(trial)
             if mt < loss:
-                vp, m, kind = trial, mt, name
-                if name == "netdir":
-                    step *= 1.25
-                break
-        if trials:
-            misses = 0 if kind != "none" else misses + 1
-        if kind == "none":  # fallback: the plain step of this iteration
+                vp, m, kind = trial, mt, "net+plain"
+                step *= 1.25
+        if kind == "none":  # the plain step of this iteration (the fallback)
             vp, m, step, ok = plain_step(p, vp, loss, g, step)
             kind = "plain" if ok else "none"
             if ok:
