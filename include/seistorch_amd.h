@@ -246,13 +246,15 @@ int seis_elastic_chain(const float* grad_coef, int32_t groups, const float* vp, 
  * of all five fields of state n+1 in slot n ((nt-1) x 5 x B x ring_len);
  * snaps (if not NULL): state m for m = j*snap_every <= nt-1 in slot j-1
  * (5 x B x nz x nx each).  fields are zeroed first and hold state nt on
- * return. */
+ * return.  If scratch (five more (B, nz, nx) buffers) is not NULL, each step
+ * runs as one fused pass (both stages, ping-ponging between fields and
+ * scratch) with the same results. */
 int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
                          const int32_t* src_zx, int32_t src_mask, const int64_t* rec_flat,
                          int64_t R, const int32_t* rec_field, int32_t nch, float* rec_out,
                          float* ring, float* snaps, int32_t snap_every, float* const* fields,
-                         int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                         int32_t shots_per_group, void* stream);
+                         float* const* scratch, int64_t B, int64_t nz, int64_t nx, float dt,
+                         float h, int32_t npml, int32_t shots_per_group, void* stream);
 
 /* Adjoint of seis_elastic_forward with boundary-saving reconstruction
  * (replaces seistorch/checkpoint.py:108-230 and elastic.py:39-122 with the

@@ -58,8 +58,8 @@ SIGNATURES = {
     "seis_elastic_chain": (ctypes.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _i64, _i64, _f, _f,
                                           _p]),
     "seis_elastic_forward": (ctypes.c_int, [_p, _p, _i64, _p, _i32, _p, _i64, _p, _i32, _p, _p,
-                                            _p, _i32, _p, _i64, _i64, _i64, _f, _f, _i32, _i32,
-                                            _p]),
+                                            _p, _i32, _p, _p, _i64, _i64, _i64, _f, _f, _i32,
+                                            _i32, _p]),
     "seis_elastic_backward": (ctypes.c_int, [_p, _p, _i64, _p, _i32, _p, _i64, _p, _i32, _p, _p,
                                              _p, _i32, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
                                              _i32, _i32, _p]),

@@ -194,6 +194,129 @@ __global__ __launch_bounds__(256) void k_el_V(const float* __restrict__ cf, floa
   }
 }
 
+// One whole forward step (stage T then stage V, elastic.py:16-35) in one
+// pass, reading state n from Fi and writing state n+1 to Fo (distinct
+// buffers: neighbouring tiles still read state n around this one).  A wave
+// is one row of 64 lanes; each lane computes the stresses of its RZ rows and
+// of the rows above and below (recomputed with the same operands as the
+// neighbouring thread row, so bit-identical), then the velocities of its own
+// rows, taking x-neighbours from the adjacent lanes by cross-lane shuffles.
+// Tiles overlap by one lane vector on each side: lanes 0 and 63 only supply
+// neighbour values, the 62 inner lanes write.  Coefficients are re-read per
+// shot (L1/L2 hits) to keep the register footprint small.
+constexpr int kElOut = kLanes - 2;
+
+template <int S, int RZ, int VX>
+__device__ __forceinline__ void el_step_body(const float* __restrict__ cf, float dti, CFields Fi,
+                                             Fields Fo, int B, Grid g, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
+  const int lane = (int)(threadIdx.x % kLanes);
+  const int x0 = (tx * kElOut + lane - 1) * VX;
+  const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
+  if (zb >= g.nz) return;  // uniform over the wave
+  const bool xin = x0 >= 0 && x0 < g.nx;  // nx is a multiple of VX
+  const bool inner = lane > 0 && lane < kLanes - 1;
+  const int64_t P = g.plane;
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int64_t off = (int64_t)(b0 + s) * P;
+    const float* vx = Fi.f[0] + off;
+    const float* vz = Fi.f[1] + off;
+    // vx on rows zb-2 .. zb+RZ (index z - zb + 2), vz on rows zb-1 .. zb+RZ+1
+    // (index z - zb + 1); zero outside the grid
+    float VXr[RZ + 3][VX], VZr[RZ + 3][VX];
+#pragma unroll
+    for (int p = 0; p < RZ + 3; ++p) {
+      const int za = zb - 2 + p, zc = zb - 1 + p;
+      if (xin && za >= 0 && za < g.nz) vload<VX>(vx + za * g.nx + x0, VXr[p]);
+      else vzero<VX>(VXr[p]);
+      if (xin && zc >= 0 && zc < g.nz) vload<VX>(vz + zc * g.nx + x0, VZr[p]);
+      else vzero<VX>(VZr[p]);
+    }
+    // stage T on rows zb-1 .. zb+RZ (index q = z - zb + 1)
+    float TXX[RZ + 2][VX], TZZ[RZ + 2][VX], TXZ[RZ + 2][VX];
+#pragma unroll
+    for (int q = 0; q < RZ + 2; ++q) {
+      const int z = zb - 1 + q;
+      const bool zok = z >= 0 && z < g.nz;
+      // vx row z is VXr[q + 1], vz row z is VZr[q]
+      float vxl = __shfl_up(VXr[q + 1][VX - 1], 1, kLanes);
+      float vzr = __shfl_down(VZr[q][0], 1, kLanes);
+      if (lane == 0) vxl = (xin && zok && x0 > 0) ? vx[z * g.nx + x0 - 1] : 0.0f;
+      if (lane == kLanes - 1) vzr = (xin && zok && x0 + VX < g.nx) ? vz[z * g.nx + x0 + VX] : 0.0f;
+      if (xin && zok) {
+        const int i = z * g.nx + x0;
+        float l2m[VX], lam[VX], mc[VX], a[VX], bm[VX];
+        vload<VX>(cf + kL2M * P + i, l2m);
+        vload<VX>(cf + kLam * P + i, lam);
+        vload<VX>(cf + kMc * P + i, mc);
+        vload<VX>(cf + kAinv * P + i, a);
+        vload<VX>(cf + kBm * P + i, bm);
+        vload<VX>(Fi.f[2] + off + i, TXX[q]);
+        vload<VX>(Fi.f[3] + off + i, TZZ[q]);
+        vload<VX>(Fi.f[4] + off + i, TXZ[q]);
+        const bool zl = z > 0, zr = z < g.nz - 1;
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+          const int x = x0 + e;
+          const float vx_x = x > 0 ? VXr[q + 1][e] - (e == 0 ? vxl : VXr[q + 1][e - 1]) : 0.0f;
+          const float vz_z = zr ? VZr[q + 1][e] - VZr[q][e] : 0.0f;
+          const float vx_z = zl ? VXr[q + 1][e] - VXr[q][e] : 0.0f;
+          const float vz_x = x < g.nx - 1 ? (e == VX - 1 ? vzr : VZr[q][e + 1]) - VZr[q][e] : 0.0f;
+          TXX[q][e] = a[e] * (dti * (l2m[e] * vx_x + lam[e] * vz_z) + bm[e] * TXX[q][e]);
+          TZZ[q][e] = a[e] * (dti * (l2m[e] * vz_z + lam[e] * vx_x) + bm[e] * TZZ[q][e]);
+          TXZ[q][e] = a[e] * ((1.0f * mc[e]) * (vz_x + vx_z) + bm[e] * TXZ[q][e]);
+        }
+      } else {
+        vzero<VX>(TXX[q]);
+        vzero<VX>(TZZ[q]);
+        vzero<VX>(TXZ[q]);
+      }
+    }
+    // stage V on rows zb .. zb+RZ-1 from the new stresses; write both
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int z = zb + r;
+      const int q = r + 1;
+      const float txxr = __shfl_down(TXX[q][0], 1, kLanes);
+      const float txzl = __shfl_up(TXZ[q][VX - 1], 1, kLanes);
+      if (!inner || !xin || z >= g.nz) continue;
+      const int i = z * g.nx + x0;
+      float rc[VX], a[VX], bm[VX], vxo[VX], vzo[VX];
+      vload<VX>(cf + kRc * P + i, rc);
+      vload<VX>(cf + kAinv * P + i, a);
+      vload<VX>(cf + kBm * P + i, bm);
+      const bool zl = z > 0, zr = z < g.nz - 1;
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const int x = x0 + e;
+        const float txx_x = x < g.nx - 1 ? (e == VX - 1 ? txxr : TXX[q][e + 1]) - TXX[q][e] : 0.0f;
+        const float txz_z = zr ? TXZ[q + 1][e] - TXZ[q][e] : 0.0f;
+        const float tzz_z = zl ? TZZ[q][e] - TZZ[q - 1][e] : 0.0f;
+        const float txz_x = x > 0 ? TXZ[q][e] - (e == 0 ? txzl : TXZ[q][e - 1]) : 0.0f;
+        const float src = 1.0f * rc[e];
+        vxo[e] = a[e] * (src * (txx_x + txz_z) + bm[e] * VXr[q + 1][e]);
+        vzo[e] = a[e] * (src * (txz_x + tzz_z) + bm[e] * VZr[q][e]);
+      }
+      vstore<VX>(Fo.f[0] + off + i, vxo);
+      vstore<VX>(Fo.f[1] + off + i, vzo);
+      vstore<VX>(Fo.f[2] + off + i, TXX[q]);
+      vstore<VX>(Fo.f[3] + off + i, TZZ[q]);
+      vstore<VX>(Fo.f[4] + off + i, TXZ[q]);
+    }
+  }
+}
+
+constexpr int kElRZ = 2;  // rows per thread of the fused step
+
+template <int S, int VX>
+__global__ __launch_bounds__(256) void k_el_step_rz(const float* __restrict__ cf, float dti,
+                                                    CFields Fi, Fields Fo, int B, Grid g,
+                                                    TileMap tm) {
+  el_step_body<S, kElRZ, VX>(cf, dti, Fi, Fo, B, g, tm);
+}
+
 // Transpose of stage V.  in: cotangents of the stage outputs; out: of its
 // inputs.  fw: the stage's input fields (stresses used for the Rc gradient);
 // gmode 0 none, 1 every region cell, 2 interior cells only; with recon the
@@ -720,8 +843,8 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
                          const int32_t* src_zx, int32_t src_mask, const int64_t* rec_flat,
                          int64_t R, const int32_t* rec_field, int32_t nch, float* rec_out,
                          float* ring, float* snaps, int32_t snap_every, float* const* fields,
-                         int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                         int32_t shots_per_group, void* stream) {
+                         float* const* scratch, int64_t B, int64_t nz, int64_t nx, float dt,
+                         float h, int32_t npml, int32_t shots_per_group, void* stream) {
   if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
   SEIS_CHECK_ARG(coef && wavelet && src_zx && fields, "null pointer argument");
   for (int f = 0; f < 5; ++f) SEIS_CHECK_ARG(fields[f], "null field pointer");
@@ -733,20 +856,35 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
   const Grid g{(int)nz, (int)nx, npml, nz * nx};
   const int S = shots_per_group;
-  const int VX = stage_vx(nx, coef, {fields});
+  if (scratch)
+    for (int f = 0; f < 5; ++f) SEIS_CHECK_ARG(scratch[f], "null scratch field pointer");
+  const int VX = scratch ? stage_vx(nx, coef, {fields, scratch}) : stage_vx(nx, coef, {fields});
   const TileMap tm = make_tiles(g, VX, 1, (int)((B + S - 1) / S));
   const Region all{0, (int)nz, 0, (int)nx};
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid(tm.per * kXcds);
   const float dti = dt * (1.0f / h);
   const int64_t field = B * g.plane;
+  // fused steps ping-pong between fields and scratch; the start is chosen so
+  // that the last step writes `fields`
   Fields F = fields_of(fields);
+  Fields Fs = scratch ? fields_of(scratch) : F;
+  TileMap tm2 = make_tiles(g, VX, kElRZ, (int)((B + S - 1) / S));
+  tm2.ntx = (g.nx + kElOut * VX - 1) / (kElOut * VX);
+  tm2.per = (int)(((int64_t)tm2.ntx * tm2.ntz * tm2.groups + kXcds - 1) / kXcds);
+  if (scratch && nt % 2 == 1) { Fields t = F; F = Fs; Fs = t; }
   for (int f = 0; f < 5; ++f)
     if (int e = zero(F.f[f], field, s)) return e;
   for (int64_t n = 0; n < nt; ++n) {
     SEIS_PROF(SEIS_KERNEL_ELASTIC_FWD, s);
-    SEIS_EL_LAUNCH_V(k_el_T, S, VX, grid, s, coef, dti, 1.0f, all, F, (int)B, g, tm);
-    SEIS_EL_LAUNCH_V(k_el_V, S, VX, grid, s, coef, 1.0f, all, F, (int)B, g, tm);
+    if (scratch) {
+      SEIS_EL_LAUNCH_V(k_el_step_rz, S, VX, dim3(tm2.per * kXcds), s, coef, dti,
+                       CFields{{F.f[0], F.f[1], F.f[2], F.f[3], F.f[4]}}, Fs, (int)B, g, tm2);
+      Fields t = F; F = Fs; Fs = t;
+    } else {
+      SEIS_EL_LAUNCH_V(k_el_T, S, VX, grid, s, coef, dti, 1.0f, all, F, (int)B, g, tm);
+      SEIS_EL_LAUNCH_V(k_el_V, S, VX, grid, s, coef, 1.0f, all, F, (int)B, g, tm);
+    }
     SEIS_PROF(SEIS_KERNEL_ELASTIC_FWD, s);
     k_el_source<<<blocks_for(B), 256, 0, s>>>(F, src_zx, wavelet + n, 1.0f, src_mask, (int)B, g);
     SEIS_CHECK_LAUNCH();

@@ -175,7 +175,7 @@ class ElasticLoop(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, vp, vs, rho, d, wavelet, src_zx, rec_flat, B, dt, h, src_type, rec_type,
-                shots_per_group, snapshot_every):
+                shots_per_group, snapshot_every, fused=True):
         _lib.require_device(vp, vs, rho, d, wavelet, src_zx, rec_flat)
         from .acoustic import _check_acquisition
         need_grad = any(ctx.needs_input_grad[:3])
@@ -201,10 +201,13 @@ class ElasticLoop(torch.autograd.Function):
             n_snap = (nt - 1) // K if K > 0 else 0
             if n_snap > 0:
                 snaps = torch.empty((n_snap, 5, B, nz, nx), device=vp.device)
+        scratch = [torch.empty_like(fields[0]) for _ in range(5)] if fused else None
         _lib.call("seis_elastic_forward", _lib.ptr(cf), _lib.ptr(wavelet), nt, _lib.ptr(src_zx),
                   _mask(src_type), _lib.ptr(rec_flat), R, _lib.ptr(rf), nch, _lib.ptr(rec),
-                  _lib.ptr(ring), _lib.ptr(snaps), K, _lib.ptr_array(fields), B, nz, nx, dt, h,
-                  NPML, S, _lib.stream_of(vp))
+                  _lib.ptr(ring), _lib.ptr(snaps), K, _lib.ptr_array(fields),
+                  _lib.ptr_array(scratch) if fused else None, B, nz, nx, dt, h, NPML, S,
+                  _lib.stream_of(vp))
+        del scratch  # ping-pong buffers of the fused steps
         if need_grad:
             ctx.save_for_backward(vp, vs, rho, cf, wavelet, src_zx, rec_flat, rf)
             ctx.state = (ring, snaps, fields)
@@ -236,12 +239,13 @@ class ElasticLoop(torch.autograd.Function):
         _lib.call("seis_elastic_chain", _lib.ptr(gco), G, _lib.ptr(vp), _lib.ptr(vs),
                   _lib.ptr(rho), _lib.ptr(gvp), _lib.ptr(gvs), _lib.ptr(grho), nz, nx, dt, h,
                   _lib.stream_of(vp))
-        return (gvp, gvs, grho) + (None,) * 11
+        return (gvp, gvs, grho) + (None,) * 12
 
 
 def propagate(vp, vs, rho, d, wavelet, src_zx, rec_flat, B, dt, h, src_type=("txx", "tzz"),
-              rec_type=("vx", "vz"), shots_per_group=None, snapshot_every=None):
-    """Records (nt, R, nch) of B shots; differentiable w.r.t. vp, vs, rho."""
+              rec_type=("vx", "vz"), shots_per_group=None, snapshot_every=None, fused=True):
+    """Records (nt, R, nch) of B shots; differentiable w.r.t. vp, vs, rho.
+    fused: both stages of a forward step in one pass (five more field buffers)."""
     return ElasticLoop.apply(vp, vs, rho, d, wavelet.contiguous(), src_zx.contiguous(),
                              rec_flat.contiguous(), int(B), _f(dt), _f(h), tuple(src_type),
-                             tuple(rec_type), shots_per_group, snapshot_every)
+                             tuple(rec_type), shots_per_group, snapshot_every, bool(fused))
