@@ -24,10 +24,14 @@ METRIC = "Gcell-updates/s fwd+adjoint, % HBM roofline; wall-clock to target misf
 HBM_PEAK = 8.0e12  # B/s, MI355X
 # algorithmic bytes per cell-update of each timed kernel (fields touched once
 # per step, damping/coefficients read once per cell)
-BYTES_PER_CELL = {"k_fwd": 16.0, "k_bwd": 36.0, "k_fwd2": 32.0,  # k_fwd2: two steps/launch
-                   "k_el_fwd": 52.0, "k_el_bwd": 116.0,
+# (k_fwd2 advances two steps per launch)
+BYTES_PER_CELL = {"k_fwd": 16.0, "k_bwd": 36.0, "k_fwd2": 32.0,
+                  "k_habc_fwd": 16.0, "k_habc_adj": 36.0,
+                  "k_el_fwd": 52.0, "k_el_bwd": 116.0,
                   "k_fwd_batch": 16.0, "k_bwd_batch": 36.0}
-KERNEL_IDS = {"acoustic": (("k_fwd", 0), ("k_bwd", 1), ("k_fwd2", 8)), "elastic": (("k_el_fwd", 2), ("k_el_bwd", 3)),
+KERNEL_IDS = {"acoustic": (("k_fwd", 0), ("k_bwd", 1), ("k_fwd2", 8)),
+              "habc": (("k_habc_fwd", 4), ("k_habc_adj", 5)),
+              "elastic": (("k_el_fwd", 2), ("k_el_bwd", 3)),
               "generator": (("k_fwd_batch", 6), ("k_bwd_batch", 7))}
 
 
@@ -41,9 +45,10 @@ def parse():
                    help="time steps of the bounded CPU sample (1 shot, full grid)")
     p.add_argument("--nt", type=int, default=None,
                    help="shorter time axis for profiler runs (the metric is quoted at full nt)")
-    p.add_argument("--workload", choices=["acoustic", "elastic", "generator", "precond"],
+    p.add_argument("--workload", choices=["acoustic", "habc", "elastic", "generator", "precond"],
                    default="acoustic",
-                   help="acoustic: 64-shot 800x2400 FWI gradient (the headline); elastic: "
+                   help="acoustic: 64-shot 800x2400 FWI gradient (the headline); habc: the "
+                        "same with the HABC boundary; elastic: "
                         "32-shot 600x1800 vp/vs/rho gradient; generator: 256 uneven seeded "
                         "models, records + misfit gradient in one batch; precond: train the "
                         "U-Net preconditioner on generator pairs, then plain vs preconditioned "
@@ -122,6 +127,20 @@ def elastic_gradient_step(P):
     loss = torch.sum((rec - P["obs"]) ** 2)
     loss.backward()
     return loss.detach(), m[0].grad
+
+
+def habc_records(P, vp):
+    from seistorch_amd.equations2d import acoustic_habc as ah
+    return ah.propagate(vp, P["b"], P["w"], P["src"], P["rec"], P["B"], P["dt"], P["h"])
+
+
+def habc_gradient_step(P):
+    import torch
+    vp = P["vp_init"].detach().requires_grad_(True)
+    rec = habc_records(P, vp)
+    loss = torch.sum((rec - P["obs"]) ** 2)
+    loss.backward()
+    return loss.detach(), vp.grad
 
 
 def gradient_step(P):
@@ -357,6 +376,13 @@ def main():
         with torch.no_grad():
             P["obs"] = elastic_records(P, P["vp_true"], P["vs_true"], P["rho_true"])
         step_fn = elastic_gradient_step
+    elif args.workload == "habc":
+        from seistorch_amd.habc import generate_habc_coefficients_2d
+        P = build_problem(dev, nt_override=args.nt)
+        P["b"] = generate_habc_coefficients_2d((P["nz"], P["nx"]), 50).to(dev)
+        with torch.no_grad():
+            P["obs"] = habc_records(P, P["vp_true"])
+        step_fn = habc_gradient_step
     else:
         P = build_problem(dev, nt_override=args.nt)
         with torch.no_grad():
@@ -368,7 +394,7 @@ def main():
     print(f"[bench] problem ready: {P['B']} shots, {P['nz']}x{P['nx']}, nt={P['nt']}",
           file=sys.stderr, flush=True)
     shape, tune = None, {}
-    if not elastic:
+    if args.workload == "acoustic":
         from seistorch_amd import autotune
         shape, tune = autotune.tune_acoustic(P["vp_init"], P["d"], P["w"], P["src"], P["rec"],
                                              P["B"], P["dt"], P["h"])
@@ -410,9 +436,12 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded anomaly model, records from the true model)",
-        "config": {"workload": ("elastic vp/vs/rho gradient, PML 50, fwd + boundary-saving "
-                                "adjoint" if elastic else
-                                "acoustic FWI gradient, PML 50, fwd + boundary-saving adjoint"),
+        "config": {"workload": {"elastic": "elastic vp/vs/rho gradient, PML 50, fwd + "
+                                           "boundary-saving adjoint",
+                                "habc": "acoustic FWI gradient, HABC 50, fwd + boundary-saving "
+                                        "adjoint",
+                                "acoustic": "acoustic FWI gradient, PML 50, fwd + "
+                                            "boundary-saving adjoint"}[args.workload],
                    "grid": [P["nz_i"], P["nx_i"]], "padded_grid": [P["nz"], P["nx"]],
                    "shots": P["B"], "nt": P["nt"], "dt": P["dt"], "dx": P["h"],
                    "receivers_per_shot": P["nx_i"], "f_peak_hz": P["fm"]},
@@ -429,7 +458,7 @@ def main():
     }
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration(dev)
-    if not args.no_cpu_baseline and not elastic:
+    if not args.no_cpu_baseline and args.workload == "acoustic":
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, args.cpu_steps)
     print(json.dumps(out), flush=True)

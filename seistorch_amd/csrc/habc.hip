@@ -23,6 +23,8 @@
 #include "seistorch_amd.h"
 #include "tiles.h"
 
+#include <initializer_list>
+
 using seis::Grid;
 using seis::lap_taps;
 using seis::acoustic_update;
@@ -188,18 +190,13 @@ __global__ __launch_bounds__(256) void k_habc_fwd(const float* __restrict__ vp,
 // adjoint source, which k_habc_post adds with its gradient share) and the
 // interior rebuild of u^{k-2} over uc.  recon: 0 none, 1 rebuild, 2 zeros.
 template <int S>
-__global__ __launch_bounds__(256) void k_habc_adj(
-    const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
-    const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
-    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
-    const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k,
-    TileMap tm) {
-  int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;
+__device__ __forceinline__ void habc_adj_cell(
+    int z, int x, int grp, const float* __restrict__ vp, const float* __restrict__ bw,
+    const float* __restrict__ y1, const float* __restrict__ y2, float* __restrict__ out,
+    const float* __restrict__ um, float* __restrict__ uc, float* __restrict__ gpart,
+    const int32_t* __restrict__ src, const float* __restrict__ wn, int recon, int B,
+    const HabcCtx& hc, const seis::TimeConsts& k) {
   const Grid& g = hc.g;
-  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
-  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
-  if (x >= g.nx || z >= g.nz) return;
   const int i = z * g.nx + x;
   const float A0 = 1.0f / k.idt2;
   const float B0 = k.idt2;
@@ -283,6 +280,142 @@ __global__ __launch_bounds__(256) void k_habc_adj(
     }
   }
   if (in && um && gpart) gpart[(int64_t)grp * g.plane + i] += 2.0f * c * A0 * gacc;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_habc_adj(
+    const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
+    const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k,
+    TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x >= g.nx || z >= g.nz) return;
+  habc_adj_cell<S>(z, x, grp, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc, k);
+}
+
+// The fused adjoint step is split in two launches.  k_habc_adj_deep covers
+// the rectangle Deep (z in [z0, z1), x in [x0, x1), at least kW+3 cells from
+// every edge, x0 and x1 multiples of VX): there no strip term reaches, so a
+// lane runs a vectorised copy of the per-cell deep path of habc_adj_cell with
+// the same operands in the same order (bit-identical).  k_habc_adj_frame
+// runs habc_adj_cell on every other cell, one cell per thread.  Both require
+// y1, y2, um and gpart (the fused loop's call).
+struct Deep {
+  int z0, z1, x0, x1;
+};
+
+template <int S>
+__global__ __launch_bounds__(256) void k_habc_adj_frame(
+    const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
+    const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
+    TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x >= g.nx || z >= g.nz) return;
+  if (z >= dp.z0 && z < dp.z1 && x >= dp.x0 && x < dp.x1) return;  // the deep launch's cell
+  habc_adj_cell<S>(z, x, grp, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc, k);
+}
+
+template <int S, int VX>
+__global__ __launch_bounds__(256) void k_habc_adj_deep(
+    const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
+    const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
+    TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
+  const int z = dp.z0 + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x0 >= dp.x1 || z >= dp.z1) return;
+  // every cell of Deep and its neighbours are interior, away from the strips
+  const int i = z * g.nx + x0;
+  const float A0 = 1.0f / k.idt2;
+  const float B0 = k.idt2;
+  float c[VX], qrow[VX + 2], qup[VX], qdn[VX];
+  vload<VX>(vp + i, c);
+  {
+    float t[VX];
+    vload<VX>(vp + i - g.nx, t);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) qup[e] = (t[e] * t[e]) * A0 * 1.0f;
+    vload<VX>(vp + i + g.nx, t);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) qdn[e] = (t[e] * t[e]) * A0 * 1.0f;
+    const float cl = vp[i - 1], cr = vp[i + VX];
+    qrow[0] = (cl * cl) * A0 * 1.0f;
+    qrow[VX + 1] = (cr * cr) * A0 * 1.0f;
+#pragma unroll
+    for (int e = 0; e < VX; ++e) qrow[e + 1] = (c[e] * c[e]) * A0 * 1.0f;
+  }
+  const float ca = k.two_idt2 * (A0 * 1.0f);
+  const float cb = -(B0 * (A0 * 1.0f));
+  const float w = wn ? *wn : 0.0f;
+  float gacc[VX];
+  vzero<VX>(gacc);
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int bb = b0 + s;
+    const int64_t off = (int64_t)bb * g.plane;
+    const float* Y = y1 + off;
+    float yc[VX], yu[VX], yd[VX], y2c[VX], acc[VX];
+    vload<VX>(Y + i, yc);
+    vload<VX>(Y + i - g.nx, yu);
+    vload<VX>(Y + i + g.nx, yd);
+    vload<VX>(y2 + off + i, y2c);
+    const float yl = Y[i - 1], yr = Y[i + VX];
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      const float left = e == 0 ? qrow[0] * yl : qrow[e] * yc[e - 1];
+      const float right = e == VX - 1 ? qrow[VX + 1] * yr : qrow[e + 2] * yc[e + 1];
+      const float Lq = lap_taps(qup[e] * yu[e], left, qrow[e + 1] * yc[e], right, qdn[e] * yd[e],
+                                k.ih2);
+      float a = ca * yc[e] + Lq;
+      a += cb * y2c[e];
+      acc[e] = a;
+    }
+    vstore<VX>(out + off + i, acc);
+    if (recon >= 0) {
+      const float* U = um + off;
+      float uu[VX], uup[VX], udn[VX], un[VX];
+      vload<VX>(U + i, uu);
+      vload<VX>(U + i - g.nx, uup);
+      vload<VX>(U + i + g.nx, udn);
+      const float ul = U[i - 1], ur = U[i + VX];
+      if (recon == 1) vload<VX>(uc + off + i, un);
+      const int sz = src[2 * bb], sx = src[2 * bb + 1];
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const float Lu = lap_taps(uup[e], e == 0 ? ul : uu[e - 1], uu[e],
+                                  e == VX - 1 ? ur : uu[e + 1], udn[e], k.ih2);
+        gacc[e] += Lu * acc[e];
+        if (recon == 1) {
+          float v = un[e];
+          if (z == sz && x0 + e == sx) v -= w;
+          un[e] = ((k.two_idt2 * uu[e] + (c[e] * c[e]) * Lu) - v / A0) / B0;
+        } else if (recon == 2) {
+          un[e] = 0.0f;
+        }
+      }
+      if (recon != 0) vstore<VX>(uc + off + i, un);
+    }
+  }
+  float gv[VX];
+  vload<VX>(gpart + (int64_t)grp * g.plane + i, gv);
+#pragma unroll
+  for (int e = 0; e < VX; ++e) gv[e] += 2.0f * c[e] * A0 * gacc[e];
+  vstore<VX>(gpart + (int64_t)grp * g.plane + i, gv);
 }
 
 // Per-step model gradient (all cells): sum over shots of
@@ -405,6 +538,35 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
     }                                                             \
     SEIS_CHECK_LAUNCH();                                          \
   } while (0)
+
+#define SEIS_HABC_VX(K, S_, VX_, grid, s, ...)                       \
+  switch (VX_) {                                                      \
+    case 1: K<S_, 1><<<grid, 256, 0, s>>>(__VA_ARGS__); break;       \
+    case 2: K<S_, 2><<<grid, 256, 0, s>>>(__VA_ARGS__); break;       \
+    default: K<S_, 4><<<grid, 256, 0, s>>>(__VA_ARGS__); break;      \
+  }
+
+#define SEIS_HABC_LAUNCH_V(K, S, VX, grid, s, ...)                   \
+  do {                                                              \
+    switch (S) {                                                    \
+      case 1: SEIS_HABC_VX(K, 1, VX, grid, s, __VA_ARGS__) break;   \
+      case 2: SEIS_HABC_VX(K, 2, VX, grid, s, __VA_ARGS__) break;   \
+      case 4: SEIS_HABC_VX(K, 4, VX, grid, s, __VA_ARGS__) break;   \
+      default: SEIS_HABC_VX(K, 8, VX, grid, s, __VA_ARGS__) break;  \
+    }                                                               \
+    SEIS_CHECK_LAUNCH();                                            \
+  } while (0)
+
+// Widest lane vector (4, 2, 1) dividing nx with every pointer aligned to it.
+int habc_vx(int64_t nx, std::initializer_list<const void*> ptrs) {
+  for (int v : {4, 2}) {
+    if (nx % v) continue;
+    bool ok = true;
+    for (const void* p : ptrs) ok = ok && (reinterpret_cast<uintptr_t>(p) % (4 * v)) == 0;
+    if (ok) return v;
+  }
+  return 1;
+}
 
 int habc_check(int64_t B, int64_t nz, int64_t nx, float dt, float h, int S) {
   SEIS_CHECK_ARG(B >= 1 && B <= 65535, "batch size %lld outside [1, 65535]", (long long)B);
@@ -529,7 +691,14 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   const Grid& g = hc.g;
   const int S = shots_per_group;
   const int G = (int)((B + S - 1) / S);
-  const TileMap tm = make_tiles(g, 1, 1, G);
+  const int VX = habc_vx(nx, {vp, u_a, u_b, l_a, l_b, l_c, grad_part});
+  const TileMap tm = make_tiles(g, 1, 1, G);  // frame launch: one cell per thread
+  constexpr int m = kW + 3;
+  Deep dp{m, (int)nz - m, (m + VX - 1) / VX * VX, ((int)nx - m) / VX * VX};
+  if (dp.x1 <= dp.x0 || dp.z1 <= dp.z0) dp = Deep{0, 0, 0, 0};  // no deep cells
+  const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
+  const TileMap tmd = make_tiles(gd, VX, 1, G);
+  const bool has_deep = gd.plane > 0;
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -547,8 +716,11 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   for (int64_t kk = nt; kk >= 1; --kk) {
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
-    SEIS_HABC_LAUNCH(k_habc_adj, S, dim3(tm.per * kXcds), s, vp, b, L1, L2, L0, X, Y, grad_part,
-                     src_zx, wavelet + (kk - 1), recon, (int)B, hc, k, tm);
+    if (has_deep)
+      SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX, dim3(tmd.per * kXcds), s, vp, b, L1, L2, L0, X,
+                         Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k, dp, tmd);
+    SEIS_HABC_LAUNCH(k_habc_adj_frame, S, dim3(tm.per * kXcds), s, vp, b, L1, L2, L0, X, Y,
+                     grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k, dp, tm);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
     if (work > 0) {

@@ -90,9 +90,12 @@ def test_fused_records(S):
     assert e_ref < TOL and e_or <= 2 * e_ro + 1e-6
 
 
-@pytest.mark.parametrize("K", [None, 0])
-def test_fused_adjoint_interior(K):
-    c = _habc_case(nshots=3, nt=200)
+@pytest.mark.parametrize("K,nx_i,nz_i", [(None, 90, 60), (0, 90, 60), (None, 540, 70),
+                                         (0, 263, 66)])
+def test_fused_adjoint_interior(K, nx_i, nz_i):
+    """nx_i 540 (padded nx 640, 4-cell lanes) and 263 (363, 1-cell lanes) give
+    tiles deep inside the strips, which take the vectorised adjoint path."""
+    c = _habc_case(nshots=3, nt=200, nx_i=nx_i, nz_i=nz_i)
     t = to_dev(c)
     rr = ref_torch.habc_forward(c.vp, c.b, c.dt, c.h, c.w, c.src, c.rec)
     syn = torch.cat(rr, 1)[..., 0].numpy()
@@ -108,7 +111,7 @@ def test_fused_adjoint_interior(K):
     I = (slice(50, -50), slice(50, -50))
     e_ref, e_64, e_r64 = (rel_l2(g[I], g32.numpy()[I]), rel_l2(g[I], g64.numpy()[I]),
                           rel_l2(g32.numpy()[I], g64.numpy()[I]))
-    print(f"habc adjoint K={K}: vs ref_torch AD {e_ref:.2e}, vs FP64 AD {e_64:.2e}, "
+    print(f"habc adjoint K={K} grid {c.nz}x{c.nx}: vs ref_torch AD {e_ref:.2e}, vs FP64 AD {e_64:.2e}, "
           f"ref vs FP64 {e_r64:.2e}")
     assert e_ref < TOL and e_64 <= 2 * e_r64 + 1e-6
     mask = np.ones_like(g, bool); mask[I] = False
