@@ -131,21 +131,51 @@ __device__ __forceinline__ HabcW habc_weights(int z, int x, const HabcCtx& c, fl
 
 // Forward step for S shots per thread: out = HABC(u, p) (+ w at the source
 // cells when src != nullptr).  Cells are 64 x 4 tiles in XCD-aware order.
+// Cells z in [z0, z1), x in [x0, x1) handled by a vectorised launch (see
+// k_habc_fwd_deep / k_habc_adj_deep); the per-cell kernels skip them.
+struct Deep {
+  int z0, z1, x0, x1;
+};
+
+__device__ __forceinline__ bool in_deep(int z, int x, const Deep& dp) {
+  return z >= dp.z0 && z < dp.z1 && x >= dp.x0 && x < dp.x1;
+}
+
+// The cells outside Deep as up to four rectangles (top, bottom, left, right),
+// tiled 64 x 4 each (one cell per thread) and numbered consecutively, so the
+// per-cell launches only cover the strips.
+struct Frame {
+  int n;
+  int z0[4], z1[4], x0[4], x1[4], ntx[4], ntz[4];
+  int64_t t0[5];  // first global tile of each rectangle (prefix over the groups too)
+  int groups, per;
+};
+
+// -> false if workgroup w (numbered within the frame's blocks) has no cell
+__device__ __forceinline__ bool frame_cell(const Frame& f, int w, int& z, int& x, int& grp) {
+  const int64_t t = (int64_t)(w % kXcds) * f.per + w / kXcds;
+  if (t >= f.t0[f.n]) return false;
+  int r = 0;
+  while (r + 1 < f.n && t >= f.t0[r + 1]) ++r;
+  const int64_t local = t - f.t0[r];
+  const int tx = (int)(local % f.ntx[r]);
+  const int tzi = (int)((local / f.ntx[r]) % f.ntz[r]);
+  grp = (int)(local / ((int64_t)f.ntx[r] * f.ntz[r]));
+  x = f.x0[r] + tx * kLanes + (int)(threadIdx.x % kLanes);
+  z = f.z0[r] + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  return x < f.x1[r] && z < f.z1[r];
+}
+
 template <int S>
-__global__ __launch_bounds__(256) void k_habc_fwd(const float* __restrict__ vp,
-                                                  const float* __restrict__ bw,
-                                                  const float* __restrict__ u,
-                                                  const float* __restrict__ p,
-                                                  float* __restrict__ out,
-                                                  const int32_t* __restrict__ src,
-                                                  const float* __restrict__ wn, int B,
-                                                  HabcCtx hc, seis::TimeConsts k, TileMap tm) {
-  int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;
+__device__ __forceinline__ void habc_fwd_cell(int z, int x, int grp, const float* __restrict__ vp,
+                                              const float* __restrict__ bw,
+                                              const float* __restrict__ u,
+                                              const float* __restrict__ p,
+                                              float* __restrict__ out,
+                                              const int32_t* __restrict__ src,
+                                              const float* __restrict__ wn, int B,
+                                              const HabcCtx& hc, const seis::TimeConsts& k) {
   const Grid& g = hc.g;
-  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
-  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
-  if (x >= g.nx || z >= g.nz) return;
   const int i = z * g.nx + x;
   const float c = vp[i];
   const float A0 = 1.0f / k.idt2;
@@ -181,6 +211,86 @@ __global__ __launch_bounds__(256) void k_habc_fwd(const float* __restrict__ vp,
     }
     if (src && z == src[2 * bb] && x == src[2 * bb + 1]) v += w;
     out[off + i] = v;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_habc_fwd(const float* __restrict__ vp,
+                                                  const float* __restrict__ bw,
+                                                  const float* __restrict__ u,
+                                                  const float* __restrict__ p,
+                                                  float* __restrict__ out,
+                                                  const int32_t* __restrict__ src,
+                                                  const float* __restrict__ wn, int B,
+                                                  HabcCtx hc, seis::TimeConsts k, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x >= g.nx || z >= g.nz) return;
+  habc_fwd_cell<S>(z, x, grp, vp, bw, u, p, out, src, wn, B, hc, k);
+}
+
+// The fused loop's HABC step in one launch of two block ranges: on Deep (at
+// least kW cells from every edge, outside every strip) the undamped update
+// only, vectorised over VX-cell lanes with the operands and order of
+// k_habc_fwd's per-cell path (bit-identical); on the frame around Deep the
+// per-cell step, one cell per thread.
+template <int S, int VX>
+__global__ __launch_bounds__(256) void k_habc_fwd_deep(const float* __restrict__ vp,
+                                                       const float* __restrict__ u,
+                                                       const float* __restrict__ p,
+                                                       float* __restrict__ out,
+                                                       const int32_t* __restrict__ src,
+                                                       const float* __restrict__ wn, int B,
+                                                       HabcCtx hc, seis::TimeConsts k, Deep dp,
+                                                       TileMap tm, const float* __restrict__ bw,
+                                                       Frame f) {
+  // blocks [0, tm.per*8) run Deep's lane vectors, the rest the frame's cells,
+  // so both kinds of work share the machine within one launch
+  const int nd = tm.per * kXcds;
+  if ((int)blockIdx.x >= nd) {
+    int zc, xc, gc;
+    if (frame_cell(f, (int)blockIdx.x - nd, zc, xc, gc))
+      habc_fwd_cell<S>(zc, xc, gc, vp, bw, u, p, out, src, wn, B, hc, k);
+    return;
+  }
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
+  const int z = dp.z0 + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x0 >= dp.x1 || z >= dp.z1) return;
+  const int i = z * g.nx + x0;
+  const float A0 = 1.0f / k.idt2;
+  const float B0 = k.idt2;
+  float c2[VX];
+  vload<VX>(vp + i, c2);
+#pragma unroll
+  for (int e = 0; e < VX; ++e) c2[e] = c2[e] * c2[e];
+  const float w = src ? *wn : 0.0f;
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int bb = b0 + s;
+    const int64_t off = (int64_t)bb * g.plane;
+    const float* U = u + off;
+    float uu[VX], uup[VX], udn[VX], pp[VX], o[VX];
+    vload<VX>(U + i, uu);
+    vload<VX>(U + i - g.nx, uup);
+    vload<VX>(U + i + g.nx, udn);
+    vload<VX>(p + off + i, pp);
+    const float ul = U[i - 1], ur = U[i + VX];
+    const int sz = src ? src[2 * bb] : -1, sx = src ? src[2 * bb + 1] : -1;
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      const float L = lap_taps(uup[e], e == 0 ? ul : uu[e - 1], uu[e],
+                               e == VX - 1 ? ur : uu[e + 1], udn[e], k.ih2);
+      float v = acoustic_update(A0, B0, k.two_idt2, uu[e], pp[e], c2[e], L);
+      if (z == sz && x0 + e == sx) v += w;
+      o[e] = v;
+    }
+    vstore<VX>(out + off + i, o);
   }
 }
 
@@ -298,41 +408,29 @@ __global__ __launch_bounds__(256) void k_habc_adj(
   habc_adj_cell<S>(z, x, grp, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc, k);
 }
 
-// The fused adjoint step is split in two launches.  k_habc_adj_deep covers
-// the rectangle Deep (z in [z0, z1), x in [x0, x1), at least kW+3 cells from
-// every edge, x0 and x1 multiples of VX): there no strip term reaches, so a
-// lane runs a vectorised copy of the per-cell deep path of habc_adj_cell with
-// the same operands in the same order (bit-identical).  k_habc_adj_frame
-// runs habc_adj_cell on every other cell, one cell per thread.  Both require
-// y1, y2, um and gpart (the fused loop's call).
-struct Deep {
-  int z0, z1, x0, x1;
-};
-
-template <int S>
-__global__ __launch_bounds__(256) void k_habc_adj_frame(
-    const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
-    const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
-    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
-    const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
-    TileMap tm) {
-  int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;
-  const Grid& g = hc.g;
-  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
-  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
-  if (x >= g.nx || z >= g.nz) return;
-  if (z >= dp.z0 && z < dp.z1 && x >= dp.x0 && x < dp.x1) return;  // the deep launch's cell
-  habc_adj_cell<S>(z, x, grp, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc, k);
-}
-
+// The fused adjoint step in one launch of two block ranges.  The first
+// covers the rectangle Deep (z in [z0, z1), x in [x0, x1), at least kW+3
+// cells from every edge, x0 and x1 multiples of VX): there no strip term
+// reaches, so a lane runs a vectorised copy of the per-cell deep path of
+// habc_adj_cell with the same operands in the same order (bit-identical).
+// The second runs habc_adj_cell on every other cell, one cell per thread.
+// Requires y1, y2, um and gpart (the fused loop's call).
 template <int S, int VX>
 __global__ __launch_bounds__(256) void k_habc_adj_deep(
     const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
     const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
     float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
     const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
-    TileMap tm) {
+    TileMap tm, Frame f) {
+  // blocks [0, tm.per*8) run Deep's lane vectors, the rest the frame's cells
+  const int nd = tm.per * kXcds;
+  if ((int)blockIdx.x >= nd) {
+    int zc, xc, gc;
+    if (frame_cell(f, (int)blockIdx.x - nd, zc, xc, gc))
+      habc_adj_cell<S>(zc, xc, gc, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc,
+                       k);
+    return;
+  }
   int tx, tzi, grp;
   if (!tile_of(tm, tx, tzi, grp)) return;
   const Grid& g = hc.g;
@@ -568,6 +666,33 @@ int habc_vx(int64_t nx, std::initializer_list<const void*> ptrs) {
   return 1;
 }
 
+// The frame around dp (the whole grid if dp is empty) as rectangles of tiles.
+Frame make_frame(const Grid& g, const Deep& dp, int groups) {
+  Frame f{};
+  auto add = [&](int z0, int z1, int x0, int x1) {
+    if (z1 <= z0 || x1 <= x0) return;
+    const int r = f.n++;
+    f.z0[r] = z0; f.z1[r] = z1; f.x0[r] = x0; f.x1[r] = x1;
+    f.ntx[r] = (x1 - x0 + kLanes - 1) / kLanes;
+    f.ntz[r] = (z1 - z0 + kRowsOfThreads - 1) / kRowsOfThreads;
+  };
+  if (dp.z1 <= dp.z0 || dp.x1 <= dp.x0) {
+    add(0, g.nz, 0, g.nx);
+  } else {
+    add(0, dp.z0, 0, g.nx);
+    add(dp.z1, g.nz, 0, g.nx);
+    add(dp.z0, dp.z1, 0, dp.x0);
+    add(dp.z0, dp.z1, dp.x1, g.nx);
+  }
+  f.groups = groups;
+  f.t0[0] = 0;
+  for (int r = 0; r < f.n; ++r)
+    f.t0[r + 1] = f.t0[r] + (int64_t)f.ntx[r] * f.ntz[r] * groups;
+  for (int r = f.n + 1; r < 5; ++r) f.t0[r] = f.t0[f.n];
+  f.per = (int)((f.t0[f.n] + kXcds - 1) / kXcds);
+  return f;
+}
+
 int habc_check(int64_t B, int64_t nz, int64_t nx, float dt, float h, int S) {
   SEIS_CHECK_ARG(B >= 1 && B <= 65535, "batch size %lld outside [1, 65535]", (long long)B);
   SEIS_CHECK_ARG(nz >= 2 * kW + 2 && nx >= 2 * kW + 2,
@@ -633,7 +758,13 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
   const HabcCtx hc{Grid{(int)nz, (int)nx, npml, nz * nx}, multiple ? 1 : 0, dt, h};
   const Grid& g = hc.g;
   const int S = shots_per_group;
-  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  // the deep interior (kW cells from every edge, outside the strips) runs vectorised
+  const int VX = habc_vx(nx, {vp, u_a, u_b, u_c});
+  Deep dp{kW, (int)nz - kW, (kW + VX - 1) / VX * VX, ((int)nx - kW) / VX * VX};
+  if (dp.x1 <= dp.x0 || dp.z1 <= dp.z0) dp = Deep{0, 0, 0, 0};
+  const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
+  const TileMap tmd = make_tiles(gd, VX, 1, (int)((B + S - 1) / S));
+  const Frame fr = make_frame(g, dp, (int)((B + S - 1) / S));
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -651,8 +782,8 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
     float* up = P[(n + 2) % 3];
     float* out = P[(n + 1) % 3];
     SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
-    SEIS_HABC_LAUNCH(k_habc_fwd, S, dim3(tm.per * kXcds), s, vp, b, un, up, out, src_zx,
-                     wavelet + n, (int)B, hc, k, tm);
+    SEIS_HABC_LAUNCH_V(k_habc_fwd_deep, S, VX, dim3((tmd.per + fr.per) * kXcds), s, vp, un,
+                       up, out, src_zx, wavelet + n, (int)B, hc, k, dp, tmd, b, fr);
     SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
@@ -692,13 +823,12 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   const int S = shots_per_group;
   const int G = (int)((B + S - 1) / S);
   const int VX = habc_vx(nx, {vp, u_a, u_b, l_a, l_b, l_c, grad_part});
-  const TileMap tm = make_tiles(g, 1, 1, G);  // frame launch: one cell per thread
   constexpr int m = kW + 3;
   Deep dp{m, (int)nz - m, (m + VX - 1) / VX * VX, ((int)nx - m) / VX * VX};
   if (dp.x1 <= dp.x0 || dp.z1 <= dp.z0) dp = Deep{0, 0, 0, 0};  // no deep cells
   const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
   const TileMap tmd = make_tiles(gd, VX, 1, G);
-  const bool has_deep = gd.plane > 0;
+  const Frame fr = make_frame(g, dp, G);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -716,11 +846,9 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   for (int64_t kk = nt; kk >= 1; --kk) {
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
-    if (has_deep)
-      SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX, dim3(tmd.per * kXcds), s, vp, b, L1, L2, L0, X,
-                         Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k, dp, tmd);
-    SEIS_HABC_LAUNCH(k_habc_adj_frame, S, dim3(tm.per * kXcds), s, vp, b, L1, L2, L0, X, Y,
-                     grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k, dp, tm);
+    SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX, dim3((tmd.per + fr.per) * kXcds), s, vp, b, L1,
+                       L2, L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k,
+                       dp, tmd, fr);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
     if (work > 0) {

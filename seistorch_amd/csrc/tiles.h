@@ -103,6 +103,17 @@ inline TileMap make_tiles(const Grid& g, int vx, int rz, int groups) {
   return m;
 }
 
+// -> false if workgroup w has no tile (padding of the XCD split)
+__device__ __forceinline__ bool tile_of_w(const TileMap m, int w, int& tx, int& tzi, int& grp) {
+  const int64_t t = (int64_t)(w % kXcds) * m.per + w / kXcds;
+  if (t >= (int64_t)m.ntx * m.ntz * m.groups) return false;
+  tx = (int)(t % m.ntx);
+  const int64_t r = t / m.ntx;
+  tzi = (int)(r % m.ntz);
+  grp = (int)(r / m.ntz);
+  return true;
+}
+
 // -> false if this workgroup has no tile (padding of the XCD split)
 __device__ __forceinline__ bool tile_of(const TileMap m, int& tx, int& tzi, int& grp) {
   const int w = blockIdx.x;

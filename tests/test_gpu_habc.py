@@ -78,16 +78,22 @@ def _fused(c, t, vp=None, K=None, S=None):
                         snapshot_every=K)
 
 
-@pytest.mark.parametrize("S", [None, 1])
-def test_fused_records(S):
-    c = _habc_case(nshots=4)
+@pytest.mark.parametrize("S,nx_i", [(None, 90), (1, 90), (None, 540), (2, 263)])
+def test_fused_records(S, nx_i):
+    """Records of the fused loop; nx_i 540 and 263 have deep interiors run by
+    the vectorised launch (4- and 1-cell lanes): still bit-identical."""
+    c = _habc_case(nshots=4, nx_i=nx_i, nt=300 if nx_i == 90 else 200)
     t = to_dev(c)
     rec = _fused(c, t, S=S).cpu().numpy()
     rr = torch.cat(ref_torch.habc_forward(c.vp, c.b, c.dt, c.h, c.w, c.src, c.rec), 1)[..., 0].numpy()
-    ro = np.concatenate(oh.forward(c.vp, c.b, c.dt, c.h, c.w, c.src, c.rec), 1)[..., 0]
-    e_ref, e_or, e_ro = rel_l2(rec, rr), rel_l2(rec, ro), rel_l2(rr, ro)
-    print(f"habc records S={S}: vs ref_torch {e_ref:.2e}, vs FP64 {e_or:.2e}, ref vs FP64 {e_ro:.2e}")
-    assert e_ref < TOL and e_or <= 2 * e_ro + 1e-6
+    e_ref = rel_l2(rec, rr)
+    if nx_i == 90:
+        ro = np.concatenate(oh.forward(c.vp, c.b, c.dt, c.h, c.w, c.src, c.rec), 1)[..., 0]
+        e_or, e_ro = rel_l2(rec, ro), rel_l2(rr, ro)
+        print(f"habc records S={S}: vs ref_torch {e_ref:.2e}, vs FP64 {e_or:.2e}, ref vs FP64 {e_ro:.2e}")
+        assert e_or <= 2 * e_ro + 1e-6
+    print(f"habc records S={S} grid {c.nz}x{c.nx}: vs ref_torch {e_ref:.2e}")
+    assert e_ref == 0.0
 
 
 @pytest.mark.parametrize("K,nx_i,nz_i", [(None, 90, 60), (0, 90, 60), (None, 540, 70),
