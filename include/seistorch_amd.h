@@ -259,7 +259,10 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
 /* Adjoint of seis_elastic_forward with boundary-saving reconstruction
  * (replaces seistorch/checkpoint.py:108-230 and elastic.py:39-122 with the
  * exact time pairing).  fields: the final state from the forward (consumed);
- * adj_a, adj_b: two sets of five (B, nz, nx) work buffers; grad_coef:
+ * adj_a, adj_b: two sets of five (B, nz, nx) work buffers; scratch: NULL,
+ * or five more (B, nz, nx) buffers, with which each reverse step runs as one
+ * fused pass (both transposed stages and the rebuild of the forward state,
+ * ping-ponging between fields and scratch) with the same results; grad_coef:
  * ceil(B/S) x 4 planes, receives the coefficient gradients on the interior
  * (pass them to seis_elastic_chain). */
 int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
@@ -267,9 +270,9 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
                           int64_t R, const int32_t* rec_field, int32_t nch,
                           const float* rec_grad, const float* ring, const float* snaps,
                           int32_t snap_every, float* const* fields, float* const* adj_a,
-                          float* const* adj_b, float* grad_coef, int64_t B, int64_t nz,
-                          int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
-                          void* stream);
+                          float* const* adj_b, float* const* scratch, float* grad_coef, int64_t B,
+                          int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                          int32_t shots_per_group, void* stream);
 
 /* ---------------- learned gradient preconditioner (U-Net) ----------------- */
 /* No reference counterpart: a small U-Net maps (gradient, model) to an

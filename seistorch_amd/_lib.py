@@ -61,8 +61,8 @@ SIGNATURES = {
                                             _p, _i32, _p, _p, _i64, _i64, _i64, _f, _f, _i32,
                                             _i32, _p]),
     "seis_elastic_backward": (ctypes.c_int, [_p, _p, _i64, _p, _i32, _p, _i64, _p, _i32, _p, _p,
-                                             _p, _i32, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
-                                             _i32, _i32, _p]),
+                                             _p, _i32, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f,
+                                             _f, _i32, _i32, _p]),
     "seis_unet_conv3x3": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _p, _p, _i32, _i32, _p, _p,
                                          _p, _p]),
     "seis_unet_flip_weights": (ctypes.c_int, [_p, _i32, _i32, _p, _p]),

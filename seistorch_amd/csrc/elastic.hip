@@ -317,6 +317,336 @@ __global__ __launch_bounds__(256) void k_el_step_rz(const float* __restrict__ cf
   el_step_body<S, kElRZ, VX>(cf, dti, Fi, Fo, B, g, tm);
 }
 
+// One whole reverse step of the fused adjoint loop in one pass: the transpose
+// of stage V, then of stage T (k_el_adjV then k_el_adjT over the whole grid,
+// sign +1, gradients on the interior), with the reverse-time rebuild of the
+// forward state.  Reads the cotangent of state n+1 (Xi) and state n+1 (Fi,
+// whose velocities on the ring already hold state n); writes the cotangent
+// of state n (Xo) and state n (Fo: rebuilt on the interior, copied
+// elsewhere; velocities zero when recon == 0, state 0) and adds the
+// coefficient gradients of its cells to the group's planes of gco.  Wave
+// layout as el_step_body: a lane recomputes what it needs on the rows above
+// and below, x-neighbours come from the adjacent lanes by shuffles, and only
+// the 62 inner lanes write (what they take from lanes 0 and 63 depends on
+// no cell outside the tile).  Every value uses the operands and order of the
+// two stage kernels, so the results are bit-identical to them.
+template <int S, int RZ, int VX>
+__device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float dti, CFields Xi,
+                                            Fields Xo, CFields Fi, Fields Fo,
+                                            float* __restrict__ gco, int recon, int B, Grid g,
+                                            TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
+  const int lane = (int)(threadIdx.x % kLanes);
+  const int x0 = (tx * kElOut + lane - 1) * VX;
+  const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
+  if (zb >= g.nz) return;  // uniform over the wave
+  const bool xin = x0 >= 0 && x0 < g.nx;  // nx is a multiple of VX
+  const bool inner = lane > 0 && lane < kLanes - 1;
+  const int64_t P = g.plane;
+  const float sgn = 1.0f;
+  const float sdti = dti;
+  // row j of the local arrays is z = zb - 1 + j
+  auto rok = [&](int j) { return xin && zb - 1 + j >= 0 && zb - 1 + j < g.nz; };
+  auto idx = [&](int j) { return (zb - 1 + j) * g.nx + x0; };
+  float G0[RZ][VX], G1[RZ][VX], G2[RZ][VX], G3[RZ][VX];
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    vzero<VX>(G0[r]);
+    vzero<VX>(G1[r]);
+    vzero<VX>(G2[r]);
+    vzero<VX>(G3[r]);
+  }
+  const int b0 = grp * S, nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int64_t off = (int64_t)(b0 + s) * P;
+    // velocity cotangents of state n+1 and their products with the stage-V
+    // weight c = s*a*Rc, rows zb-1 .. zb+RZ
+    float XV[RZ + 2][VX], ZV[RZ + 2][VX], PX[RZ + 2][VX], PZ[RZ + 2][VX];
+#pragma unroll
+    for (int j = 0; j < RZ + 2; ++j) {
+      if (rok(j)) {
+        const int i = idx(j);
+        float a[VX], rc[VX];
+        vload<VX>(Xi.f[0] + off + i, XV[j]);
+        vload<VX>(Xi.f[1] + off + i, ZV[j]);
+        vload<VX>(cf + kAinv * P + i, a);
+        vload<VX>(cf + kRc * P + i, rc);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+          const float c = sgn * a[e] * rc[e];
+          PX[j][e] = c * XV[j][e];
+          PZ[j][e] = c * ZV[j][e];
+        }
+      } else {
+        vzero<VX>(XV[j]);
+        vzero<VX>(ZV[j]);
+        vzero<VX>(PX[j]);
+        vzero<VX>(PZ[j]);
+      }
+    }
+    // stress cotangents after the stage-V transpose, times a:
+    // U1 = a*txx, U2 = a*tzz on rows zb-1 .. zb+RZ-1 (index j),
+    // U3 = a*txz on rows zb .. zb+RZ (index j-1)
+    float U1[RZ + 1][VX], U2[RZ + 1][VX], U3[RZ + 1][VX];
+#pragma unroll
+    for (int j = 0; j <= RZ; ++j) {
+      const float pxl = __shfl_up(PX[j][VX - 1], 1, kLanes);
+      if (rok(j)) {
+        const int i = idx(j);
+        const int z = zb - 1 + j;
+        const bool zl = z >= 1, zr = z <= g.nz - 2;
+        float t2[VX], t3[VX], a[VX];
+        vload<VX>(Xi.f[2] + off + i, t2);
+        vload<VX>(Xi.f[3] + off + i, t3);
+        vload<VX>(cf + kAinv * P + i, a);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+          const int x = x0 + e;
+          const bool xl = x >= 1, xr = x <= g.nx - 2;
+          const float txm = xl ? (e == 0 ? pxl : PX[j][e - 1]) : 0.0f;
+          const float tzzp = zr ? PZ[j + 1][e] : 0.0f;
+          t2[e] = t2[e] + ((xl ? txm : 0.0f) - (xr ? PX[j][e] : 0.0f));
+          t3[e] = t3[e] + ((zl ? PZ[j][e] : 0.0f) - (zr ? tzzp : 0.0f));
+          U1[j][e] = a[e] * t2[e];
+          U2[j][e] = a[e] * t3[e];
+        }
+      } else {
+        vzero<VX>(U1[j]);
+        vzero<VX>(U2[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 1; j <= RZ + 1; ++j) {
+      const float pzr = __shfl_down(PZ[j][0], 1, kLanes);
+      if (rok(j)) {
+        const int i = idx(j);
+        const int z = zb - 1 + j;
+        const bool zl = z >= 1, zr = z <= g.nz - 2;
+        float t4[VX], a[VX];
+        vload<VX>(Xi.f[4] + off + i, t4);
+        vload<VX>(cf + kAinv * P + i, a);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+          const int x = x0 + e;
+          const bool xl = x >= 1, xr = x <= g.nx - 2;
+          const float txzm = zl ? PX[j - 1][e] : 0.0f;
+          const float tzxp = xr ? (e == VX - 1 ? pzr : PZ[j][e + 1]) : 0.0f;
+          t4[e] = t4[e] + ((zl ? txzm : 0.0f) - (zr ? PX[j][e] : 0.0f)) +
+                  ((xl ? PZ[j][e] : 0.0f) - (xr ? tzxp : 0.0f));
+          U3[j - 1][e] = a[e] * t4[e];
+        }
+      } else {
+        vzero<VX>(U3[j - 1]);
+      }
+    }
+    // stage-T transpose: q1 = sdti*(l2m*U1 + lam*U2) on the own rows,
+    // q2 = sdti*(lam*U1 + l2m*U2) on rows zb-1 .. zb+RZ-1, q3 = s*Mc*U3 on
+    // rows zb .. zb+RZ; then the velocity cotangents of state n
+    float Q2[RZ + 1][VX], Q3[RZ + 1][VX];
+#pragma unroll
+    for (int j = 0; j <= RZ; ++j) {
+      if (rok(j)) {
+        const int i = idx(j);
+        float l2m[VX], lam[VX];
+        vload<VX>(cf + kL2M * P + i, l2m);
+        vload<VX>(cf + kLam * P + i, lam);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) Q2[j][e] = sdti * (lam[e] * U1[j][e] + l2m[e] * U2[j][e]);
+      } else {
+        vzero<VX>(Q2[j]);
+      }
+      if (rok(j + 1)) {
+        float mc[VX];
+        vload<VX>(cf + kMc * P + idx(j + 1), mc);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) Q3[j][e] = sgn * mc[e] * U3[j][e];
+      } else {
+        vzero<VX>(Q3[j]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int j = r + 1;
+      float Q1[VX];
+      if (rok(j)) {
+        const int i = idx(j);
+        float l2m[VX], lam[VX];
+        vload<VX>(cf + kL2M * P + i, l2m);
+        vload<VX>(cf + kLam * P + i, lam);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) Q1[e] = sdti * (l2m[e] * U1[j][e] + lam[e] * U2[j][e]);
+      } else {
+        vzero<VX>(Q1);
+      }
+      const float q1r = __shfl_down(Q1[0], 1, kLanes);
+      const float q3l = __shfl_up(Q3[r][VX - 1], 1, kLanes);
+      if (!inner || !rok(j)) continue;
+      const int i = idx(j);
+      const int z = zb + r;
+      const bool zl = z >= 1, zr = z <= g.nz - 2;
+      float a[VX], bm[VX], o0[VX], o1[VX], o2[VX], o3[VX], o4[VX];
+      vload<VX>(cf + kAinv * P + i, a);
+      vload<VX>(cf + kBm * P + i, bm);
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const int x = x0 + e;
+        const bool xl = x >= 1, xr = x <= g.nx - 2;
+        const float own = a[e] * bm[e];
+        const float q1xp = xr ? (e == VX - 1 ? q1r : Q1[e + 1]) : 0.0f;
+        const float q3zp = zr ? Q3[r + 1][e] : 0.0f;
+        const float q2zm = zl ? Q2[j - 1][e] : 0.0f;
+        const float q3xm = xl ? (e == 0 ? q3l : Q3[r][e - 1]) : 0.0f;
+        o0[e] = own * XV[j][e];
+        o1[e] = own * ZV[j][e];
+        o0[e] = o0[e] + ((xl ? Q1[e] : 0.0f) - q1xp) + ((zl ? Q3[r][e] : 0.0f) - q3zp);
+        o1[e] = o1[e] + (q2zm - (zr ? Q2[j][e] : 0.0f)) + (q3xm - (xr ? Q3[r][e] : 0.0f));
+        o2[e] = bm[e] * U1[j][e];
+        o3[e] = bm[e] * U2[j][e];
+        o4[e] = bm[e] * U3[r][e];
+      }
+      vstore<VX>(Xo.f[0] + off + i, o0);
+      vstore<VX>(Xo.f[1] + off + i, o1);
+      vstore<VX>(Xo.f[2] + off + i, o2);
+      vstore<VX>(Xo.f[3] + off + i, o3);
+      vstore<VX>(Xo.f[4] + off + i, o4);
+    }
+    // state n+1 stresses on rows zb-1 .. zb+RZ; the stage-V sums
+    // sx = Dx+ txx + Dz+ txz (rows zb-1 .. zb+RZ-1) and sz = Dx- txz + Dz- tzz
+    // (rows zb .. zb+RZ); velocities of state n: vx on rows zb-1 .. zb+RZ-1
+    // (index j), vz on rows zb .. zb+RZ (index j-1)
+    float SXX[RZ + 2][VX], SZZ[RZ + 2][VX], SXZ[RZ + 2][VX];
+#pragma unroll
+    for (int j = 0; j < RZ + 2; ++j) {
+      if (rok(j)) {
+        const int i = idx(j);
+        vload<VX>(Fi.f[2] + off + i, SXX[j]);
+        vload<VX>(Fi.f[3] + off + i, SZZ[j]);
+        vload<VX>(Fi.f[4] + off + i, SXZ[j]);
+      } else {
+        vzero<VX>(SXX[j]);
+        vzero<VX>(SZZ[j]);
+        vzero<VX>(SXZ[j]);
+      }
+    }
+    float SX[RZ + 1][VX], SZ[RZ + 1][VX], VXN[RZ + 1][VX], VZN[RZ + 1][VX];
+#pragma unroll
+    for (int j = 0; j <= RZ; ++j) {
+      const float txxr = __shfl_down(SXX[j][0], 1, kLanes);
+      const int z = zb - 1 + j;
+      const bool zr = z < g.nz - 1;
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const int x = x0 + e;
+        SX[j][e] = (x < g.nx - 1 ? (e == VX - 1 ? txxr : SXX[j][e + 1]) - SXX[j][e] : 0.0f) +
+                   (zr ? SXZ[j + 1][e] - SXZ[j][e] : 0.0f);
+      }
+      if (rok(j) && recon) {
+        const int i = idx(j);
+        float rc[VX];
+        vload<VX>(Fi.f[0] + off + i, VXN[j]);
+        vload<VX>(cf + kRc * P + i, rc);
+#pragma unroll
+        for (int e = 0; e < VX; ++e)
+          if (seis::interior(z, x0 + e, g)) VXN[j][e] = VXN[j][e] - rc[e] * SX[j][e];
+      } else {
+        vzero<VX>(VXN[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 1; j <= RZ + 1; ++j) {
+      const float txzl = __shfl_up(SXZ[j][VX - 1], 1, kLanes);
+      const int z = zb - 1 + j;
+      const bool zl = z >= 1;
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const int x = x0 + e;
+        SZ[j - 1][e] = (x > 0 ? SXZ[j][e] - (e == 0 ? txzl : SXZ[j][e - 1]) : 0.0f) +
+                       (zl ? SZZ[j][e] - SZZ[j - 1][e] : 0.0f);
+      }
+      if (rok(j) && recon) {
+        const int i = idx(j);
+        float rc[VX];
+        vload<VX>(Fi.f[1] + off + i, VZN[j - 1]);
+        vload<VX>(cf + kRc * P + i, rc);
+#pragma unroll
+        for (int e = 0; e < VX; ++e)
+          if (seis::interior(z, x0 + e, g)) VZN[j - 1][e] = VZN[j - 1][e] - rc[e] * SZ[j - 1][e];
+      } else {
+        vzero<VX>(VZN[j - 1]);
+      }
+    }
+    // own rows: coefficient gradients, the stress rebuild, state n out
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int j = r + 1;
+      const float vxl = __shfl_up(VXN[j][VX - 1], 1, kLanes);
+      const float vzr = __shfl_down(VZN[r][0], 1, kLanes);
+      if (!inner || !rok(j)) continue;
+      const int i = idx(j);
+      const int z = zb + r;
+      const bool zl = z >= 1, zr = z <= g.nz - 2;
+      float l2m[VX], lam[VX], mc[VX], a[VX];
+      vload<VX>(cf + kL2M * P + i, l2m);
+      vload<VX>(cf + kLam * P + i, lam);
+      vload<VX>(cf + kMc * P + i, mc);
+      vload<VX>(cf + kAinv * P + i, a);
+      float txx[VX], tzz[VX], txz[VX];
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const int x = x0 + e;
+        txx[e] = SXX[j][e];
+        tzz[e] = SZZ[j][e];
+        txz[e] = SXZ[j][e];
+        if (!seis::interior(z, x, g)) continue;
+        G3[r][e] += a[e] * sgn * (SX[j][e] * XV[j][e] + SZ[r][e] * ZV[j][e]);
+        const float vx_x = x > 0 ? VXN[j][e] - (e == 0 ? vxl : VXN[j][e - 1]) : 0.0f;
+        const float vz_z = zr ? VZN[r + 1][e] - VZN[r][e] : 0.0f;
+        const float vx_z = zl ? VXN[j][e] - VXN[j - 1][e] : 0.0f;
+        const float vz_x = x < g.nx - 1 ? (e == VX - 1 ? vzr : VZN[r][e + 1]) - VZN[r][e] : 0.0f;
+        G0[r][e] += sdti * (vx_x * U1[j][e] + vz_z * U2[j][e]);
+        G1[r][e] += sdti * (vz_z * U1[j][e] + vx_x * U2[j][e]);
+        G2[r][e] += sgn * (vz_x + vx_z) * U3[r][e];
+        if (recon) {
+          txx[e] = txx[e] - sdti * (l2m[e] * vx_x + lam[e] * vz_z);
+          tzz[e] = tzz[e] - sdti * (l2m[e] * vz_z + lam[e] * vx_x);
+          txz[e] = txz[e] - sgn * mc[e] * (vz_x + vx_z);
+        }
+      }
+      vstore<VX>(Fo.f[0] + off + i, VXN[j]);
+      vstore<VX>(Fo.f[1] + off + i, VZN[r]);
+      vstore<VX>(Fo.f[2] + off + i, txx);
+      vstore<VX>(Fo.f[3] + off + i, tzz);
+      vstore<VX>(Fo.f[4] + off + i, txz);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int j = r + 1;
+    if (!inner || !rok(j)) continue;
+    const int z = zb + r;
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      if (!seis::interior(z, x0 + e, g)) continue;
+      float* Gp = gco + (int64_t)grp * 4 * P + idx(j) + e;
+      Gp[0] += G0[r][e];
+      Gp[P] += G1[r][e];
+      Gp[2 * P] += G2[r][e];
+      Gp[3 * P] += G3[r][e];
+    }
+  }
+}
+
+constexpr int kElAdjRZ = 1;  // rows per thread of the fused reverse step
+
+template <int S, int VX>
+__global__ __launch_bounds__(256) void k_el_adj_rz(const float* __restrict__ cf, float dti,
+                                                   CFields Xi, Fields Xo, CFields Fi, Fields Fo,
+                                                   float* __restrict__ gco, int recon, int B,
+                                                   Grid g, TileMap tm) {
+  el_adj_body<S, kElAdjRZ, VX>(cf, dti, Xi, Xo, Fi, Fo, gco, recon, B, g, tm);
+}
+
 // Transpose of stage V.  in: cotangents of the stage outputs; out: of its
 // inputs.  fw: the stage's input fields (stresses used for the Rc gradient);
 // gmode 0 none, 1 every region cell, 2 interior cells only; with recon the
@@ -915,14 +1245,16 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
                           int64_t R, const int32_t* rec_field, int32_t nch,
                           const float* rec_grad, const float* ring, const float* snaps,
                           int32_t snap_every, float* const* fields, float* const* adj_a,
-                          float* const* adj_b, float* grad_coef, int64_t B, int64_t nz,
-                          int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
-                          void* stream) {
+                          float* const* adj_b, float* const* scratch, float* grad_coef, int64_t B,
+                          int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                          int32_t shots_per_group, void* stream) {
   if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
   SEIS_CHECK_ARG(coef && wavelet && src_zx && fields && adj_a && adj_b && grad_coef,
                  "null pointer argument");
   for (int f = 0; f < 5; ++f)
     SEIS_CHECK_ARG(fields[f] && adj_a[f] && adj_b[f], "null field pointer");
+  if (scratch)
+    for (int f = 0; f < 5; ++f) SEIS_CHECK_ARG(scratch[f], "null scratch field pointer");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && nch >= 1 && (R == 0 || (rec_flat && rec_field && rec_grad)),
                  "bad receiver arguments");
@@ -933,7 +1265,8 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
   const Grid g{(int)nz, (int)nx, npml, nz * nx};
   const int S = shots_per_group;
   const int G = (int)((B + S - 1) / S);
-  const int VX = stage_vx(nx, coef, {fields, adj_a, adj_b}, grad_coef);
+  const int VX = scratch ? stage_vx(nx, coef, {fields, adj_a, adj_b, scratch}, grad_coef)
+                        : stage_vx(nx, coef, {fields, adj_a, adj_b}, grad_coef);
   const TileMap tm = make_tiles(g, VX, 1, G);
   const Region all{0, (int)nz, 0, (int)nx};
   hipStream_t s = (hipStream_t)stream;
@@ -957,6 +1290,51 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
     SEIS_CHECK_LAUNCH();
   }
   const int64_t rwork = 5 * B * ring_len;
+  if (scratch) {
+    // fused reverse steps: X/Y and fields/scratch ping-pong
+    Fields Fn = fields_of(scratch);
+    TileMap tm2 = make_tiles(g, VX, kElAdjRZ, G);
+    tm2.ntx = (g.nx + kElOut * VX - 1) / (kElOut * VX);
+    tm2.per = (int)(((int64_t)tm2.ntx * tm2.ntz * tm2.groups + kXcds - 1) / kXcds);
+    for (int64_t n = nt - 1; n >= 0; --n) {
+      const int recon = n >= 1 ? 1 : 0;
+      // velocities of state n on the ring (state n+1's are not read there)
+      if (recon) {
+        k_el_ring_restore<<<blocks_for(rwork), 256, 0, s>>>(
+            F, ring + (n - 1) * 5 * B * ring_len, 0x3, ring_len, (int)B, g);
+        SEIS_CHECK_LAUNCH();
+      }
+      SEIS_PROF(SEIS_KERNEL_ELASTIC_BWD, s);
+      SEIS_EL_LAUNCH_V(k_el_adj_rz, S, VX, dim3(tm2.per * kXcds), s, coef, dti,
+                       CFields{{X.f[0], X.f[1], X.f[2], X.f[3], X.f[4]}}, Y,
+                       CFields{{F.f[0], F.f[1], F.f[2], F.f[3], F.f[4]}}, Fn, grad_coef, recon,
+                       (int)B, g, tm2);
+      SEIS_PROF(SEIS_KERNEL_ELASTIC_BWD, s);
+      { Fields t = X; X = Y; Y = t; }
+      { Fields t = F; F = Fn; Fn = t; }
+      if (n >= 1) {
+        k_el_ring_restore<<<blocks_for(rwork), 256, 0, s>>>(
+            F, ring + (n - 1) * 5 * B * ring_len, 0x1c, ring_len, (int)B, g);
+        SEIS_CHECK_LAUNCH();
+        if (snaps && n % snap_every == 0 && n <= nt - 1) {
+          const float* sp = snaps + (n / snap_every - 1) * 5 * field;
+          for (int f = 0; f < 5; ++f) {
+            k_el_copy<<<blocks_for(field), 256, 0, s>>>(sp + f * field, F.f[f], field);
+            SEIS_CHECK_LAUNCH();
+          }
+        }
+        k_el_source<<<blocks_for(B), 256, 0, s>>>(F, src_zx, wavelet + (n - 1), -1.0f,
+                                                  src_mask, (int)B, g);
+        SEIS_CHECK_LAUNCH();
+        if (R > 0) {
+          k_el_inject<<<blocks_for(R * nch), 256, 0, s>>>(X, rec_flat, R, rec_field, nch,
+                                                          rec_grad + (n - 1) * R * nch);
+          SEIS_CHECK_LAUNCH();
+        }
+      }
+    }
+    return 0;
+  }
   for (int64_t n = nt - 1; n >= 0; --n) {
     // reverse of forward step n: fields hold state n+1 (stresses pre-source);
     // X holds the cotangent of state n+1.  recon rebuilds state n.

@@ -211,7 +211,7 @@ class ElasticLoop(torch.autograd.Function):
         if need_grad:
             ctx.save_for_backward(vp, vs, rho, cf, wavelet, src_zx, rec_flat, rf)
             ctx.state = (ring, snaps, fields)
-            ctx.meta = (B, dt, h, S, K, _mask(src_type), nch)
+            ctx.meta = (B, dt, h, S, K, _mask(src_type), nch, bool(fused))
         return rec
 
     @staticmethod
@@ -220,7 +220,7 @@ class ElasticLoop(torch.autograd.Function):
         vp, vs, rho, cf, wavelet, src_zx, rec_flat, rf = ctx.saved_tensors
         ring, snaps, fields = ctx.state
         ctx.state = None
-        B, dt, h, S, K, smask, nch = ctx.meta
+        B, dt, h, S, K, smask, nch, fused = ctx.meta
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -228,12 +228,14 @@ class ElasticLoop(torch.autograd.Function):
         grad_rec = grad_rec.contiguous().to(torch.float32)
         adj_a = [torch.empty_like(fields[0]) for _ in range(5)]
         adj_b = [torch.empty_like(fields[0]) for _ in range(5)]
+        scratch = [torch.empty_like(fields[0]) for _ in range(5)] if fused else None
         gco = torch.empty((G, 4, nz, nx), device=vp.device)
         _lib.call("seis_elastic_backward", _lib.ptr(cf), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), smask, _lib.ptr(rec_flat), R, _lib.ptr(rf), nch,
                   _lib.ptr(grad_rec), _lib.ptr(ring), _lib.ptr(snaps),
                   K if snaps is not None else 0, _lib.ptr_array(fields), _lib.ptr_array(adj_a),
-                  _lib.ptr_array(adj_b), _lib.ptr(gco), B, nz, nx, dt, h, NPML, S,
+                  _lib.ptr_array(adj_b), _lib.ptr_array(scratch) if fused else None,
+                  _lib.ptr(gco), B, nz, nx, dt, h, NPML, S,
                   _lib.stream_of(vp))
         gvp, gvs, grho = (torch.empty_like(vp) for _ in range(3))
         _lib.call("seis_elastic_chain", _lib.ptr(gco), G, _lib.ptr(vp), _lib.ptr(vs),
@@ -245,7 +247,8 @@ class ElasticLoop(torch.autograd.Function):
 def propagate(vp, vs, rho, d, wavelet, src_zx, rec_flat, B, dt, h, src_type=("txx", "tzz"),
               rec_type=("vx", "vz"), shots_per_group=None, snapshot_every=None, fused=True):
     """Records (nt, R, nch) of B shots; differentiable w.r.t. vp, vs, rho.
-    fused: both stages of a forward step in one pass (five more field buffers)."""
+    fused: each forward step and each reverse step of the adjoint in one pass
+    (five more field buffers)."""
     return ElasticLoop.apply(vp, vs, rho, d, wavelet.contiguous(), src_zx.contiguous(),
                              rec_flat.contiguous(), int(B), _f(dt), _f(h), tuple(src_type),
                              tuple(rec_type), shots_per_group, snapshot_every, bool(fused))

@@ -194,8 +194,9 @@ def test_lane_vector_widths(nx_i):
 
 @pytest.mark.parametrize("nt,S,nx_i", [(120, None, 60), (121, 1, 62), (1, 2, 61), (2, 8, 60)])
 def test_fused_step_matches_two_stages(nt, S, nx_i):
-    """One pass per step (stresses of the neighbouring rows recomputed, lane
-    shuffles) gives bit-identical records and the same gradients."""
+    """One pass per forward step and per reverse step (neighbouring rows
+    recomputed, lane shuffles, ping-pong buffers) gives bit-identical records
+    and bit-identical gradients."""
     c = elastic_case(nx_i=nx_i, nt=nt, delay=min(40, nt))
     t = elastic_dev(c)
     rng = np.random.default_rng(nt)
@@ -211,7 +212,6 @@ def test_fused_step_matches_two_stages(nt, S, nx_i):
         (rec * G).sum().backward()
         out[fused] = (rec.detach().cpu().numpy(), [x.grad.cpu().numpy() for x in m])
     assert np.array_equal(out[False][0], out[True][0])
-    for a, b in zip(out[True][1], out[False][1]):
-        e = rel_l2(a, b) if np.abs(b).max() > 0 else float(np.abs(a).max())
-        assert e <= 1e-6, e
+    for name, a, b in zip(("vp", "vs", "rho"), out[True][1], out[False][1]):
+        assert np.array_equal(a, b), (name, float(np.abs(a - b).max()), float(np.abs(b).max()))
     print(f"elastic fused step nt={nt} S={S} nx={c.nx}: records equal, gradients equal")
