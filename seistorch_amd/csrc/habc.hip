@@ -174,7 +174,8 @@ __device__ __forceinline__ void habc_fwd_cell(int z, int x, int grp, const float
                                               float* __restrict__ out,
                                               const int32_t* __restrict__ src,
                                               const float* __restrict__ wn, int B,
-                                              const HabcCtx& hc, const seis::TimeConsts& k) {
+                                              const HabcCtx& hc, const seis::TimeConsts& k,
+                                              int ng = 1) {
   const Grid& g = hc.g;
   const int i = z * g.nx + x;
   const float c = vp[i];
@@ -191,7 +192,8 @@ __device__ __forceinline__ void habc_fwd_cell(int z, int x, int grp, const float
   if (z >= g.nz - kW && x == z - (g.nz - kW)) { d1 = 1; d2 = 2; }
   if (z >= g.nz - kW && x == g.nx - kW + (z - (g.nz - kW))) { d1 = 1; d2 = 3; }
   const float w = src ? *wn : 0.0f;
-  const int b0 = grp * S, nb = min(S, B - b0);
+  // groups grp .. grp+ng-1 of S shots
+  const int b0 = grp * S, nb = min(S * ng, B - b0);
   for (int s = 0; s < nb; ++s) {
     const int bb = b0 + s;
     const int64_t off = (int64_t)bb * g.plane;
@@ -246,18 +248,19 @@ __global__ __launch_bounds__(256) void k_habc_fwd_deep(const float* __restrict__
                                                        const float* __restrict__ wn, int B,
                                                        HabcCtx hc, seis::TimeConsts k, Deep dp,
                                                        TileMap tm, const float* __restrict__ bw,
-                                                       Frame f) {
-  // blocks [0, tm.per*8) run Deep's lane vectors, the rest the frame's cells,
-  // so both kinds of work share the machine within one launch
-  const int nd = tm.per * kXcds;
-  if ((int)blockIdx.x >= nd) {
+                                                       Frame f, int G) {
+  // blocks [0, f.per*8) run the frame's cells, one cell per thread for all G
+  // shot groups (the per-cell set-up once per step), the rest Deep's lane
+  // vectors, so both kinds of work share the machine within one launch
+  const int nf = f.per * kXcds;
+  if ((int)blockIdx.x < nf) {
     int zc, xc, gc;
-    if (frame_cell(f, (int)blockIdx.x - nd, zc, xc, gc))
-      habc_fwd_cell<S>(zc, xc, gc, vp, bw, u, p, out, src, wn, B, hc, k);
+    if (frame_cell(f, (int)blockIdx.x, zc, xc, gc))
+      habc_fwd_cell<S>(zc, xc, 0, vp, bw, u, p, out, src, wn, B, hc, k, G);
     return;
   }
   int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;
+  if (!tile_of_w(tm, (int)blockIdx.x - nf, tx, tzi, grp)) return;
   const Grid& g = hc.g;
   const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
   const int z = dp.z0 + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
@@ -305,7 +308,7 @@ __device__ __forceinline__ void habc_adj_cell(
     const float* __restrict__ y1, const float* __restrict__ y2, float* __restrict__ out,
     const float* __restrict__ um, float* __restrict__ uc, float* __restrict__ gpart,
     const int32_t* __restrict__ src, const float* __restrict__ wn, int recon, int B,
-    const HabcCtx& hc, const seis::TimeConsts& k) {
+    const HabcCtx& hc, const seis::TimeConsts& k, int ng = 1) {
   const Grid& g = hc.g;
   const int i = z * g.nx + x;
   const float A0 = 1.0f / k.idt2;
@@ -357,39 +360,42 @@ __device__ __forceinline__ void habc_adj_cell(
   }
   const bool in = interior(z, x, g);
   const float w = wn ? *wn : 0.0f;
-  float gacc = 0.0f;
-  const int b0 = grp * S, nb = min(S, B - b0);
-  for (int s = 0; s < nb; ++s) {
-    const int bb = b0 + s;
-    const int64_t off = (int64_t)bb * g.plane;
-    float acc = 0.0f;
-    if (y1) {
-      const float* Y = y1 + off;
-      const float Lq = lap_taps(z > 0 ? qzm * Y[i - g.nx] : 0.0f, x > 0 ? qxm * Y[i - 1] : 0.0f,
-                                q0 * Y[i], x < g.nx - 1 ? qxp * Y[i + 1] : 0.0f,
-                                z < g.nz - 1 ? qzp * Y[i + g.nx] : 0.0f, k.ih2);
-      acc = (k.two_idt2 * (A0 * al0)) * Y[i] + Lq;
-      for (int o = 0; o < no; ++o) acc += oc1[o] * Y[on[o]];
-    }
-    if (y2) {
-      const float* Y = y2 + off;
-      acc += -(B0 * (A0 * al0)) * Y[i];
-      for (int o = 0; o < no; ++o) acc += oc2[o] * Y[on[o]];
-    }
-    out[off + i] = acc;
-    if (in && recon >= 0 && um) {
-      const float Lu = lap5(um + off, z, x, i, g, k.ih2);
-      gacc += Lu * acc;
-      if (recon == 1) {
-        float un = uc[off + i];
-        if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
-        uc[off + i] = ((k.two_idt2 * um[off + i] + (c * c) * Lu) - un / A0) / B0;
-      } else if (recon == 2) {
-        uc[off + i] = 0.0f;
+  // groups grp .. grp+ng-1 of S shots, each with its own gradient partial
+  for (int gi = grp; gi < grp + ng; ++gi) {
+    float gacc = 0.0f;
+    const int b0 = gi * S, nb = min(S, B - b0);
+    for (int s = 0; s < nb; ++s) {
+      const int bb = b0 + s;
+      const int64_t off = (int64_t)bb * g.plane;
+      float acc = 0.0f;
+      if (y1) {
+        const float* Y = y1 + off;
+        const float Lq = lap_taps(z > 0 ? qzm * Y[i - g.nx] : 0.0f, x > 0 ? qxm * Y[i - 1] : 0.0f,
+                                  q0 * Y[i], x < g.nx - 1 ? qxp * Y[i + 1] : 0.0f,
+                                  z < g.nz - 1 ? qzp * Y[i + g.nx] : 0.0f, k.ih2);
+        acc = (k.two_idt2 * (A0 * al0)) * Y[i] + Lq;
+        for (int o = 0; o < no; ++o) acc += oc1[o] * Y[on[o]];
+      }
+      if (y2) {
+        const float* Y = y2 + off;
+        acc += -(B0 * (A0 * al0)) * Y[i];
+        for (int o = 0; o < no; ++o) acc += oc2[o] * Y[on[o]];
+      }
+      out[off + i] = acc;
+      if (in && recon >= 0 && um) {
+        const float Lu = lap5(um + off, z, x, i, g, k.ih2);
+        gacc += Lu * acc;
+        if (recon == 1) {
+          float un = uc[off + i];
+          if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
+          uc[off + i] = ((k.two_idt2 * um[off + i] + (c * c) * Lu) - un / A0) / B0;
+        } else if (recon == 2) {
+          uc[off + i] = 0.0f;
+        }
       }
     }
+    if (in && um && gpart) gpart[(int64_t)gi * g.plane + i] += 2.0f * c * A0 * gacc;
   }
-  if (in && um && gpart) gpart[(int64_t)grp * g.plane + i] += 2.0f * c * A0 * gacc;
 }
 
 template <int S>
@@ -408,12 +414,12 @@ __global__ __launch_bounds__(256) void k_habc_adj(
   habc_adj_cell<S>(z, x, grp, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc, k);
 }
 
-// The fused adjoint step in one launch of two block ranges.  The first
-// covers the rectangle Deep (z in [z0, z1), x in [x0, x1), at least kW+3
+// The fused adjoint step in one launch of two block ranges.  One covers the
+// rectangle Deep (z in [z0, z1), x in [x0, x1), at least kW+3
 // cells from every edge, x0 and x1 multiples of VX): there no strip term
 // reaches, so a lane runs a vectorised copy of the per-cell deep path of
 // habc_adj_cell with the same operands in the same order (bit-identical).
-// The second runs habc_adj_cell on every other cell, one cell per thread.
+// The other runs habc_adj_cell on every other cell, one cell per thread.
 // Requires y1, y2, um and gpart (the fused loop's call).
 template <int S, int VX>
 __global__ __launch_bounds__(256) void k_habc_adj_deep(
@@ -421,18 +427,20 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
     const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
     float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
     const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
-    TileMap tm, Frame f) {
-  // blocks [0, tm.per*8) run Deep's lane vectors, the rest the frame's cells
-  const int nd = tm.per * kXcds;
-  if ((int)blockIdx.x >= nd) {
+    TileMap tm, Frame f, int G) {
+  // blocks [0, f.per*8) run the frame's cells, one cell per thread for all G
+  // shot groups (the per-cell set-up once per step, not once per group), the
+  // rest Deep's lane vectors; the frame goes first, being the longer work
+  const int nf = f.per * kXcds;
+  if ((int)blockIdx.x < nf) {
     int zc, xc, gc;
-    if (frame_cell(f, (int)blockIdx.x - nd, zc, xc, gc))
-      habc_adj_cell<S>(zc, xc, gc, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc,
-                       k);
+    if (frame_cell(f, (int)blockIdx.x, zc, xc, gc))
+      habc_adj_cell<S>(zc, xc, 0, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc,
+                       k, G);
     return;
   }
   int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;
+  if (!tile_of_w(tm, (int)blockIdx.x - nf, tx, tzi, grp)) return;
   const Grid& g = hc.g;
   const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
   const int z = dp.z0 + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
@@ -763,8 +771,9 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
   Deep dp{kW, (int)nz - kW, (kW + VX - 1) / VX * VX, ((int)nx - kW) / VX * VX};
   if (dp.x1 <= dp.x0 || dp.z1 <= dp.z0) dp = Deep{0, 0, 0, 0};
   const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
-  const TileMap tmd = make_tiles(gd, VX, 1, (int)((B + S - 1) / S));
-  const Frame fr = make_frame(g, dp, (int)((B + S - 1) / S));
+  const int G = (int)((B + S - 1) / S);
+  const TileMap tmd = make_tiles(gd, VX, 1, G);
+  const Frame fr = make_frame(g, dp, 1);  // one frame thread runs all G groups
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -783,7 +792,7 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
     float* out = P[(n + 1) % 3];
     SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
     SEIS_HABC_LAUNCH_V(k_habc_fwd_deep, S, VX, dim3((tmd.per + fr.per) * kXcds), s, vp, un,
-                       up, out, src_zx, wavelet + n, (int)B, hc, k, dp, tmd, b, fr);
+                       up, out, src_zx, wavelet + n, (int)B, hc, k, dp, tmd, b, fr, G);
     SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
@@ -828,7 +837,7 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   if (dp.x1 <= dp.x0 || dp.z1 <= dp.z0) dp = Deep{0, 0, 0, 0};  // no deep cells
   const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
   const TileMap tmd = make_tiles(gd, VX, 1, G);
-  const Frame fr = make_frame(g, dp, G);
+  const Frame fr = make_frame(g, dp, 1);  // one frame thread runs all G groups
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -848,7 +857,7 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX, dim3((tmd.per + fr.per) * kXcds), s, vp, b, L1,
                        L2, L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k,
-                       dp, tmd, fr);
+                       dp, tmd, fr, G);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
     if (work > 0) {
