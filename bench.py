@@ -45,14 +45,16 @@ def parse():
                    help="time steps of the bounded CPU sample (1 shot, full grid)")
     p.add_argument("--nt", type=int, default=None,
                    help="shorter time axis for profiler runs (the metric is quoted at full nt)")
-    p.add_argument("--workload", choices=["acoustic", "habc", "elastic", "generator", "precond"],
+    p.add_argument("--workload", choices=["acoustic", "habc", "elastic", "generator", "precond",
+                                          "filter"],
                    default="acoustic",
                    help="acoustic: 64-shot 800x2400 FWI gradient (the headline); habc: the "
                         "same with the HABC boundary; elastic: "
                         "32-shot 600x1800 vp/vs/rho gradient; generator: 256 uneven seeded "
                         "models, records + misfit gradient in one batch; precond: train the "
                         "U-Net preconditioner on generator pairs, then plain vs preconditioned "
-                        "FWI on the 800x2400 problem")
+                        "FWI on the 800x2400 problem; filter: zero-phase Butterworth filter "
+                        "of the 64-shot 800x2400 problem's records (multiscale FWI)")
     p.add_argument("--models", type=int, default=256, help="generator: number of models")
     p.add_argument("--graph", type=int, default=0,
                    help="generator: 1 records each batched loop into one hipGraph (kernel "
@@ -293,6 +295,73 @@ def run_generator(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_filter(args, dev):
+    """Multiscale record filter: the zero-phase 1-3 Hz order-3 Butterworth
+    band (the reference template's first scale) over the records of the
+    64-shot 800x2400 problem (nt = 5000, dt = 0.8 ms, 2400 receivers per shot,
+    153600 traces).  One step = one filtfilt of all records (forward and
+    backward recursion in FP64, one kernel).  value = samples (nt x traces)
+    per second."""
+    import torch
+    from seistorch_amd import signal as sg
+    nt, shots, nrec, dt = 5000, 64, 2400, 0.8e-3
+    if args.nt:
+        nt = args.nt
+    b, a = sg.butter(3, [1.0, 3.0], dt)
+    g = torch.Generator(device=dev).manual_seed(20230503)
+    x = torch.randn((nt, shots * nrec), device=dev, generator=g)
+    samples = x.numel()
+    for _ in range(args.warmup):
+        sg.filtfilt(x, b, a)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[2 * i].record()
+        y = sg.filtfilt(x, b, a)
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    launch_s = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)) \
+        / args.steps / 1e3
+    bytes_per_sample = 24.0  # read x (4), write + read the FP64 pass (16), write y (4)
+    achieved = bytes_per_sample * samples / launch_s
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        per = json.load(open(tpath)).get("k_filtfilt_bytes_per_sample")
+        traffic = round(per * samples) if per else None
+    out = {
+        "metric": "Gsamples/s zero-phase record filter (multiscale FWI)",
+        "value": round(samples * args.steps / el / 1e9, 3), "unit": "Gsamples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Gaussian records of the 64-shot 800x2400 problem's shape)",
+        "config": {"workload": "zero-phase Butterworth filter of records (filtfilt, FP64)",
+                   "nt": nt, "traces": shots * nrec, "dt": dt, "band_hz": [1.0, 3.0],
+                   "order": 3, "coefficients": len(b)},
+        "roofline": {"bound": "hbm", "kernel": "k_filtfilt", "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": traffic, "avg_launch_us": round(launch_s * 1e6, 2),
+                     "launches": args.steps, "bytes_per_sample": bytes_per_sample},
+        "checksum": float(y.double().abs().sum()),
+    }
+    if args.copy_calibration:
+        out["copy_calibration_bytes"] = copy_calibration(dev)
+    if not args.no_cpu_baseline:
+        from oracle import signal as osg
+        xs = x.double().cpu().numpy()  # the whole workload: 64 shots, full nt
+        t1 = time.perf_counter()
+        osg.filtfilt(b, a, xs)
+        cel = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(xs.size / cel / 1e9, 5), "unit": "Gsamples/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"all {shots * nrec} traces x {nt} samples, "
+                                         f"oracle/signal.filtfilt (numpy FP64, vectorised over "
+                                         f"traces, one thread), {cel:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
 def run_precond(args, dev):
     """Learned preconditioner: pairs from the generator, AdamW training with
     the hand-written kernels, then plain and preconditioned FWI on the
@@ -370,6 +439,8 @@ def main():
         return run_generator(args, dev)
     if args.workload == "precond":
         return run_precond(args, dev)
+    if args.workload == "filter":
+        return run_filter(args, dev)
     elastic = args.workload == "elastic"
     if elastic:
         P = build_elastic_problem(dev, nt_override=args.nt)

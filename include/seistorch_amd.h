@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 1
+#define SEIS_ABI_VERSION 2
 
 #ifdef __cplusplus
 extern "C" {
@@ -273,6 +273,19 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
                           float* const* adj_b, float* const* scratch, float* grad_coef, int64_t B,
                           int64_t nz, int64_t nx, float dt, float h, int32_t npml,
                           int32_t shots_per_group, void* stream);
+
+/* ---------------- multiscale record filtering ----------------------------- */
+/* Zero-phase IIR filter of traces along time: replaces seistorch/signal.py:91-96
+ * (backend 'torch', called from seistorch_dist.py:222-225), i.e.
+ * torchaudio.functional.filtfilt(x.double(), a, b, clamp=False).float(): an
+ * lfilter with zero initial state over time, then over reversed time, in
+ * FP64, without edge padding.  x, y: (nt, ntr) float32, time-major, traces
+ * contiguous (the records layout; y may be x); work: (nt, ntr) FP64
+ * scratch; b, a: host arrays of ncoef FP64 coefficients (scipy.signal.butter
+ * output, a[0] != 0), 1 <= ncoef <= 32.  Self-adjoint: the gradient of a
+ * loss through the filter is the same call on the cotangent. */
+int seis_filtfilt(const float* x, float* y, double* work, int64_t nt, int64_t ntr,
+                  const double* b, const double* a, int32_t ncoef, void* stream);
 
 /* ---------------- learned gradient preconditioner (U-Net) ----------------- */
 /* No reference counterpart: a small U-Net maps (gradient, model) to an

@@ -70,3 +70,21 @@ def test_batch_and_unet_argument_checks():
                   None)
     with pytest.raises(RuntimeError, match="even H, W"):
         _lib.call("seis_unet_avgpool2", None, 1, 15, 16, 32, None, None)
+
+
+def test_filtfilt_argument_checks():
+    """Host-side validation of the record filter (before any device call)."""
+    import ctypes
+    from seistorch_amd import _lib
+    b = (ctypes.c_double * 4)(0.1, 0.3, 0.3, 0.1)
+    a = (ctypes.c_double * 4)(1.0, -0.5, 0.2, 0.0)
+    a0 = (ctypes.c_double * 4)(0.0, -0.5, 0.2, 0.0)
+    fake = ctypes.c_void_p(16)  # never dereferenced: validation fails first
+    with pytest.raises(RuntimeError, match="null pointer"):
+        _lib.call("seis_filtfilt", None, fake, fake, 10, 10, b, a, 4, None)
+    with pytest.raises(RuntimeError, match="nt >= 1"):
+        _lib.call("seis_filtfilt", fake, fake, fake, 0, 10, b, a, 4, None)
+    with pytest.raises(RuntimeError, match="ncoef 33"):
+        _lib.call("seis_filtfilt", fake, fake, fake, 10, 10, b, a, 33, None)
+    with pytest.raises(RuntimeError, match="a\\[0\\]"):
+        _lib.call("seis_filtfilt", fake, fake, fake, 10, 10, b, a0, 4, None)
