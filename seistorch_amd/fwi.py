@@ -1,5 +1,8 @@
-"""Single-GPU acoustic FWI driver: plain-gradient iterations and iterations
-preconditioned by the learned U-Net, compared at equal wall-clock time.
+"""Single-GPU acoustic FWI driver: the reference's multiscale loop
+(``run_multiscale``: band-filtered records, Adam or max-normalised steepest
+descent per frequency scale), and plain-gradient iterations against
+iterations preconditioned by the learned U-Net, compared at equal wall-clock
+time (``run_plain`` / ``run_preconditioned``).
 
 The plain method follows the reference's FWI loop (seistorch/fwi.py and
 seistorch_dist.py:166-303: forward, L2 misfit of loss.py:409-421, backward,
@@ -43,6 +46,9 @@ class Problem:
     mute: int = 5
     vmin: float = 1400.0
     vmax: float = 5000.0
+    band: object = "all"     # multiscale frequency spec (seistorch/signal.py:25-37)
+    filter_ord: int = 3      # Butterworth order (template.yml:44)
+    _fobs: tuple = None      # (band, filtered observed records)
 
     def interior(self, v):
         return v[PAD:v.shape[0] - PAD, PAD:v.shape[1] - PAD]
@@ -55,14 +61,32 @@ class Problem:
             return ac.propagate(vp, self.d, self.wavelet, self.src, self.rec, self.B, self.dt,
                                 self.h)
 
+    def _filter(self, rec):
+        """Records filtered to the current band (seistorch_dist.py:222-225:
+        synthetic and observed records through the same zero-phase filter);
+        differentiable."""
+        if self.band == "all":
+            return rec
+        from . import signal as sg
+        b, a = sg.butter(self.filter_ord, self.band, self.dt)
+        return sg.filtfilt(rec, b, a)
+
+    def observed(self):
+        if self.band == "all":
+            return self.obs
+        if self._fobs is None or self._fobs[0] != self.band:
+            with torch.no_grad():
+                self._fobs = (self.band, self._filter(self.obs))
+        return self._fobs[1]
+
     def misfit(self, vp):
         with torch.no_grad():
-            return float(torch.sum((self.records(vp) - self.obs) ** 2))
+            return float(torch.sum((self._filter(self.records(vp)) - self.observed()) ** 2))
 
     def gradient(self, vp):
         v = vp.detach().clone().requires_grad_(True)
         rec = ac.propagate(v, self.d, self.wavelet, self.src, self.rec, self.B, self.dt, self.h)
-        loss = torch.sum((rec - self.obs) ** 2)
+        loss = torch.sum((self._filter(rec) - self.observed()) ** 2)
         loss.backward()
         g = self.interior(v.grad).clone()
         g[:self.mute] = 0.0
@@ -184,3 +208,67 @@ def time_to(hist, target):
         if m <= target:
             return ti
     return None
+
+
+def _gd_direction(g, grad_clamp=True):
+    """seistorch/optimizer.py:124-142 (Steepestdescent): the gradient clamped
+    to its 2 % and 98 % quantiles, scaled so that its largest entry is 1 (the
+    update is then lr times this)."""
+    if grad_clamp:
+        q = torch.quantile(g.flatten().float(),
+                           torch.tensor([0.02, 0.98], device=g.device, dtype=torch.float32))
+        g = torch.clamp(g, min=float(q[0]), max=float(q[1]))
+    m = float(g.abs().max())
+    return g / m if m > 0 else g
+
+
+def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85,
+                   lr_decay=0.995, grad_clamp=True, log=None):
+    """The reference's FWI loop on one GPU (seistorch_dist.py:166-303): for
+    each frequency scale (seistorch/signal.py band spec, or "all") a fresh
+    optimizer with lr * scale_decay**scale (setup.py:157-183), `epochs`
+    iterations of forward, band-filtered L2 misfit, adjoint gradient (seabed
+    rows muted) and update, the learning rate decayed by lr_decay per epoch
+    (ExponentialLR).  optimizer "adam": torch.optim.Adam with eps = 1e-22
+    (optimizer.py:8-15); "steepestdescent": v -= lr * g / max|g| with the
+    2-98 % quantile clamp (optimizer.py:124-142).  The unpadded model is the
+    optimised variable; the boundary strips are re-padded from its edges after
+    every update (process.py:50-63).  Returns (vp, History)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vi = p.interior(p.vp_init).clone().contiguous()
+    vp = p.pad(vi)
+    hist = History()
+    band0 = p.band
+    try:
+        for si, band in enumerate(scales):
+            p.band = band
+            slr = lr * scale_decay ** si
+            x = torch.nn.Parameter(vi.clone())
+            opt = torch.optim.Adam([x], lr=slr, eps=1e-22) if optimizer == "adam" else None
+            sched = torch.optim.lr_scheduler.ExponentialLR(opt, lr_decay) if opt else None
+            cur = slr
+            for ep in range(epochs):
+                loss, g = p.gradient(p.pad(x.detach()))
+                if opt is not None:
+                    opt.zero_grad()
+                    x.grad = g.contiguous()
+                    opt.step()
+                    sched.step()
+                else:
+                    with torch.no_grad():
+                        x -= cur * _gd_direction(g, grad_clamp)
+                    cur *= lr_decay
+                with torch.no_grad():
+                    x.clamp_(p.vmin, p.vmax)
+                torch.cuda.synchronize()
+                hist.add(time.perf_counter() - t0, loss, p.rmse(p.pad(x.detach())),
+                         f"scale{si}")
+                if log:
+                    log(f"[multiscale] band {band} epoch {ep + 1}: misfit {loss:.4e} "
+                        f"rmse {hist.rmse[-1]:.1f}")
+            vi = x.detach().contiguous()
+            vp = p.pad(vi)
+    finally:
+        p.band = band0
+    return vp, hist
