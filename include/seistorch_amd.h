@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 2
+#define SEIS_ABI_VERSION 3
 
 #ifdef __cplusplus
 extern "C" {
@@ -40,7 +40,8 @@ int32_t seis_abi_version(void);
 #define SEIS_KERNEL_BATCH_FWD 6  /* k_fwd_batch, direct launches only */
 #define SEIS_KERNEL_BATCH_BWD 7  /* k_bwd_batch, direct launches only */
 #define SEIS_KERNEL_ACOUSTIC_FWD2 8  /* k_fwd2: two forward steps per launch */
-#define SEIS_PROFILE_KERNELS 9
+#define SEIS_KERNEL_ACOUSTIC_BWD2 9  /* k_bwd2: two reverse steps per launch */
+#define SEIS_PROFILE_KERNELS 10
 int seis_profile_enable(int32_t on);
 int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
 
@@ -90,8 +91,8 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
  * thread owns rows_per_thread (1, 2 or 4) consecutive rows of vec_x (1, 2 or
  * 4; must divide nx, pointers vec_x*4-byte aligned) consecutive cells.
  * If u_c and u_d (two more (B, nz, nx) buffers, distinct from u_a, u_b) are
- * given, two time steps run per launch (the intermediate field is staged in
- * LDS), with the same results; NULL keeps one step per launch. */
+ * given, two time steps run per launch (the intermediate field stays in
+ * registers), with the same results; NULL keeps one step per launch. */
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
@@ -106,13 +107,21 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
  * snap_every; u_a/u_b hold the final forward state (consumed); mu_a, mu_b are
  * (B, nz, nx) work buffers;
  * grad_part is ceil(B/shots_per_group) x (nz, nx) work; grad (nz, nx) gets
- * dJ/dvp on the interior [npml+1, n-npml-1) and zero elsewhere. */
+ * dJ/dvp on the interior [npml+1, n-npml-1) and zero elsewhere.
+ * If u_c, u_d, mu_c, mu_d (four more (B, nz, nx) buffers) and, when R > 0,
+ * rec_map are given, two reverse steps run per launch where the ring is
+ * available: rec_map (nz x nx int32) holds each cell's receiver index within
+ * a shot or -1, which requires that every shot has its receivers at the same
+ * cells in the same order (rec_flat[b*R/B + r] = b*nz*nx + cell_r) and no cell
+ * twice.  Results equal the one-step path except for the order of the
+ * gradient additions at receiver cells.  NULL keeps one step per launch. */
 int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                            const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                            const float* rec_grad, const float* ring, const float* snaps,
                            int32_t snap_every, float* u_a, float* u_b,
-                           float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
-                           int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                           float* mu_a, float* mu_b, float* u_c, float* u_d, float* mu_c,
+                           float* mu_d, const int32_t* rec_map, float* grad_part, float* grad,
+                           int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
                            int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                            void* stream);
 

@@ -504,8 +504,231 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
   bwd_tile<S, RZ, VX>(tx, tzi, grp, vp, d, um, uc, m1, m2, gpart, src, wn, recon, B, g, k);
 }
 
+// Two reverse iterations per launch, kk and kk-1 (both rebuilding, kk >= 4),
+// over rotating buffers (neighbouring tiles still read the inputs around
+// this one): reads ua = u^{kk-1}, ub = u^{kk}, ma = mu^{kk+1}, mb = mu^{kk+2};
+// writes mc = mu^{kk} (with the receivers' A*s^{kk}), md = mu^{kk-1} (its
+// receivers are added afterwards by k_bwd_post), uc = u^{kk-2} (interior
+// rebuilt, ring from ring_slot = the ring of u^{kk-2}, elsewhere 0) and ud =
+// u^{kk-3} on the interior; adds both iterations' gradient terms to gpart.
+// The adjoint source of iteration kk enters mu^{kk} inside the launch (the
+// second iteration reads it): rmap (nz x nx) holds each cell's receiver
+// index within a shot (-1 if none), shot b's cotangent at rg[b*Rs + r]; all
+// shots share the receiver cells.  Layout as k_fwd2: a lane computes the
+// first iteration on its RZ rows and the rows above and below, x-neighbours
+// come by shuffles, lanes 0 and 63 load their outer neighbours and only the
+// 62 inner lanes write.  Every value uses bwd_tile's operands and order, so
+// fields and gradients equal two k_bwd iterations bit for bit, except the
+// gradient at receiver cells, where the receivers' gradient shares are added
+// after both iterations' terms instead of between them.
+constexpr int kBwd2MaxVX = 4;
+
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_bwd2(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ ua,
+    const float* __restrict__ ub, float* __restrict__ uc, float* __restrict__ ud,
+    const float* __restrict__ ma, const float* __restrict__ mb, float* __restrict__ mc,
+    float* __restrict__ md, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, const int32_t* __restrict__ rmap, const float* __restrict__ rg,
+    int Rs, const float* __restrict__ ring_slot, int64_t ring_len, int B, Grid g,
+    seis::TimeConsts k, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
+  const int lane = (int)(threadIdx.x % kLanes);
+  const int x0 = (tx * kFwd2Out + lane - 1) * VX;
+  const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
+  if (zb >= g.nz) return;  // uniform over the wave
+  const bool xin = x0 >= 0 && x0 < g.nx;  // nx is a multiple of VX
+  const bool inner = lane > 0 && lane < kLanes - 1;
+  const float wA = wn[1], wB = wn[0];  // w[kk-1], w[kk-2]
+  const int ur0 = g.npml, ur1 = g.nz - 1 - g.npml;  // rows of u that are read
+  bool xint[VX];
+  bool anyi = false, anyu = false;
+#pragma unroll
+  for (int e = 0; e < VX; ++e) {
+    const int x = x0 + e;
+    xint[e] = xin && x > g.npml && x < g.nx - 1 - g.npml;
+    anyi = anyi || xint[e];
+    anyu = anyu || (xin && x >= g.npml && x <= g.nx - 1 - g.npml);
+  }
+  // c^2 on rows zb-2 .. zb+RZ+1 (index p = z - zb + 2); A, Bc and the
+  // receiver map on rows zb-1 .. zb+RZ (index q = z - zb + 1)
+  float c2[RZ + 4][VX], A[RZ + 2][VX], Bc[RZ + 2][VX], cc[RZ][VX];
+  int rm[RZ + 2][VX];
+#pragma unroll
+  for (int p = 0; p < RZ + 4; ++p) {
+    const int z = zb - 2 + p;
+    if (xin && z >= 0 && z < g.nz) vload<VX>(vp + z * g.nx + x0, c2[p]);
+    else vzero<VX>(c2[p]);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) c2[p][e] *= c2[p][e];
+  }
+#pragma unroll
+  for (int q = 0; q < RZ + 2; ++q) {
+    const int z = min(max(zb - 1 + q, 0), g.nz - 1);
+    const int i = z * g.nx + (xin ? x0 : 0);
+    float dd[VX];
+    vload<VX>(d + i, dd);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      A[q][e] = 1.0f / (k.idt2 + dd[e] * k.idt);
+      Bc[q][e] = k.idt2 - dd[e] * k.idt;
+      rm[q][e] = (rmap && xin && zb - 1 + q >= 0 && zb - 1 + q < g.nz) ? rmap[i + e] : -1;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int z = min(zb + r, g.nz - 1);
+    if (xin) vload<VX>(vp + z * g.nx + x0, cc[r]);
+    else vzero<VX>(cc[r]);
+  }
+  float gA[RZ][VX], gB[RZ][VX];
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    vzero<VX>(gA[r]);
+    vzero<VX>(gB[r]);
+  }
+  const int b0 = grp * S;
+  const int nb = min(S, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int b = b0 + s;
+    const int64_t off = (int64_t)b * g.plane;
+    const int sz = src[2 * b], sx = src[2 * b + 1];
+    // mu^{kk+1} and u^{kk-1} on rows p
+    float M1[RZ + 4][VX], U1[RZ + 4][VX];
+#pragma unroll
+    for (int p = 0; p < RZ + 4; ++p) {
+      const int z = zb - 2 + p;
+      if (xin && z >= 0 && z < g.nz) vload<VX>(ma + off + z * g.nx + x0, M1[p]);
+      else vzero<VX>(M1[p]);
+      if (anyu && z >= ur0 && z <= ur1) vload<VX>(ua + off + z * g.nx + x0, U1[p]);
+      else vzero<VX>(U1[p]);
+    }
+    // iteration kk on rows q: mu^{kk} (MU, with the receivers' source; the
+    // own rows also without it, for the gradient) and u^{kk-2} (UA)
+    float MU[RZ + 2][VX], MUn[RZ][VX], UA[RZ + 2][VX];
+#pragma unroll
+    for (int q = 0; q < RZ + 2; ++q) {
+      const int z = zb - 1 + q;
+      const bool zok = z >= 0 && z < g.nz;
+      float pl = __shfl_up(c2[q + 1][VX - 1] * M1[q + 1][VX - 1], 1, kLanes);
+      float pr = __shfl_down(c2[q + 1][0] * M1[q + 1][0], 1, kLanes);
+      float ul = __shfl_up(U1[q + 1][VX - 1], 1, kLanes);
+      float ur = __shfl_down(U1[q + 1][0], 1, kLanes);
+      const bool urow = z >= ur0 && z <= ur1;
+      if (lane == 0) {
+        const bool ok = xin && zok && x0 > 0;
+        const int j = z * g.nx + x0 - 1;
+        const float cl = ok ? vp[j] : 0.0f;
+        pl = cl * cl * (ok ? ma[off + j] : 0.0f);
+        ul = (ok && urow) ? ua[off + j] : 0.0f;
+      }
+      if (lane == kLanes - 1) {
+        const bool ok = xin && zok && x0 + VX < g.nx;
+        const int j = z * g.nx + x0 + VX;
+        const float cr = ok ? vp[j] : 0.0f;
+        pr = cr * cr * (ok ? ma[off + j] : 0.0f);
+        ur = (ok && urow) ? ua[off + j] : 0.0f;
+      }
+      if (!(xin && zok)) {
+        vzero<VX>(MU[q]);
+        vzero<VX>(UA[q]);
+        if (q >= 1 && q <= RZ) vzero<VX>(MUn[q - 1]);
+        continue;
+      }
+      const int i = z * g.nx + x0;
+      float m2v[VX];
+      vload<VX>(mb + off + i, m2v);
+      const bool zi = z > g.npml && z < g.nz - 1 - g.npml;
+      float ukk[VX];
+      if (anyi && zi) vload<VX>(ub + off + i, ukk);
+      else vzero<VX>(ukk);
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const float left = e == 0 ? pl : c2[q + 1][e - 1] * M1[q + 1][e - 1];
+        const float right = e == VX - 1 ? pr : c2[q + 1][e + 1] * M1[q + 1][e + 1];
+        const float Lm = lap_taps(c2[q][e] * M1[q][e], left, c2[q + 1][e] * M1[q + 1][e], right,
+                                  c2[q + 2][e] * M1[q + 2][e], k.ih2);
+        float mu = acoustic_update(A[q][e], Bc[q][e], k.two_idt2, M1[q + 1][e], m2v[e], 1.0f, Lm);
+        if (q >= 1 && q <= RZ) MUn[q - 1][e] = mu;
+        if (rm[q][e] >= 0) mu = mu + A[q][e] * rg[(int64_t)b * Rs + rm[q][e]];
+        MU[q][e] = mu;
+        float uo = 0.0f;
+        if (zi && xint[e]) {
+          const float l = e == 0 ? ul : U1[q + 1][e - 1];
+          const float r = e == VX - 1 ? ur : U1[q + 1][e + 1];
+          const float Lu = lap_taps(U1[q][e], l, U1[q + 1][e], r, U1[q + 2][e], k.ih2);
+          if (q >= 1 && q <= RZ) gA[q - 1][e] += Lu * MUn[q - 1][e];
+          float un = ukk[e];
+          if (z == sz && x0 + e == sx) un -= wA;
+          uo = ((k.two_idt2 * U1[q + 1][e] + c2[q + 1][e] * Lu) - un / A[q][e]) / Bc[q][e];
+        } else if (urow) {
+          const int64_t jr = ring_index(z, x0 + e, g);
+          if (jr >= 0) uo = ring_slot[(int64_t)b * ring_len + jr];
+        }
+        UA[q][e] = uo;
+      }
+    }
+    // iteration kk-1 on the own rows (inner lanes)
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int q = r + 1;
+      const int z = zb + r;
+      const float pl = __shfl_up(c2[q + 1][VX - 1] * MU[q][VX - 1], 1, kLanes);
+      const float pr = __shfl_down(c2[q + 1][0] * MU[q][0], 1, kLanes);
+      const float ul = __shfl_up(UA[q][VX - 1], 1, kLanes);
+      const float ur = __shfl_down(UA[q][0], 1, kLanes);
+      if (!inner || !xin || z >= g.nz) continue;
+      const int i = z * g.nx + x0;
+      const bool zi = z > g.npml && z < g.nz - 1 - g.npml;
+      float mo[VX], uo[VX];
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        const float left = e == 0 ? pl : c2[q + 1][e - 1] * MU[q][e - 1];
+        const float right = e == VX - 1 ? pr : c2[q + 1][e + 1] * MU[q][e + 1];
+        const float Lm = lap_taps(c2[q][e] * MU[q - 1][e], left, c2[q + 1][e] * MU[q][e], right,
+                                  c2[q + 2][e] * MU[q + 1][e], k.ih2);
+        mo[e] = acoustic_update(A[q][e], Bc[q][e], k.two_idt2, MU[q][e], M1[q + 1][e], 1.0f, Lm);
+        uo[e] = 0.0f;
+        if (zi && xint[e]) {
+          const float l = e == 0 ? ul : UA[q][e - 1];
+          const float rr = e == VX - 1 ? ur : UA[q][e + 1];
+          const float Lu = lap_taps(UA[q - 1][e], l, UA[q][e], rr, UA[q + 1][e], k.ih2);
+          gB[r][e] += Lu * mo[e];
+          float un = U1[q + 1][e];
+          if (z == sz && x0 + e == sx) un -= wB;
+          uo[e] = ((k.two_idt2 * UA[q][e] + c2[q + 1][e] * Lu) - un / A[q][e]) / Bc[q][e];
+        }
+      }
+      vstore<VX>(mc + off + i, MU[q]);
+      vstore<VX>(md + off + i, mo);
+      vstore<VX>(uc + off + i, UA[q]);
+      vstore<VX>(ud + off + i, uo);
+    }
+  }
+  if (!inner || !anyi) return;
+  float* G = gpart + (int64_t)grp * g.plane;
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int z = zb + r;
+    if (z > g.npml && z < g.nz - 1 - g.npml) {
+      float gv[VX];
+      vload<VX>(G + z * g.nx + x0, gv);
+#pragma unroll
+      for (int e = 0; e < VX; ++e) {
+        if (!xint[e]) continue;
+        gv[e] += 2.0f * cc[r][e] * gA[r][e];
+        gv[e] += 2.0f * cc[r][e] * gB[r][e];
+      }
+      vstore<VX>(G + z * g.nx + x0, gv);
+    }
+  }
+}
+
 // After k_bwd: inject the adjoint source A*s^k at the receivers of mu^k (and
 // its gradient share 2c*L(u^{k-1})*A*s^k), restore the ring of u^{k-2}.
+// m2 == nullptr: the source is already in mu^k (k_bwd2), only the gradient
+// share is added.
 __global__ __launch_bounds__(256) void k_bwd_post(
     const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ um,
     float* __restrict__ uc, float* __restrict__ m2, float* __restrict__ gpart,
@@ -521,7 +744,7 @@ __global__ __launch_bounds__(256) void k_bwd_post(
     const int x = i - z * g.nx;
     const float A = 1.0f / (k.idt2 + d[i] * k.idt);
     const float as = A * rgrad[t];
-    atomicAdd(m2 + f, as);
+    if (m2) atomicAdd(m2 + f, as);
     if (interior(z, x, g)) {
       const float Lu = lap5(um + b * g.plane, z, x, i, g, k.ih2);
       atomicAdd(gpart + (b / S) * g.plane + i, 2.0f * vp[i] * Lu * as);
@@ -779,6 +1002,7 @@ inline bool valid_shape(int S, int RZ, int VX) {
 SEIS_DEFINE_LAUNCH(launch_fwd, k_fwd)
 SEIS_DEFINE_LAUNCH(launch_bwd, k_bwd)
 SEIS_DEFINE_LAUNCH(launch_fwd2, k_fwd2)
+SEIS_DEFINE_LAUNCH(launch_bwd2, k_bwd2)
 SEIS_DEFINE_LAUNCH(launch_fwd_batch, k_fwd_batch)
 SEIS_DEFINE_LAUNCH(launch_bwd_batch, k_bwd_batch)
 
@@ -1250,8 +1474,9 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                            const float* rec_grad, const float* ring, const float* snaps,
                            int32_t snap_every, float* u_a, float* u_b,
-                           float* mu_a, float* mu_b, float* grad_part, float* grad, int64_t B,
-                           int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                           float* mu_a, float* mu_b, float* u_c, float* u_d, float* mu_c,
+                           float* mu_d, const int32_t* rec_map, float* grad_part, float* grad,
+                           int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
                            int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                            void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
@@ -1288,7 +1513,66 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   float* Y = u_a;   // u^k  -> u^{k-2}
   float* M1 = mu_a; // mu^{k+1}
   float* M2 = mu_b; // mu^{k+2} -> mu^k
-  for (int64_t kk = nt; kk >= 1; --kk) {
+  // two reverse iterations per launch (k_bwd2) where possible
+  const bool two = u_c && u_d && mu_c && mu_d && ring && (R == 0 || rec_map);
+  if (two) {
+    SEIS_CHECK_ARG(aligned(u_c, 4 * vec_x) && aligned(u_d, 4 * vec_x) &&
+                       aligned(mu_c, 4 * vec_x) && aligned(mu_d, 4 * vec_x),
+                   "field pointers must be %d-byte aligned for vec_x %d", 4 * vec_x, vec_x);
+    SEIS_CHECK_ARG(R % B == 0, "rec_map needs the same receiver count in every shot");
+  }
+  const int Rs = (int)(R / B);
+  float* XC = u_c;  // free buffers for the pair's outputs
+  float* XD = u_d;
+  float* MC = mu_c;
+  float* MD = mu_d;
+  // k_bwd2 keeps two iterations' fields of its rows in registers: one row per
+  // thread and at most kBwd2MaxVX cells per lane (vec_x divides nx, so does
+  // any smaller power of two)
+  const int rz2 = 1;
+  const int vx2 = vec_x < kBwd2MaxVX ? vec_x : kBwd2MaxVX;
+  TileMap tm2 = make_tiles(g, vx2, rz2, G);
+  tm2.ntx = (g.nx + kFwd2Out * vx2 - 1) / (kFwd2Out * vx2);
+  tm2.per = (int)(((int64_t)tm2.ntx * tm2.ntz * tm2.groups + kXcds - 1) / kXcds);
+  auto snap_at = [&](int64_t m) {
+    return snaps && m >= snap_every && m % snap_every == 0 && m <= nt - 2;
+  };
+  int64_t kk = nt;
+  while (kk >= 1) {
+    if (two && kk >= 4 && !snap_at(kk - 2)) {
+      // iterations kk and kk-1: X = u^{kk-1}, Y = u^kk, M1 = mu^{kk+1}, M2 =
+      // mu^{kk+2} -> XC = u^{kk-2}, XD = u^{kk-3}, MC = mu^kk, MD = mu^{kk-1}
+      SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD2, s);
+      if (int e = launch_bwd2(shots_per_group, rz2, vx2, (int64_t)tm2.per * kXcds, s, vp, d, X, Y, XC, XD, M1, M2, MC, MD,
+                              grad_part, src_zx, wavelet + (kk - 2), R ? rec_map : nullptr,
+                              R ? rec_grad + (kk - 1) * R : nullptr, Rs,
+                              ring + (kk - 3) * B * ring_len, ring_len, (int)B, g, k, tm2))
+        return e;
+      SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD2, s);
+      if (R > 0) {  // gradient share of iteration kk's receivers (already in mu^kk)
+        k_bwd_post<<<(unsigned)((R + 255) / 256), 256, 0, s>>>(
+            vp, d, X, nullptr, nullptr, grad_part, rec_flat, rec_grad + (kk - 1) * R, R,
+            nullptr, ring_len, 0, (int)B, S, g, k);
+        SEIS_CHECK_LAUNCH();
+      }
+      {  // iteration kk-1's receivers and the ring of u^{kk-3}
+        const int64_t work = R + B * ring_len;
+        k_bwd_post<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(
+            vp, d, XC, XD, MD, grad_part, rec_flat, R ? rec_grad + (kk - 2) * R : nullptr, R,
+            ring + (kk - 4) * B * ring_len, ring_len, 1, (int)B, S, g, k);
+        SEIS_CHECK_LAUNCH();
+      }
+      if (snap_at(kk - 3)) {  // reload the stored pair (u^{kk-3}, u^{kk-2})
+        const float* slot_p = snaps + ((kk - 3) / snap_every - 1) * 2 * B * g.plane;
+        if (int e = copy2(slot_p, XD, slot_p + B * g.plane, XC, B * g.plane, s)) return e;
+      }
+      float* t0 = X; float* t1 = Y;
+      X = XD; Y = XC; XC = t0; XD = t1;
+      t0 = M1; t1 = M2;
+      M1 = MD; M2 = MC; MC = t0; MD = t1;
+      kk -= 2;
+      continue;
+    }
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
     if (int e = launch_bwd(S, rows_per_thread, vec_x, (int64_t)tm.per * kXcds, s, vp, d, X, Y, M1,
@@ -1311,6 +1595,7 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
     }
     float* t = X; X = Y; Y = t;
     t = M1; M1 = M2; M2 = t;
+    --kk;
   }
   k_sum_groups<<<(unsigned)((g.plane + 255) / 256), 256, 0, s>>>(grad_part, G, grad, g.plane);
   SEIS_CHECK_LAUNCH();

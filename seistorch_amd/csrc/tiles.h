@@ -37,6 +37,19 @@ __device__ __forceinline__ int64_t ring_cell(int64_t j, const Grid g) {
   return (int64_t)z * g.nx + x;
 }
 
+// Ring index j of cell (z, x), the inverse of ring_cell, or -1 off the ring.
+__device__ __forceinline__ int64_t ring_index(int z, int x, const Grid g) {
+  const int nxi = g.nx - 2 * g.npml - 2;
+  const int nzi = g.nz - 2 * g.npml - 2;
+  const bool xr = x > g.npml && x < g.nx - 1 - g.npml;
+  const bool zr = z > g.npml && z < g.nz - 1 - g.npml;
+  if (z == g.npml && xr) return x - g.npml - 1;
+  if (z == g.nz - 1 - g.npml && xr) return nxi + (x - g.npml - 1);
+  if (x == g.npml && zr) return 2 * nxi + (z - g.npml - 1);
+  if (x == g.nx - 1 - g.npml && zr) return 2 * (int64_t)nxi + nzi + (z - g.npml - 1);
+  return -1;
+}
+
 // 5-point Laplacian from its taps: the h^-2-weighted taps accumulated in
 // row-major kernel order (z-1, x-1, centre, x+1, z+1) with fused multiply-adds,
 // the way a direct 3x3 convolution with weights h^-2*[[0,1,0],[1,-4,1],[0,1,0]]

@@ -173,6 +173,32 @@ def _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad):
             raise RuntimeError("boundary-saving adjoint needs d == 0 inside the 50-cell boundary")
 
 
+def shared_receiver_map(rec_flat, B, nz, nx):
+    """(ok, map): when every shot has its receivers at the same cells in the
+    same order (rec_flat[b*Rs + r] = b*nz*nx + cell_r, no cell twice), map is
+    the (nz, nx) int32 receiver index of each cell (-1 elsewhere), which the
+    two-step adjoint needs; ok is False otherwise.  R == 0 gives (True, None)."""
+    R = rec_flat.numel()
+    if R == 0:
+        return True, None
+    if R % B:
+        return False, None
+    Rs = R // B
+    plane = nz * nx
+    rf = rec_flat.view(B, Rs)
+    cells = rf[0]
+    shift = torch.arange(B, device=rec_flat.device, dtype=torch.int64)[:, None] * plane
+    if not torch.equal(rf - shift, cells.expand(B, Rs)):
+        return False, None
+    if bool((cells < 0).any()) or bool((cells >= plane).any()):
+        return False, None
+    m = torch.full((plane,), -1, dtype=torch.int32, device=rec_flat.device)
+    m[cells] = torch.arange(Rs, dtype=torch.int32, device=rec_flat.device)
+    if int((m >= 0).sum()) != Rs:  # a cell holds two receivers
+        return False, None
+    return True, m.view(nz, nx)
+
+
 class AcousticLoop(torch.autograd.Function):
     """Whole forward loop -> records (nt, R); backward = hand-written adjoint.
 
@@ -180,12 +206,15 @@ class AcousticLoop(torch.autograd.Function):
     rec_flat (R,) int64 flat indices into (B, nz, nx); B; dt; h; launch shape
     (shots_per_group, rows_per_thread, vec_x), None entries automatic;
     snapshot_every (None: automatic, 0: ring only as in the reference);
-    two_step: two time steps per forward launch (two more field buffers).
+    two_step: two time steps per forward launch (two more field buffers);
+    two_step_adjoint: two reverse steps per adjoint launch (four more, when
+    every shot records at the same receiver cells; slower than one step per
+    launch at present, kept for its bit-identical results).
     """
 
     @staticmethod
     def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shape, snapshot_every,
-                two_step=True):
+                two_step=True, two_step_adjoint=False):
         _lib.require_device(vp, d, wavelet, src_zx, rec_flat)
         nz, nx = vp.shape
         nt = wavelet.shape[0]
@@ -222,7 +251,7 @@ class AcousticLoop(torch.autograd.Function):
         if need_grad:
             ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
             ctx.state = (ring, snaps, u_a, u_b)
-            ctx.meta = (B, dt, h, S, K, rz, vx)
+            ctx.meta = (B, dt, h, S, K, rz, vx, bool(two_step_adjoint))
         return rec
 
     @staticmethod
@@ -231,7 +260,7 @@ class AcousticLoop(torch.autograd.Function):
         vp, d, wavelet, src_zx, rec_flat = ctx.saved_tensors
         ring, snaps, u_a, u_b = ctx.state
         ctx.state = None  # the backward consumes the final forward state
-        B, dt, h, S, K, rz, vx = ctx.meta
+        B, dt, h, S, K, rz, vx, two_step_adjoint = ctx.meta
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -241,25 +270,35 @@ class AcousticLoop(torch.autograd.Function):
         mu_b = torch.empty_like(u_a)
         gpart = torch.empty((G, nz, nx), device=vp.device, dtype=torch.float32)
         grad = torch.empty_like(vp)
+        extra = [None] * 5  # u_c, u_d, mu_c, mu_d, rec_map of the two-step adjoint
+        if two_step_adjoint and nt >= 4 and ring is not None:
+            ok, rmap = shared_receiver_map(rec_flat, B, nz, nx)
+            if ok:
+                extra = [torch.empty_like(u_a) for _ in range(4)] + [rmap]
         _lib.call("seis_acoustic_backward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(grad_rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b),
-                  _lib.ptr(mu_a), _lib.ptr(mu_b), _lib.ptr(gpart),
-                  _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, rz, vx, _lib.stream_of(vp))
-        return grad, None, None, None, None, None, None, None, None, None, None
+                  _lib.ptr(mu_a), _lib.ptr(mu_b), *[_lib.ptr(t) for t in extra],
+                  _lib.ptr(gpart), _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, rz, vx,
+                  _lib.stream_of(vp))
+        return grad, None, None, None, None, None, None, None, None, None, None, None
 
 
 def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
-              snapshot_every=None, rows_per_thread=None, vec_x=None, two_step=True):
+              snapshot_every=None, rows_per_thread=None, vec_x=None, two_step=True,
+              two_step_adjoint=False):
     """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint.
 
     The launch shape -- shots_per_group (1, 2, 4, 8), rows_per_thread
     (1, 2, 4), vec_x (cells per lane along x: 1, 2, 4, dividing nx) --
     defaults to the autotuned values for this shape (``seistorch_amd.autotune``),
     else to fixed defaults.  two_step runs two forward time steps per launch
-    (same results, half the forward's field traffic).
+    (same results, half the forward's field traffic).  two_step_adjoint runs
+    two reverse steps per adjoint launch (same results up to the order of the
+    gradient additions at receiver cells; at present slower than one step per
+    launch: its fields of two iterations need ~250 registers per lane).
     """
     return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
                               rec_flat.contiguous(), int(B), _f(dt), _f(h),
                               (shots_per_group, rows_per_thread, vec_x), snapshot_every,
-                              bool(two_step))
+                              bool(two_step), bool(two_step_adjoint))
