@@ -141,6 +141,16 @@ __device__ __forceinline__ bool in_deep(int z, int x, const Deep& dp) {
   return z >= dp.z0 && z < dp.z1 && x >= dp.x0 && x < dp.x1;
 }
 
+// Shot groups per frame thread: the per-cell set-up of the strips runs once
+// per kFrameGroups groups of S shots.
+constexpr int kFrameGroups = 8;
+
+// Rows per thread of the vectorised adjoint on Deep.  One: the deep and
+// frame block ranges share one kernel, whose register count (and so the
+// frame threads' occupancy) follows the larger branch; two rows per thread
+// took 146 VGPRs against 91 and made the launch 20 % slower.
+constexpr int kHabcRZ = 1;
+
 // The cells outside Deep as up to four rectangles (top, bottom, left, right),
 // tiled 64 x 4 each (one cell per thread) and numbered consecutively, so the
 // per-cell launches only cover the strips.
@@ -248,7 +258,7 @@ __global__ __launch_bounds__(256) void k_habc_fwd_deep(const float* __restrict__
                                                        const float* __restrict__ wn, int B,
                                                        HabcCtx hc, seis::TimeConsts k, Deep dp,
                                                        TileMap tm, const float* __restrict__ bw,
-                                                       Frame f, int G) {
+                                                       Frame f, int ngf, int G) {
   // blocks [0, f.per*8) run the frame's cells, one cell per thread for all G
   // shot groups (the per-cell set-up once per step), the rest Deep's lane
   // vectors, so both kinds of work share the machine within one launch
@@ -256,7 +266,8 @@ __global__ __launch_bounds__(256) void k_habc_fwd_deep(const float* __restrict__
   if ((int)blockIdx.x < nf) {
     int zc, xc, gc;
     if (frame_cell(f, (int)blockIdx.x, zc, xc, gc))
-      habc_fwd_cell<S>(zc, xc, 0, vp, bw, u, p, out, src, wn, B, hc, k, G);
+      habc_fwd_cell<S>(zc, xc, gc * ngf, vp, bw, u, p, out, src, wn, B, hc, k,
+                       min(ngf, G - gc * ngf));
     return;
   }
   int tx, tzi, grp;
@@ -427,7 +438,7 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
     const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
     float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
     const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
-    TileMap tm, Frame f, int G) {
+    TileMap tm, Frame f, int ngf, int G) {
   // blocks [0, f.per*8) run the frame's cells, one cell per thread for all G
   // shot groups (the per-cell set-up once per step, not once per group), the
   // rest Deep's lane vectors; the frame goes first, being the longer work
@@ -435,93 +446,113 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
   if ((int)blockIdx.x < nf) {
     int zc, xc, gc;
     if (frame_cell(f, (int)blockIdx.x, zc, xc, gc))
-      habc_adj_cell<S>(zc, xc, 0, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc,
-                       k, G);
+      habc_adj_cell<S>(zc, xc, gc * ngf, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B,
+                       hc, k, min(ngf, G - gc * ngf));
     return;
   }
   int tx, tzi, grp;
   if (!tile_of_w(tm, (int)blockIdx.x - nf, tx, tzi, grp)) return;
   const Grid& g = hc.g;
   const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
-  const int z = dp.z0 + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
-  if (x0 >= dp.x1 || z >= dp.z1) return;
-  // every cell of Deep and its neighbours are interior, away from the strips
-  const int i = z * g.nx + x0;
+  const int zb = dp.z0 + (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * kHabcRZ;
+  if (x0 >= dp.x1 || zb >= dp.z1) return;
+  // every cell of Deep and its neighbours are interior, away from the strips;
+  // a thread owns rows zb .. zb+kHabcRZ-1 (those below dp.z1) and reads rows
+  // zb-1 .. zb+kHabcRZ once per shot for all of them
+  constexpr int RZ = kHabcRZ;
+  const int nr = min(RZ, dp.z1 - zb);
   const float A0 = 1.0f / k.idt2;
   const float B0 = k.idt2;
-  float c[VX], qrow[VX + 2], qup[VX], qdn[VX];
-  vload<VX>(vp + i, c);
-  {
+  // q = c^2*A0 on rows zb-1 .. zb+RZ (index p = z - zb + 1) and at the
+  // vector's two x-ends of the own rows; c on the own rows
+  float qcol[RZ + 2][VX], qL[RZ], qR[RZ], c[RZ][VX];
+#pragma unroll
+  for (int p = 0; p < RZ + 2; ++p) {
+    const int z = min(zb - 1 + p, dp.z1);  // rows past dp.z1 + 1 are never used
     float t[VX];
-    vload<VX>(vp + i - g.nx, t);
+    vload<VX>(vp + z * g.nx + x0, t);
 #pragma unroll
-    for (int e = 0; e < VX; ++e) qup[e] = (t[e] * t[e]) * A0 * 1.0f;
-    vload<VX>(vp + i + g.nx, t);
+    for (int e = 0; e < VX; ++e) qcol[p][e] = (t[e] * t[e]) * A0 * 1.0f;
+  }
 #pragma unroll
-    for (int e = 0; e < VX; ++e) qdn[e] = (t[e] * t[e]) * A0 * 1.0f;
+  for (int r = 0; r < RZ; ++r) {
+    const int i = min(zb + r, dp.z1 - 1) * g.nx + x0;
+    vload<VX>(vp + i, c[r]);
     const float cl = vp[i - 1], cr = vp[i + VX];
-    qrow[0] = (cl * cl) * A0 * 1.0f;
-    qrow[VX + 1] = (cr * cr) * A0 * 1.0f;
-#pragma unroll
-    for (int e = 0; e < VX; ++e) qrow[e + 1] = (c[e] * c[e]) * A0 * 1.0f;
+    qL[r] = (cl * cl) * A0 * 1.0f;
+    qR[r] = (cr * cr) * A0 * 1.0f;
   }
   const float ca = k.two_idt2 * (A0 * 1.0f);
   const float cb = -(B0 * (A0 * 1.0f));
   const float w = wn ? *wn : 0.0f;
-  float gacc[VX];
-  vzero<VX>(gacc);
+  float gacc[RZ][VX];
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) vzero<VX>(gacc[r]);
   const int b0 = grp * S, nb = min(S, B - b0);
   for (int s = 0; s < nb; ++s) {
     const int bb = b0 + s;
     const int64_t off = (int64_t)bb * g.plane;
     const float* Y = y1 + off;
-    float yc[VX], yu[VX], yd[VX], y2c[VX], acc[VX];
-    vload<VX>(Y + i, yc);
-    vload<VX>(Y + i - g.nx, yu);
-    vload<VX>(Y + i + g.nx, yd);
-    vload<VX>(y2 + off + i, y2c);
-    const float yl = Y[i - 1], yr = Y[i + VX];
+    const float* U = um + off;
+    const int sz = src[2 * bb], sx = src[2 * bb + 1];
+    float ycol[RZ + 2][VX], ucol[RZ + 2][VX];
 #pragma unroll
-    for (int e = 0; e < VX; ++e) {
-      const float left = e == 0 ? qrow[0] * yl : qrow[e] * yc[e - 1];
-      const float right = e == VX - 1 ? qrow[VX + 1] * yr : qrow[e + 2] * yc[e + 1];
-      const float Lq = lap_taps(qup[e] * yu[e], left, qrow[e + 1] * yc[e], right, qdn[e] * yd[e],
-                                k.ih2);
-      float a = ca * yc[e] + Lq;
-      a += cb * y2c[e];
-      acc[e] = a;
+    for (int p = 0; p < RZ + 2; ++p) {
+      const int z = min(zb - 1 + p, dp.z1);
+      vload<VX>(Y + z * g.nx + x0, ycol[p]);
+      if (recon >= 0) vload<VX>(U + z * g.nx + x0, ucol[p]);
     }
-    vstore<VX>(out + off + i, acc);
-    if (recon >= 0) {
-      const float* U = um + off;
-      float uu[VX], uup[VX], udn[VX], un[VX];
-      vload<VX>(U + i, uu);
-      vload<VX>(U + i - g.nx, uup);
-      vload<VX>(U + i + g.nx, udn);
-      const float ul = U[i - 1], ur = U[i + VX];
-      if (recon == 1) vload<VX>(uc + off + i, un);
-      const int sz = src[2 * bb], sx = src[2 * bb + 1];
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      if (r >= nr) break;
+      const int z = zb + r;
+      const int i = z * g.nx + x0;
+      const int p = r + 1;
+      float y2c[VX], acc[VX];
+      vload<VX>(y2 + off + i, y2c);
+      const float yl = Y[i - 1], yr = Y[i + VX];
 #pragma unroll
       for (int e = 0; e < VX; ++e) {
-        const float Lu = lap_taps(uup[e], e == 0 ? ul : uu[e - 1], uu[e],
-                                  e == VX - 1 ? ur : uu[e + 1], udn[e], k.ih2);
-        gacc[e] += Lu * acc[e];
-        if (recon == 1) {
-          float v = un[e];
-          if (z == sz && x0 + e == sx) v -= w;
-          un[e] = ((k.two_idt2 * uu[e] + (c[e] * c[e]) * Lu) - v / A0) / B0;
-        } else if (recon == 2) {
-          un[e] = 0.0f;
-        }
+        const float left = e == 0 ? qL[r] * yl : qcol[p][e - 1] * ycol[p][e - 1];
+        const float right = e == VX - 1 ? qR[r] * yr : qcol[p][e + 1] * ycol[p][e + 1];
+        const float Lq = lap_taps(qcol[p - 1][e] * ycol[p - 1][e], left, qcol[p][e] * ycol[p][e],
+                                  right, qcol[p + 1][e] * ycol[p + 1][e], k.ih2);
+        float a = ca * ycol[p][e] + Lq;
+        a += cb * y2c[e];
+        acc[e] = a;
       }
-      if (recon != 0) vstore<VX>(uc + off + i, un);
+      vstore<VX>(out + off + i, acc);
+      if (recon >= 0) {
+        float un[VX];
+        const float ul = U[i - 1], ur = U[i + VX];
+        if (recon == 1) vload<VX>(uc + off + i, un);
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+          const float Lu = lap_taps(ucol[p - 1][e], e == 0 ? ul : ucol[p][e - 1], ucol[p][e],
+                                    e == VX - 1 ? ur : ucol[p][e + 1], ucol[p + 1][e], k.ih2);
+          gacc[r][e] += Lu * acc[e];
+          if (recon == 1) {
+            float v = un[e];
+            if (z == sz && x0 + e == sx) v -= w;
+            un[e] = ((k.two_idt2 * ucol[p][e] + (c[r][e] * c[r][e]) * Lu) - v / A0) / B0;
+          } else if (recon == 2) {
+            un[e] = 0.0f;
+          }
+        }
+        if (recon != 0) vstore<VX>(uc + off + i, un);
+      }
     }
   }
-  float gv[VX];
-  vload<VX>(gpart + (int64_t)grp * g.plane + i, gv);
 #pragma unroll
-  for (int e = 0; e < VX; ++e) gv[e] += 2.0f * c[e] * A0 * gacc[e];
-  vstore<VX>(gpart + (int64_t)grp * g.plane + i, gv);
+  for (int r = 0; r < RZ; ++r) {
+    if (r >= nr) break;
+    const int i = (zb + r) * g.nx + x0;
+    float gv[VX];
+    vload<VX>(gpart + (int64_t)grp * g.plane + i, gv);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) gv[e] += 2.0f * c[r][e] * A0 * gacc[r][e];
+    vstore<VX>(gpart + (int64_t)grp * g.plane + i, gv);
+  }
 }
 
 // Per-step model gradient (all cells): sum over shots of
@@ -773,7 +804,8 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
   const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
   const int G = (int)((B + S - 1) / S);
   const TileMap tmd = make_tiles(gd, VX, 1, G);
-  const Frame fr = make_frame(g, dp, 1);  // one frame thread runs all G groups
+  const int ngf = G < kFrameGroups ? G : kFrameGroups;  // groups per frame thread
+  const Frame fr = make_frame(g, dp, (G + ngf - 1) / ngf);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -792,7 +824,7 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
     float* out = P[(n + 1) % 3];
     SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
     SEIS_HABC_LAUNCH_V(k_habc_fwd_deep, S, VX, dim3((tmd.per + fr.per) * kXcds), s, vp, un,
-                       up, out, src_zx, wavelet + n, (int)B, hc, k, dp, tmd, b, fr, G);
+                       up, out, src_zx, wavelet + n, (int)B, hc, k, dp, tmd, b, fr, ngf, G);
     SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
@@ -836,8 +868,9 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   Deep dp{m, (int)nz - m, (m + VX - 1) / VX * VX, ((int)nx - m) / VX * VX};
   if (dp.x1 <= dp.x0 || dp.z1 <= dp.z0) dp = Deep{0, 0, 0, 0};  // no deep cells
   const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
-  const TileMap tmd = make_tiles(gd, VX, 1, G);
-  const Frame fr = make_frame(g, dp, 1);  // one frame thread runs all G groups
+  const TileMap tmd = make_tiles(gd, VX, kHabcRZ, G);
+  const int ngf = G < kFrameGroups ? G : kFrameGroups;  // groups per frame thread
+  const Frame fr = make_frame(g, dp, (G + ngf - 1) / ngf);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -857,7 +890,7 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX, dim3((tmd.per + fr.per) * kXcds), s, vp, b, L1,
                        L2, L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k,
-                       dp, tmd, fr, G);
+                       dp, tmd, fr, ngf, G);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
     if (work > 0) {
