@@ -274,8 +274,13 @@ __global__ __launch_bounds__(kBlock) void k_fwd2(
       const bool zok = z >= 0 && z < g.nz;
       float xm = __shfl_up(un[q + 1][VX - 1], 1, kLanes);
       float xp = __shfl_down(un[q + 1][0], 1, kLanes);
-      if (lane == 0) xm = (xin && zok && x0 > 0) ? f[z * g.nx + x0 - 1] : 0.0f;
-      if (lane == kLanes - 1) xp = (xin && zok && x0 + VX < g.nx) ? f[z * g.nx + x0 + VX] : 0.0f;
+      // lanes 0 and 63 pass on only their inner-side vector element, which
+      // needs the outer neighbour only when a vector is a single cell
+      if (VX == 1) {
+        if (lane == 0) xm = (xin && zok && x0 > 0) ? f[z * g.nx + x0 - 1] : 0.0f;
+        if (lane == kLanes - 1)
+          xp = (xin && zok && x0 + VX < g.nx) ? f[z * g.nx + x0 + VX] : 0.0f;
+      }
       if (xin && zok) {
         float prev[VX];
         vload<VX>(fp + z * g.nx + x0, prev);

@@ -241,10 +241,10 @@ __device__ __forceinline__ void el_step_body(const float* __restrict__ cf, float
       const int z = zb - 1 + q;
       const bool zok = z >= 0 && z < g.nz;
       // vx row z is VXr[q + 1], vz row z is VZr[q]
-      float vxl = __shfl_up(VXr[q + 1][VX - 1], 1, kLanes);
-      float vzr = __shfl_down(VZr[q][0], 1, kLanes);
-      if (lane == 0) vxl = (xin && zok && x0 > 0) ? vx[z * g.nx + x0 - 1] : 0.0f;
-      if (lane == kLanes - 1) vzr = (xin && zok && x0 + VX < g.nx) ? vz[z * g.nx + x0 + VX] : 0.0f;
+      // lane 0's stresses reach lane 1 only through its last txz, lane 63's
+      // only through its first txx: neither uses the outer neighbour
+      const float vxl = __shfl_up(VXr[q + 1][VX - 1], 1, kLanes);
+      const float vzr = __shfl_down(VZr[q][0], 1, kLanes);
       if (xin && zok) {
         const int i = z * g.nx + x0;
         float l2m[VX], lam[VX], mc[VX], a[VX], bm[VX];
