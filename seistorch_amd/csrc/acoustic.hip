@@ -621,14 +621,16 @@ __global__ __launch_bounds__(kBlock) void k_bwd2(
       float ul = __shfl_up(U1[q + 1][VX - 1], 1, kLanes);
       float ur = __shfl_down(U1[q + 1][0], 1, kLanes);
       const bool urow = z >= ur0 && z <= ur1;
-      if (lane == 0) {
+      // lanes 0 and 63 pass on only their inner-side vector element, which
+      // needs the outer neighbour only when a vector is a single cell
+      if (VX == 1 && lane == 0) {
         const bool ok = xin && zok && x0 > 0;
         const int j = z * g.nx + x0 - 1;
         const float cl = ok ? vp[j] : 0.0f;
         pl = cl * cl * (ok ? ma[off + j] : 0.0f);
         ul = (ok && urow) ? ua[off + j] : 0.0f;
       }
-      if (lane == kLanes - 1) {
+      if (VX == 1 && lane == kLanes - 1) {
         const bool ok = xin && zok && x0 + VX < g.nx;
         const int j = z * g.nx + x0 + VX;
         const float cr = ok ? vp[j] : 0.0f;
