@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--nt", type=int, default=None,
                    help="shorter time axis for profiler runs (the metric is quoted at full nt)")
     p.add_argument("--workload", choices=["acoustic", "habc", "elastic", "generator", "precond",
-                                          "filter"],
+                                          "filter", "postprocess"],
                    default="acoustic",
                    help="acoustic: 64-shot 800x2400 FWI gradient (the headline); habc: the "
                         "same with the HABC boundary; elastic: "
@@ -54,7 +54,9 @@ def parse():
                         "models, records + misfit gradient in one batch; precond: train the "
                         "U-Net preconditioner on generator pairs, then plain vs preconditioned "
                         "FWI on the 800x2400 problem; filter: zero-phase Butterworth filter "
-                        "of the 64-shot 800x2400 problem's records (multiscale FWI)")
+                        "of the 64-shot 800x2400 problem's records (multiscale FWI); "
+                        "postprocess: smoothing, seabed cut and illumination of one "
+                        "900x2500 gradient")
     p.add_argument("--models", type=int, default=256, help="generator: number of models")
     p.add_argument("--graph", type=int, default=0,
                    help="generator: 1 records each batched loop into one hipGraph (kernel "
@@ -362,6 +364,71 @@ def run_filter(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_postprocess(args, dev):
+    """Gradient post-processing between the adjoint and the update
+    (seistorch_dist.py:270-280): seabed pre-mask, 5 Gaussian smoothing
+    passes per axis (radius 10, sigma 3: 21 taps), seabed cut, division by the
+    illumination, on one 900x2500 gradient (the padded 800x2400 grid).  One
+    step = one seis_smooth_gradient call (10 kernel launches).  value = cell
+    passes per second."""
+    import torch
+    from seistorch_amd import process as pp
+    nz, nx = 900, 2500
+    sm = {"counts": 5, "sigma": {"z": 3.0, "x": 3.0}, "radius": {"z": 10, "x": 10}}
+    g = torch.Generator(device=dev).manual_seed(20230504)
+    grad = torch.randn((nz, nx), device=dev, generator=g)
+    mask = torch.ones((nz, nx), device=dev)
+    mask[:150] = 0.0
+    illum = torch.rand((nz, nx), device=dev, generator=g) + 0.1
+    passes = 2 * sm["counts"]
+    for _ in range(args.warmup):
+        pp.smooth_gradient(grad, sm, pre_mask=mask, post_mask=mask, illum=illum)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[2 * i].record()
+        y = pp.smooth_gradient(grad, sm, pre_mask=mask, post_mask=mask, illum=illum)
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    call_s = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)) \
+        / args.steps / 1e3
+    cells = nz * nx
+    # per pass: read + write the field (8 B/cell); the first pass also reads
+    # the pre-mask, the last the cut mask and the illumination (12 B/cell)
+    algo_bytes = cells * (8 * passes + 12)
+    achieved = algo_bytes / call_s
+    out = {
+        "metric": "Gcell-passes/s gradient smoothing + seabed cut + illumination",
+        "value": round(cells * passes * args.steps / el / 1e9, 3), "unit": "Gcell-passes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Gaussian gradient, uniform illumination, 150-row water layer)",
+        "config": {"workload": "gradient post-processing of one 900x2500 gradient",
+                   "counts": sm["counts"], "radius": 10, "sigma": 3.0, "taps": 21},
+        "roofline": {"bound": "hbm", "kernel": "k_gauss (10 launches per call)",
+                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "avg_launch_us": round(call_s * 1e6 / passes, 2),
+                     "launches": args.steps * passes, "bytes_per_call": algo_bytes},
+        "checksum": float(y.double().abs().sum()),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import process as opp
+        gs, ms, il = grad.cpu().numpy(), mask.cpu().numpy(), illum.cpu().numpy()
+        t1 = time.perf_counter()
+        opp.postprocess(gs, sm, pre_mask=ms, post_mask=ms, illum=il)
+        cel = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(cells * passes / cel / 1e9, 5),
+                               "unit": "Gcell-passes/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole call on the same gradient, "
+                                         f"oracle/process.postprocess (numpy FP64, one "
+                                         f"thread), {cel:.2f} s"}
+    print(json.dumps(out), flush=True)
+
+
 def run_precond(args, dev):
     """Learned preconditioner: pairs from the generator, AdamW training with
     the hand-written kernels, then plain and preconditioned FWI on the
@@ -441,6 +508,8 @@ def main():
         return run_precond(args, dev)
     if args.workload == "filter":
         return run_filter(args, dev)
+    if args.workload == "postprocess":
+        return run_postprocess(args, dev)
     elastic = args.workload == "elastic"
     if elastic:
         P = build_elastic_problem(dev, nt_override=args.nt)

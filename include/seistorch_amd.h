@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 3
+#define SEIS_ABI_VERSION 4
 
 #ifdef __cplusplus
 extern "C" {
@@ -295,6 +295,38 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
  * loss through the filter is the same call on the cotangent. */
 int seis_filtfilt(const float* x, float* y, double* work, int64_t nt, int64_t ntr,
                   const double* b, const double* a, int32_t ncoef, void* stream);
+
+/* ---------------- gradient post-processing -------------------------------- */
+/* One 1-D Gaussian pass over a 2-D field: replaces seistorch/signal.py:247-304
+ * (gaussian_filter, 2-D case).  x: (nz, nx) float32; w: host array of ks
+ * FP32 taps (the caller builds them as the reference does: ks = 2r+1 for an
+ * even radius r, 2r for an odd one, exp(-(k - ks/2)^2 / (2 sigma^2))
+ * normalised in FP32); axis 0 = z, 1 = x.  y receives
+ * y[i] = sum_k w[k] x[refl(i + k - ks/2)] along the axis (numpy 'reflect'),
+ * and has n + 2*(ks/2) - ks + 1 cells along it: one more than x when ks is
+ * even, as in the reference.  Requires ks/2 < the axis length; y must not
+ * alias x. */
+int seis_gaussian_filter(const float* x, float* y, int64_t nz, int64_t nx, int32_t axis,
+                         const float* w, int32_t ks, void* stream);
+
+/* The post-processing between the adjoint and the optimizer step, in the
+ * order of seistorch_dist.py:270-280: replaces seistorch/process.py:66-112
+ * (smooth_gradient, whose seabed pre-mask is pre_mask), :43-47
+ * (cut_gradient: post_mask) and :114-117 (precondition: illum).  Each step
+ * runs when its argument is given (non-NULL, counts > 0):
+ *   g = grad * pre_mask;
+ *   counts times: g = z pass (kz taps wz), then x pass (kx taps wx);
+ *   g = g * post_mask;
+ *   g = g / illum.
+ * grad, out, work, pre_mask, post_mask, illum: (nz, nx) float32 (work unused
+ * when counts == 0); out and work must not alias grad or each other.  kz and
+ * kx must be odd (an even size, i.e. an odd radius, lengthens the gradient by
+ * one cell per pass, which the reference's own callers cannot use) and
+ * kz/2 < nz, kx/2 < nx. */
+int seis_smooth_gradient(const float* grad, float* out, float* work, int64_t nz, int64_t nx,
+                         const float* wz, int32_t kz, const float* wx, int32_t kx, int32_t counts,
+                         const float* pre_mask, const float* post_mask, const float* illum,
+                         void* stream);
 
 /* ---------------- learned gradient preconditioner (U-Net) ----------------- */
 /* No reference counterpart: a small U-Net maps (gradient, model) to an

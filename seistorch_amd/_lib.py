@@ -64,6 +64,9 @@ SIGNATURES = {
                                              _p, _i32, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f,
                                              _f, _i32, _i32, _p]),
     "seis_filtfilt": (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _p, _i32, _p]),
+    "seis_gaussian_filter": (ctypes.c_int, [_p, _p, _i64, _i64, _i32, _p, _i32, _p]),
+    "seis_smooth_gradient": (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _i32, _p, _i32, _i32,
+                                            _p, _p, _p, _p]),
     "seis_unet_conv3x3": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _p, _p, _i32, _i32, _p, _p,
                                          _p, _p]),
     "seis_unet_flip_weights": (ctypes.c_int, [_p, _i32, _i32, _p, _p]),
@@ -84,7 +87,7 @@ SIGNATURES = {
     "seis_unet_fill": (ctypes.c_int, [_p, _i64, _f, _p]),
 }
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 BATCH_COLS = 22  # SEIS_BATCH_COLS: int64 columns per instance of the batch table
 
 _lock = threading.Lock()

@@ -223,7 +223,7 @@ def _gd_direction(g, grad_clamp=True):
 
 
 def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85,
-                   lr_decay=0.995, grad_clamp=True, log=None):
+                   lr_decay=0.995, grad_clamp=True, smooth=None, seabed=None, log=None):
     """The reference's FWI loop on one GPU (seistorch_dist.py:166-303): for
     each frequency scale (seistorch/signal.py band spec, or "all") a fresh
     optimizer with lr * scale_decay**scale (setup.py:157-183), `epochs`
@@ -233,7 +233,12 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
     (optimizer.py:8-15); "steepestdescent": v -= lr * g / max|g| with the
     2-98 % quantile clamp (optimizer.py:124-142).  The unpadded model is the
     optimised variable; the boundary strips are re-padded from its edges after
-    every update (process.py:50-63).  Returns (vp, History)."""
+    every update (process.py:50-63).  smooth (cfg['training']['smooth']:
+    counts, sigma and radius per axis) and seabed (a mask of the interior's
+    shape) switch on --grad-smooth and --grad-cut (seistorch_dist.py:270-275)
+    on the HIP post-processing kernels, applied to the interior gradient the
+    loop optimises.  Returns (vp, History)."""
+    from . import process
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     vi = p.interior(p.vp_init).clone().contiguous()
@@ -250,6 +255,10 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
             cur = slr
             for ep in range(epochs):
                 loss, g = p.gradient(p.pad(x.detach()))
+                if smooth is not None or seabed is not None:
+                    g = process.smooth_gradient(g.contiguous(), smooth,
+                                                pre_mask=seabed if smooth is not None else None,
+                                                post_mask=seabed)
                 if opt is not None:
                     opt.zero_grad()
                     x.grad = g.contiguous()

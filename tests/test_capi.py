@@ -88,3 +88,29 @@ def test_filtfilt_argument_checks():
         _lib.call("seis_filtfilt", fake, fake, fake, 10, 10, b, a, 33, None)
     with pytest.raises(RuntimeError, match="a\\[0\\]"):
         _lib.call("seis_filtfilt", fake, fake, fake, 10, 10, b, a0, 4, None)
+
+
+def test_gradient_postprocess_argument_checks():
+    """Host-side validation of the gradient smoothing entry points."""
+    import ctypes
+    from seistorch_amd import _lib
+    w3 = (ctypes.c_float * 3)(0.25, 0.5, 0.25)
+    w2 = (ctypes.c_float * 2)(0.5, 0.5)
+    f1, f2, f3 = ctypes.c_void_p(16), ctypes.c_void_p(32), ctypes.c_void_p(48)
+    with pytest.raises(RuntimeError, match="null pointer"):
+        _lib.call("seis_gaussian_filter", None, f2, 10, 10, 0, w3, 3, None)
+    with pytest.raises(RuntimeError, match="alias"):
+        _lib.call("seis_gaussian_filter", f1, f1, 10, 10, 0, w3, 3, None)
+    with pytest.raises(RuntimeError, match="axis"):
+        _lib.call("seis_gaussian_filter", f1, f2, 10, 10, 2, w3, 3, None)
+    with pytest.raises(RuntimeError, match="smaller than the axis"):
+        _lib.call("seis_gaussian_filter", f1, f2, 1, 10, 0, w3, 3, None)
+    with pytest.raises(RuntimeError, match="must be odd"):
+        _lib.call("seis_smooth_gradient", f1, f2, f3, 10, 10, w2, 2, w3, 3, 1, None, None, None,
+                  None)
+    with pytest.raises(RuntimeError, match="counts"):
+        _lib.call("seis_smooth_gradient", f1, f2, f3, 10, 10, w3, 3, w3, 3, -1, None, None, None,
+                  None)
+    with pytest.raises(RuntimeError, match="alias"):
+        _lib.call("seis_smooth_gradient", f1, f2, f2, 10, 10, w3, 3, w3, 3, 1, None, None, None,
+                  None)
