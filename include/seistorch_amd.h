@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 4
+#define SEIS_ABI_VERSION 5
 
 #ifdef __cplusplus
 extern "C" {
@@ -92,13 +92,18 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
  * 4; must divide nx, pointers vec_x*4-byte aligned) consecutive cells.
  * If u_c and u_d (two more (B, nz, nx) buffers, distinct from u_a, u_b) are
  * given, two time steps run per launch (the intermediate field stays in
- * registers), with the same results; NULL keeps one step per launch. */
+ * registers), with the same results; NULL keeps one step per launch.
+ * illum: NULL, or an (nz, nx) buffer that receives the source illumination
+ * sum_n sum_b (u_b^{n+1})^2 over the post-source fields (rnn.py:127-128,
+ * 204-205: zeroed per forward, the shots summed in order per time level and
+ * then added), the divisor of seistorch/process.py:114-117. */
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
-                          float* u_a, float* u_b, float* u_c, float* u_d, int64_t B, int64_t nz,
-                          int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
-                          int32_t rows_per_thread, int32_t vec_x, void* stream);
+                          float* illum, float* u_a, float* u_b, float* u_c, float* u_d,
+                          int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                          int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
+                          void* stream);
 
 /* Adjoint of seis_acoustic_forward (replaces the boundary-saving backward of
  * seistorch/checkpoint_new.py:146-217 and acoustic.py:88-137, with the exact

@@ -348,6 +348,28 @@ __global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
   }
 }
 
+// Source illumination of one time level (rnn.py:204-205): illum += sum over
+// shots of u^2, the shots summed in order into one FP32 value per cell, then
+// added, as torch.sum(u**2, 0) followed by +=.
+__global__ __launch_bounds__(256) void k_illum(const float* __restrict__ u, int B, int64_t plane,
+                                               float* __restrict__ illum) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= plane) return;
+  float acc = 0.0f;
+  for (int b = 0; b < B; ++b) {
+    const float v = u[b * plane + c];
+    acc = acc + v * v;
+  }
+  illum[c] = illum[c] + acc;
+}
+
+inline int illuminate(const float* u, int B, int64_t plane, float* illum, hipStream_t s) {
+  if (!illum) return 0;
+  k_illum<<<(unsigned)((plane + 255) / 256), 256, 0, s>>>(u, B, plane, illum);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
 // One reverse iteration k (see file header): mu^k over the whole grid into
 // m2 (was mu^{k+2}); on the interior the gradient term 2c*L(u^{k-1})*mu^k and
 // the rebuilt u^{k-2} = (2/dt^2 u^{k-1} + c^2 L u^{k-1} - (u^k - w[k-1]*src)/A)/B into
@@ -1043,7 +1065,8 @@ int check_grid(int64_t B, int64_t nz, int64_t nx, float dt, float h) {
 // in the one-step loop; u^{nt} and u^{nt-1} end in u_a and u_b.
 int forward_two_step(const float* vp, const float* d, const float* wavelet, int64_t nt,
                      const int32_t* src_zx, const int64_t* rec_flat, int64_t R, float* rec_out,
-                     float* ring, float* snaps, int32_t snap_every, float* u_a, float* u_b,
+                     float* ring, float* snaps, int32_t snap_every, float* illum, float* u_a,
+                     float* u_b,
                      float* u_c, float* u_d, int B, const Grid& g, const seis::TimeConsts& k,
                      int S, int RZ, int VX, const TileMap& tm, int64_t ring_len, int64_t aux,
                      hipStream_t s) {
@@ -1072,6 +1095,7 @@ int forward_two_step(const float* vp, const float* d, const float* wavelet, int6
   tm2.ntx = (g.nx + kFwd2Out * VX - 1) / (kFwd2Out * VX);
   tm2.per = (int)(((int64_t)tm2.ntx * tm2.ntz * tm2.groups + kXcds - 1) / kXcds);
   auto record = [&](const float* u, int64_t n) -> int {  // u = u^{n+1}
+    if (int e = illuminate(u, B, g.plane, illum, s)) return e;
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
     if (work > 0 && aux > 0) {
@@ -1411,9 +1435,10 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
-                          float* u_a, float* u_b, float* u_c, float* u_d, int64_t B, int64_t nz,
-                          int64_t nx, float dt, float h, int32_t npml, int32_t shots_per_group,
-                          int32_t rows_per_thread, int32_t vec_x, void* stream) {
+                          float* illum, float* u_a, float* u_b, float* u_c, float* u_d,
+                          int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                          int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
+                          void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
   SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, vec_x),
                  "launch shape (shots_per_group %d, rows_per_thread %d, vec_x %d) not in "
@@ -1441,13 +1466,18 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   // holds u^{n-1}, so u^{n} lives in the step-0 "current" buffer iff n is even.
   const TileMap tm = make_tiles(g, vec_x, rows_per_thread, G);
   const int64_t aux = R + ((ring && nt > 1) ? B * ring_len : 0);
+  if (illum) {
+    SEIS_CHECK_ARG(illum != u_a && illum != u_b && illum != u_c && illum != u_d,
+                   "illum must not alias a field buffer");
+    if (int e = fill(illum, g.plane, 0.0f, s)) return e;
+  }
   if (u_c && u_d) {
     SEIS_CHECK_ARG(aligned(u_c, 4 * vec_x) && aligned(u_d, 4 * vec_x),
                    "u_c and u_d must be %d-byte aligned for vec_x %d", 4 * vec_x, vec_x);
     SEIS_CHECK_ARG(u_c != u_a && u_c != u_b && u_d != u_a && u_d != u_b && u_c != u_d,
                    "the four field buffers must be distinct");
     return forward_two_step(vp, d, wavelet, nt, src_zx, rec_flat, R, rec_out, ring, snaps,
-                            snap_every, u_a, u_b, u_c, u_d, (int)B, g, k, S, rows_per_thread,
+                            snap_every, illum, u_a, u_b, u_c, u_d, (int)B, g, k, S, rows_per_thread,
                             vec_x, tm, ring_len, aux, s);
   }
   float* cur = (nt % 2 == 0) ? u_a : u_b;
@@ -1460,6 +1490,7 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
                            prev, src_zx, wavelet + n, (int)B, g, k, tm))
       return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
+    if (int e = illuminate(prev, (int)B, g.plane, illum, s)) return e;
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
     if (work > 0 && aux > 0) {

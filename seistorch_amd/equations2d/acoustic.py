@@ -209,14 +209,19 @@ class AcousticLoop(torch.autograd.Function):
     two_step: two time steps per forward launch (two more field buffers);
     two_step_adjoint: two reverse steps per adjoint launch (four more, when
     every shot records at the same receiver cells; slower than one step per
-    launch at present, kept for its bit-identical results).
+    launch at present, kept for its bit-identical results); illum: None, or
+    an (nz, nx) float32 tensor that receives the source illumination (the sum
+    over steps and shots of the squared post-source field, rnn.py:204-205).
     """
 
     @staticmethod
     def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shape, snapshot_every,
-                two_step=True, two_step_adjoint=False):
-        _lib.require_device(vp, d, wavelet, src_zx, rec_flat)
+                two_step=True, two_step_adjoint=False, illum=None):
+        _lib.require_device(vp, d, wavelet, src_zx, rec_flat, illum)
         nz, nx = vp.shape
+        if illum is not None and (illum.shape != (nz, nx) or illum.dtype != torch.float32
+                                  or not illum.is_contiguous()):
+            raise RuntimeError(f"illum must be a contiguous float32 ({nz}, {nx}) tensor")
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
         S, rz, vx = _launch_params(B, nz, nx, shape)
@@ -245,8 +250,9 @@ class AcousticLoop(torch.autograd.Function):
             u_d = torch.empty_like(u_a)
         _lib.call("seis_acoustic_forward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
-                  _lib.ptr(snaps), K, _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(u_c), _lib.ptr(u_d),
-                  B, nz, nx, dt, h, NPML, S, rz, vx, _lib.stream_of(vp))
+                  _lib.ptr(snaps), K, _lib.ptr(illum), _lib.ptr(u_a), _lib.ptr(u_b),
+                  _lib.ptr(u_c), _lib.ptr(u_d), B, nz, nx, dt, h, NPML, S, rz, vx,
+                  _lib.stream_of(vp))
         del u_c, u_d  # scratch of the two-step launches
         if need_grad:
             ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
@@ -281,12 +287,12 @@ class AcousticLoop(torch.autograd.Function):
                   _lib.ptr(mu_a), _lib.ptr(mu_b), *[_lib.ptr(t) for t in extra],
                   _lib.ptr(gpart), _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, rz, vx,
                   _lib.stream_of(vp))
-        return grad, None, None, None, None, None, None, None, None, None, None, None
+        return grad, None, None, None, None, None, None, None, None, None, None, None, None
 
 
 def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
               snapshot_every=None, rows_per_thread=None, vec_x=None, two_step=True,
-              two_step_adjoint=False):
+              two_step_adjoint=False, illum=None):
     """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint.
 
     The launch shape -- shots_per_group (1, 2, 4, 8), rows_per_thread
@@ -297,8 +303,11 @@ def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
     two reverse steps per adjoint launch (same results up to the order of the
     gradient additions at receiver cells; at present slower than one step per
     launch: its fields of two iterations need ~250 registers per lane).
+    illum: None, or an (nz, nx) float32 tensor filled with the source
+    illumination of this forward (rnn.py:127-128, 204-205), the divisor of
+    ``process.PostProcess.precondition``.
     """
     return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
                               rec_flat.contiguous(), int(B), _f(dt), _f(h),
                               (shots_per_group, rows_per_thread, vec_x), snapshot_every,
-                              bool(two_step), bool(two_step_adjoint))
+                              bool(two_step), bool(two_step_adjoint), illum)

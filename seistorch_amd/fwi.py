@@ -83,9 +83,12 @@ class Problem:
         with torch.no_grad():
             return float(torch.sum((self._filter(self.records(vp)) - self.observed()) ** 2))
 
-    def gradient(self, vp):
+    def gradient(self, vp, illum=None):
+        """(misfit, interior gradient); illum: optional (nz, nx) buffer that
+        receives the forward's source illumination."""
         v = vp.detach().clone().requires_grad_(True)
-        rec = ac.propagate(v, self.d, self.wavelet, self.src, self.rec, self.B, self.dt, self.h)
+        rec = ac.propagate(v, self.d, self.wavelet, self.src, self.rec, self.B, self.dt, self.h,
+                           illum=illum)
         loss = torch.sum((self._filter(rec) - self.observed()) ** 2)
         loss.backward()
         g = self.interior(v.grad).clone()
@@ -223,7 +226,8 @@ def _gd_direction(g, grad_clamp=True):
 
 
 def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85,
-                   lr_decay=0.995, grad_clamp=True, smooth=None, seabed=None, log=None):
+                   lr_decay=0.995, grad_clamp=True, smooth=None, seabed=None,
+                   source_illumination=False, log=None):
     """The reference's FWI loop on one GPU (seistorch_dist.py:166-303): for
     each frequency scale (seistorch/signal.py band spec, or "all") a fresh
     optimizer with lr * scale_decay**scale (setup.py:157-183), `epochs`
@@ -235,9 +239,11 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
     optimised variable; the boundary strips are re-padded from its edges after
     every update (process.py:50-63).  smooth (cfg['training']['smooth']:
     counts, sigma and radius per axis) and seabed (a mask of the interior's
-    shape) switch on --grad-smooth and --grad-cut (seistorch_dist.py:270-275)
-    on the HIP post-processing kernels, applied to the interior gradient the
-    loop optimises.  Returns (vp, History)."""
+    shape) switch on --grad-smooth and --grad-cut (seistorch_dist.py:270-275),
+    source_illumination --source-illumination (:279-280: the gradient divided
+    by the forward's illumination), on the HIP post-processing kernels,
+    applied to the interior gradient the loop optimises.  Returns
+    (vp, History)."""
     from . import process
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -254,11 +260,13 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
             sched = torch.optim.lr_scheduler.ExponentialLR(opt, lr_decay) if opt else None
             cur = slr
             for ep in range(epochs):
-                loss, g = p.gradient(p.pad(x.detach()))
-                if smooth is not None or seabed is not None:
-                    g = process.smooth_gradient(g.contiguous(), smooth,
-                                                pre_mask=seabed if smooth is not None else None,
-                                                post_mask=seabed)
+                illum = torch.empty_like(p.vp_true) if source_illumination else None
+                loss, g = p.gradient(p.pad(x.detach()), illum=illum)
+                if smooth is not None or seabed is not None or illum is not None:
+                    g = process.smooth_gradient(
+                        g.contiguous(), smooth, pre_mask=seabed if smooth is not None else None,
+                        post_mask=seabed,
+                        illum=p.interior(illum).contiguous() if illum is not None else None)
                 if opt is not None:
                     opt.zero_grad()
                     x.grad = g.contiguous()

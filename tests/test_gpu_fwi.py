@@ -67,9 +67,9 @@ def test_multiscale_loop_lowers_each_band_misfit(optimizer):
 
 
 def test_multiscale_loop_with_smoothing_and_cut():
-    """--grad-smooth and --grad-cut in the loop: the misfit still drops, and
-    the model stays at its start above the seabed (the cut zeroes the update
-    there)."""
+    """--grad-smooth, --grad-cut and --source-illumination in the loop: the
+    misfit still drops, and the model stays at its start above the seabed (the
+    cut zeroes the update there)."""
     import torch
     from seistorch_amd import fwi
     p = _small_problem()
@@ -78,6 +78,6 @@ def test_multiscale_loop_with_smoothing_and_cut():
     seabed[:8] = 0.0
     sm = {"counts": 2, "sigma": {"z": 1.0, "x": 1.0}, "radius": {"z": 2, "x": 2}}
     vp, hist = fwi.run_multiscale(p, ["all"], epochs=4, lr=10.0, smooth=sm, seabed=seabed,
-                                  log=print)
+                                  source_illumination=True, log=print)
     assert hist.misfit[-1] < hist.misfit[0]
     assert torch.equal(p.interior(vp)[:8], vi0[:8])

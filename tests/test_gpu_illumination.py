@@ -1,0 +1,73 @@
+"""Source illumination accumulated by the fused acoustic forward
+(rnn.py:127-128, 204-205: zeroed per forward, then per time level the sum
+over shots of the squared post-source field is added), the divisor of
+process.py:114-117."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import acoustic as oa
+from tests.cases import acoustic_case, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _per_step_illumination(c, t):
+    """The reference's loop on the drop-in per-step HIP function, with the
+    illumination summed in FP32 torch ops in the reference's order."""
+    from seistorch_amd.equations2d import acoustic as ac
+    B = c.B
+    smask = torch.zeros((B, c.nz, c.nx), device="cuda")
+    smask[torch.arange(B), t["src"][:, 0].long(), t["src"][:, 1].long()] = 1.0
+    h1 = torch.zeros_like(smask)
+    h2 = torch.zeros_like(smask)
+    P = torch.zeros((c.nz, c.nx), device="cuda")
+    for n in range(c.nt):
+        h1, h2 = ac._time_step(t["vp"], h1, h2, c.dt, c.h, t["d"])
+        h1 = h1 + smask * t["w"][n]
+        s = torch.zeros_like(P)
+        for b in range(B):
+            s = s + h1[b] * h1[b]
+        P = P + s
+    return P
+
+
+@pytest.mark.parametrize("nt,two_step,S", [(200, True, None), (201, True, None),
+                                           (201, False, None), (200, True, 1), (7, True, 4)])
+def test_illumination_matches_the_reference_loop(nt, two_step, S):
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case(nshots=3, nt=nt)
+    t = to_dev(c)
+    illum = torch.full((c.nz, c.nx), 7.0, device="cuda")  # overwritten, not accumulated
+    rec = ac.propagate(t["vp"], t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h, S,
+                       two_step=two_step, illum=illum)
+    rec0 = ac.propagate(t["vp"], t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h, S,
+                        two_step=two_step)
+    assert torch.equal(rec, rec0)
+    ref32 = _per_step_illumination(c, t)
+    e32 = rel_l2(illum.cpu().numpy(), ref32.cpu().numpy())
+    bitwise = torch.equal(illum, ref32)
+    _, fields = oa.forward(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec, keep_fields=True)
+    ref64 = sum((u ** 2).sum(0) for u in fields[1:])
+    e64 = rel_l2(illum.cpu().numpy(), ref64)
+    print(f"illumination nt={nt} two_step={two_step} S={S}: vs FP32 reference loop {e32:.2e} "
+          f"(bitwise {bitwise}), vs FP64 oracle {e64:.2e}")
+    assert e32 < 1e-6 and e64 < 1e-5
+
+
+def test_illumination_preconditioned_gradient():
+    """propagate(illum=) -> gradient -> PostProcess.precondition: the
+    preconditioned gradient is the plain one divided by the illumination."""
+    from seistorch_amd import process as pp
+    from seistorch_amd.equations2d import acoustic as ac
+    c = acoustic_case(nshots=2, nt=200)
+    t = to_dev(c)
+    illum = torch.empty((c.nz, c.nx), device="cuda")
+    vp = torch.nn.Parameter(t["vp"].clone())
+    rec = ac.propagate(vp, t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h, illum=illum)
+    (rec ** 2).sum().backward()
+    g0 = vp.grad.clone()
+    post = pp.PostProcess(pp.Params([vp], precondition=illum), {}, type("C", (), {})())
+    post.precondition()
+    # cells the wavefield never reached hold 0 / 0, as in the reference
+    assert torch.allclose(vp.grad, g0 / illum, rtol=0, atol=0, equal_nan=True)
