@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--nt", type=int, default=None,
                    help="shorter time axis for profiler runs (the metric is quoted at full nt)")
     p.add_argument("--workload", choices=["acoustic", "habc", "elastic", "generator", "precond",
-                                          "filter", "postprocess"],
+                                          "filter", "postprocess", "rho"],
                    default="acoustic",
                    help="acoustic: 64-shot 800x2400 FWI gradient (the headline); habc: the "
                         "same with the HABC boundary; elastic: "
@@ -56,7 +56,8 @@ def parse():
                         "FWI on the 800x2400 problem; filter: zero-phase Butterworth filter "
                         "of the 64-shot 800x2400 problem's records (multiscale FWI); "
                         "postprocess: smoothing, seabed cut and illumination of one "
-                        "900x2500 gradient")
+                        "900x2500 gradient; rho: variable-density HABC forward records of "
+                        "the 64-shot 800x2400 problem")
     p.add_argument("--models", type=int, default=256, help="generator: number of models")
     p.add_argument("--graph", type=int, default=0,
                    help="generator: 1 records each batched loop into one hipGraph (kernel "
@@ -429,6 +430,66 @@ def run_postprocess(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_rho(args, dev):
+    """Variable-density acoustic equation with the HABC boundary
+    (acoustic_rho_habc.py): records of the 64-shot 800x2400 problem (padded
+    900x2500) over a density model rising with depth, 1000 time steps (the
+    per-step rate does not depend on nt).  One step = one forward loop.
+    value = cell-updates per second."""
+    import torch
+    from seistorch_amd.equations2d import acoustic_rho_habc as ar
+    from seistorch_amd.habc import generate_habc_coefficients_2d
+    P = build_problem(dev, nt_override=args.nt or 1000)
+    nz, nx, B, nt = P["nz"], P["nx"], P["B"], P["nt"]
+    z = torch.arange(nz, device=dev, dtype=torch.float32)[:, None]
+    rho = (1000.0 + 1.2 * z).expand(nz, nx).contiguous()
+    b = generate_habc_coefficients_2d((nz, nx), 50).to(dev)
+
+    def fwd():
+        return ar.propagate(P["vp_true"], rho, b, P["w"], P["src"], P["rec"], B, P["dt"], P["h"])
+
+    for _ in range(args.warmup):
+        fwd()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[2 * i].record()
+        rec = fwd()
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    loop_s = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)) \
+        / args.steps / 1e3
+    cells = B * nz * nx * nt
+    # per cell-update: read u^n and u^{n-1}, write u^{n+1} (12 B); the model,
+    # density and strip weights are shared by the shots of a thread
+    bytes_per_cell = 12.0
+    step_s = loop_s / nt
+    achieved = bytes_per_cell * B * nz * nx / step_s
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("k_rho_habc_fwd")
+    out = {
+        "metric": "Gcell-updates/s variable-density HABC forward",
+        "value": round(cells * args.steps / el / 1e9, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded anomaly model, density 1000 + 1.2 kg/m^3 per cell of depth)",
+        "config": {"workload": "variable-density acoustic HABC forward records",
+                   "grid": [P["nz_i"], P["nx_i"]], "padded_grid": [nz, nx], "shots": B, "nt": nt},
+        "roofline": {"bound": "hbm", "kernel": "k_rho_habc_fwd",
+                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "avg_launch_us": round(step_s * 1e6, 2), "launches": args.steps * nt,
+                     "bytes_per_cell_update": bytes_per_cell},
+        "checksum": float(rec.double().abs().sum()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_precond(args, dev):
     """Learned preconditioner: pairs from the generator, AdamW training with
     the hand-written kernels, then plain and preconditioned FWI on the
@@ -510,6 +571,8 @@ def main():
         return run_filter(args, dev)
     if args.workload == "postprocess":
         return run_postprocess(args, dev)
+    if args.workload == "rho":
+        return run_rho(args, dev)
     elastic = args.workload == "elastic"
     if elastic:
         P = build_elastic_problem(dev, nt_override=args.nt)

@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 5
+#define SEIS_ABI_VERSION 6
 
 #ifdef __cplusplus
 extern "C" {
@@ -212,6 +212,32 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
                        float* l_c, float* grad_part, float* grad, int64_t B, int64_t nz,
                        int64_t nx, float dt, float h, int32_t npml, int32_t multiple,
                        int32_t shots_per_group, void* stream);
+
+/* ---------------- variable-density acoustic equation (HABC) --------------- */
+/* One step of seistorch/equations2d/acoustic_rho_habc.py:31-57 (blend = 1:
+ * _time_step, the HABC strips of b applied) or of its reverse step :59-89
+ * before the ring restore and the source (blend = 0, b may be NULL):
+ * y = 2 h1 - h2 + dt^2 h^-2 rho vp^2 (sum over x and z of
+ * b_pos (h1[+1] - h1) - b_neg (h1 - h1[-1])), b = 1/rho averaged over the
+ * two cells of each face, zero outside the grid.  vp, rho, b: (nz, nx);
+ * h1, h2, y: (B, nz, nx).  dt and h are doubles: the constant dt^2 h^-2 is
+ * formed in double, as from the reference's configuration scalars. */
+int seis_rho_habc_step(const float* vp, const float* rho, const float* b, const float* h1,
+                       const float* h2, float* y, int64_t B, int64_t nz, int64_t nx, double dt,
+                       double h, int32_t multiple, int32_t blend, void* stream);
+
+/* Whole forward loop of the variable-density equation (rnn.py:178-212 with
+ * acoustic_rho_habc.py:31-57): nt steps from zero fields, wavelet[n] added at
+ * src_zx[b] after step n, receivers rec_flat[r] recorded into
+ * rec_out[n*R + r]; u_a, u_b, u_c: (B, nz, nx) rotating buffers (u^{nt} ends
+ * in u_a, u^{nt-1} in u_b); shots_per_group >= 1 shots share one thread
+ * (the per-cell set-up runs once for them).  Forward only: the gradient with
+ * respect to vp and rho has no HIP adjoint yet. */
+int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
+                          const float* wavelet, int64_t nt, const int32_t* src_zx,
+                          const int64_t* rec_flat, int64_t R, float* rec_out, float* u_a,
+                          float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx, double dt,
+                          double h, int32_t multiple, int32_t shots_per_group, void* stream);
 
 /* ---------------- elastic, isotropic velocity-stress, PML ---------------- */
 /* Fields are passed as an array of five (B, nz, nx) pointers in the order

@@ -285,6 +285,12 @@ def habc_step(c, h1, h2, dt, h, b, w=50, multiple=False):
     """FP32 restatement of acoustic_habc._time_step (:206-221) with habc() (:147-204)."""
     y = torch.mul((dt ** -2).pow(-1), (2 / dt ** 2 * h1 - torch.mul((dt ** -2), h2)
                                        + torch.mul(c.pow(2), laplacian(h1, h))))
+    return habc_blend(y, h1, h2, c, dt, h, b, w, multiple), h1
+
+
+def habc_blend(y, h1, h2, c, dt, h, b, w=50, multiple=False):
+    """habc() of acoustic_habc.py:147-204: the strips of y replaced by the
+    weighted one-way extrapolation of (h1, h2), corners averaged."""
     Bn, nz, nx = y.shape
     fields = [y, h1, h2, c.unsqueeze(0).expand(Bn, nz, nx), b.unsqueeze(0).expand(Bn, nz, nx)]
     # top / bottom strips: rows from the edge inwards
@@ -311,7 +317,66 @@ def habc_step(c, h1, h2, dt, h, b, w=50, multiple=False):
         y[:, d, nx - w + d] = 0.5 * y_top[:, d, nx - w + d] + 0.5 * y_right[:, d, d]
     y[:, nz - w + d, d] = 0.5 * y_bottom[:, d, d] + 0.5 * y_left[:, nz - w + d, d]
     y[:, nz - w + d, nx - w + d] = 0.5 * y_bottom[:, d, nx - w + d] + 0.5 * y_right[:, nz - w + d, d]
-    return y, h1
+    return y
+
+
+def _shift(p, dz, dx):
+    """p at (z + dz, x + dx), zero outside the grid (the zero padding of the
+    reference's conv2d with padding 1)."""
+    nz, nx = p.shape[-2:]
+    out = torch.zeros_like(p)
+    zs, zd = (slice(dz, nz), slice(0, nz - dz)) if dz >= 0 else (slice(0, nz + dz), slice(-dz, nz))
+    xs, xd = (slice(dx, nx), slice(0, nx - dx)) if dx >= 0 else (slice(0, nx + dx), slice(-dx, nx))
+    out[..., zd, xd] = p[..., zs, xs]
+    return out
+
+
+def rho_operator(rho, p):
+    """The bracket of acoustic_rho_habc.py:37-53: sum over x and z of
+    b_pos * (p[+1] - p) - b_neg * (p - p[-1]), with b = 1/rho averaged over
+    the two cells of each face (Geiger and Daley 2003, eq. 9), zero outside
+    the grid for both p and b."""
+    bu = rho.pow(-1)
+    pxp = _shift(p, 0, 1) - p
+    pxn = p - _shift(p, 0, -1)
+    pzp = _shift(p, 1, 0) - p
+    pzn = p - _shift(p, -1, 0)
+    bxp = (bu + _shift(bu, 0, 1)) / 2.0
+    bxn = (_shift(bu, 0, -1) + bu) / 2.0
+    bzp = (bu + _shift(bu, 1, 0)) / 2.0
+    bzn = (_shift(bu, -1, 0) + bu) / 2.0
+    return bxp * pxp - bxn * pxn + bzp * pzp - bzn * pzn
+
+
+def rho_habc_step(c, rho, h1, h2, dt, h, b, w=50, multiple=False):
+    """acoustic_rho_habc._time_step (:31-57): p' = 2 p1 - p2 +
+    dt^2 h^-2 rho c^2 * bracket (dt, h Python floats: the constant is formed
+    in double, as the reference's configuration scalars are), then habc()."""
+    C = float(dt) ** 2 * float(h) ** (-2)
+    y = 2 * h1 - h2 + C * rho * c ** 2 * rho_operator(rho, h1)
+    # the strips take dt and h as tensors of the field's precision, as
+    # habc_step does (the one-way coefficients square dt in that precision)
+    dt_t = torch.tensor(float(dt), dtype=c.dtype)
+    h_t = torch.tensor(float(h), dtype=c.dtype)
+    return habc_blend(y, h1, h2, c, dt_t, h_t, b, w, multiple), h1
+
+
+def rho_habc_forward(c, rho, b, dt, h, wavelet, src, rec, dtype=torch.float32, multiple=False):
+    """Records of the variable-density HABC loop (rnn.py:178-212), list of
+    (nt, nrec_b, 1)."""
+    c, b, _, _, w, smask, (bidx, rz, rx), counts = _setup(c, b, dt, h, wavelet, src, rec, dtype)
+    rho = torch.as_tensor(rho).to(dtype)
+    h1 = torch.zeros_like(smask)
+    h2 = torch.zeros_like(smask)
+    recs = []
+    for n in range(w.shape[0]):
+        h1, h2 = rho_habc_step(c, rho, h1, h2, dt, h, b, multiple=multiple)
+        y = h1.clone()
+        y += smask * w[n]
+        h1 = y
+        recs.append(h1[bidx, rz, rx])
+    stacked = torch.stack(recs, 0).unsqueeze(2)
+    return list(torch.split(stacked, counts, dim=1))
 
 
 def habc_forward(c, b, dt, h, wavelet, src, rec, dtype=torch.float32, c_param=None,

@@ -19,6 +19,8 @@
 // reaches it.  The forward uses three rotating field buffers (the blend reads
 // u^{n-1} one cell inward, so it cannot overwrite it in place); the adjoint
 // three rotating cotangent buffers for the same reason.
+#include <cmath>
+
 #include "common.h"
 #include "seistorch_amd.h"
 #include "tiles.h"
@@ -242,6 +244,81 @@ __global__ __launch_bounds__(256) void k_habc_fwd(const float* __restrict__ vp,
   const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
   if (x >= g.nx || z >= g.nz) return;
   habc_fwd_cell<S>(z, x, grp, vp, bw, u, p, out, src, wn, B, hc, k);
+}
+
+// Variable-density acoustic step with the HABC strips
+// (acoustic_rho_habc.py:31-57, Geiger and Daley 2003): with b = 1/rho and
+// the face averages b_pos(x) = (b[x] + b[x+1]) / 2, b_neg(x) = (b[x-1] +
+// b[x]) / 2 (per axis, b and u zero outside the grid),
+//   y = (2u - p) + ((C * rho) * c^2) * (((bxp (u[x+1] - u) - bxn (u - u[x-1]))
+//                                        + bzp (u[z+1] - u)) - bzn (u - u[z-1]))
+// with C = dt^2 h^-2 formed in double on the host and rounded to FP32 (the
+// reference's scalars are Python floats), then the same strip blend as
+// habc_fwd_cell (blend = 0 skips it: the reverse step of
+// acoustic_rho_habc.py:59-89).  One cell per thread for spt shots (a runtime
+// count): the face averages, the strip set-up and their divisions are formed
+// once per cell for all of them.
+__device__ __forceinline__ float buoyancy(const float* __restrict__ rho, int z, int x,
+                                          const Grid& g) {
+  return (z >= 0 && z < g.nz && x >= 0 && x < g.nx) ? 1.0f / rho[z * g.nx + x] : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_rho_habc_fwd(
+    const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ bw,
+    const float* __restrict__ u, const float* __restrict__ p, float* __restrict__ out,
+    const int32_t* __restrict__ src, const float* __restrict__ wn, int B, int spt, HabcCtx hc,
+    float C, int blend, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x >= g.nx || z >= g.nz) return;
+  const int i = z * g.nx + x;
+  const float c = vp[i];
+  const float bc = 1.0f / rho[i];
+  const float bxp = (bc + buoyancy(rho, z, x + 1, g)) / 2.0f;
+  const float bxn = (buoyancy(rho, z, x - 1, g) + bc) / 2.0f;
+  const float bzp = (bc + buoyancy(rho, z + 1, x, g)) / 2.0f;
+  const float bzn = (buoyancy(rho, z - 1, x, g) + bc) / 2.0f;
+  const float coef = (C * rho[i]) * (c * c);
+  const bool edge = blend && near_edge(z, x, g, kW);
+  const float b = edge ? bw[i] : 0.0f;
+  const OwCoef kc = ow_coef(c, hc);
+  const bool mt = m_top(z, x, hc), mb = m_bottom(z, x, hc), ml = m_left(z, x, hc),
+             mr = m_right(z, x, hc);
+  int d1 = -1, d2 = -1;
+  if (!hc.multiple && z < kW && x == z) { d1 = 0; d2 = 2; }
+  if (!hc.multiple && z < kW && x == g.nx - kW + z) { d1 = 0; d2 = 3; }
+  if (z >= g.nz - kW && x == z - (g.nz - kW)) { d1 = 1; d2 = 2; }
+  if (z >= g.nz - kW && x == g.nx - kW + (z - (g.nz - kW))) { d1 = 1; d2 = 3; }
+  const float w = src ? *wn : 0.0f;
+  const int b0 = grp * spt, nb = min(spt, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int bb = b0 + s;
+    const int64_t off = (int64_t)bb * g.plane;
+    const float* U = u + off;
+    const float* P = p + off;
+    const float uc = U[i];
+    const float pxp = (x + 1 < g.nx ? U[i + 1] : 0.0f) - uc;
+    const float pxn = uc - (x > 0 ? U[i - 1] : 0.0f);
+    const float pzp = (z + 1 < g.nz ? U[i + g.nx] : 0.0f) - uc;
+    const float pzn = uc - (z > 0 ? U[i - g.nx] : 0.0f);
+    const float br = ((bxp * pxp - bxn * pxn) + bzp * pzp) - bzn * pzn;
+    const float y = (2.0f * uc - P[i]) + coef * br;
+    float v = y;
+    if (edge) {
+      if (mt) v = strip_value(0, z, x, i, U, P, kc, b, y, g);
+      if (mb) v = strip_value(1, z, x, i, U, P, kc, b, y, g);
+      if (ml) v = strip_value(2, z, x, i, U, P, kc, b, y, g);
+      if (mr) v = strip_value(3, z, x, i, U, P, kc, b, y, g);
+      if (d1 >= 0)
+        v = 0.5f * strip_value(d1, z, x, i, U, P, kc, b, y, g) +
+            0.5f * strip_value(d2, z, x, i, U, P, kc, b, y, g);
+    }
+    if (src && z == src[2 * bb] && x == src[2 * bb + 1]) v += w;
+    out[off + i] = v;
+  }
 }
 
 // The fused loop's HABC step in one launch of two block ranges: on Deep (at
@@ -836,6 +913,68 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
     if (snaps && n >= snap_every && n % snap_every == 0 && n <= nt - 2) {
       float* sp = snaps + (n / snap_every - 1) * 2 * field;
       k_habc_copy2<<<nblk(field), 256, 0, s>>>(un, sp, out, sp + field, field);
+      SEIS_CHECK_LAUNCH();
+    }
+  }
+  return 0;
+}
+
+// shots per thread of the per-step variable-density entry point
+constexpr int kRhoShots = 16;
+
+int seis_rho_habc_step(const float* vp, const float* rho, const float* b, const float* h1,
+                       const float* h2, float* y, int64_t B, int64_t nz, int64_t nx, double dt,
+                       double h, int32_t multiple, int32_t blend, void* stream) {
+  if (int e = habc_check(B, nz, nx, (float)dt, (float)h, 1)) return e;
+  SEIS_CHECK_ARG(vp && rho && h1 && h2 && y && (b || !blend), "null pointer argument");
+  const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, (float)dt, (float)h};
+  const int spt = B < kRhoShots ? (int)B : kRhoShots;
+  const TileMap tm = make_tiles(hc.g, 1, 1, (int)((B + spt - 1) / spt));
+  const float C = (float)(dt * dt * std::pow(h, -2.0));
+  hipStream_t s = (hipStream_t)stream;
+  k_rho_habc_fwd<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, b, h1, h2, y, nullptr, nullptr,
+                                                      (int)B, spt, hc, C, blend ? 1 : 0, tm);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
+                          const float* wavelet, int64_t nt, const int32_t* src_zx,
+                          const int64_t* rec_flat, int64_t R, float* rec_out, float* u_a,
+                          float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx, double dt,
+                          double h, int32_t multiple, int32_t shots_per_group, void* stream) {
+  if (int e = habc_check(B, nz, nx, (float)dt, (float)h, 1)) return e;
+  SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1, got %d", shots_per_group);
+  SEIS_CHECK_ARG(vp && rho && b && wavelet && src_zx && u_a && u_b && u_c,
+                 "null pointer argument");
+  SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
+  SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
+  const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, (float)dt, (float)h};
+  const Grid& g = hc.g;
+  const int S = shots_per_group;
+  const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
+  const float C = (float)(dt * dt * std::pow(h, -2.0));
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t field = B * g.plane;
+  // u^m lives in P[m mod 3]; u^{nt} ends in u_a, u^{nt-1} in u_b
+  float* P[3];
+  P[nt % 3] = u_a;
+  P[(nt + 2) % 3] = u_b;
+  P[(nt + 1) % 3] = u_c;
+  for (float* q : {u_a, u_b, u_c}) {
+    k_habc_fill<<<nblk(field), 256, 0, s>>>(q, field);
+    SEIS_CHECK_LAUNCH();
+  }
+  for (int64_t n = 0; n < nt; ++n) {
+    float* un = P[n % 3];
+    float* up = P[(n + 2) % 3];
+    float* out = P[(n + 1) % 3];
+    k_rho_habc_fwd<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, b, un, up, out, src_zx,
+                                                        wavelet + n, (int)B, S, hc, C, 1, tm);
+    SEIS_CHECK_LAUNCH();
+    if (R > 0) {
+      k_habc_record<<<nblk(R), 256, 0, s>>>(out, rec_flat, R, rec_out + n * R, nullptr, 0,
+                                            (int)B, g);
       SEIS_CHECK_LAUNCH();
     }
   }

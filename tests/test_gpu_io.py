@@ -39,7 +39,8 @@ def test_files_to_records_round_trip(tmp_path):
                        torch.from_numpy(flat).cuda(), len(sources), dt, h)
     rs = sio.SeisRecord(cfg)
     rs.setup("forward")
-    for i, shot in enumerate(sio.split_records(out.cpu().numpy(), [len(r) for r in rec])):
+    shots = sio.split_records(out.cpu().numpy(), [len(r) for r in rec])
+    for i, shot in enumerate(shots):
         rs[i] = shot
     rs.write()
     back = sio.SeisRecord(cfg)
@@ -47,6 +48,9 @@ def test_files_to_records_round_trip(tmp_path):
     ref = oa.forward(vp, d, dt, h, w, src, rec)
     for i in range(3):
         assert back[i].shape == (nt, len(receivers[i][0]), 1)
+        assert np.array_equal(back[i], shots[i])  # the files hold exactly what was written
+        # the padded coordinates reach the oracle's cells: a shifted source or
+        # receiver would differ at O(1); FP32 propagation leaves ~1e-5
         e = np.linalg.norm(back[i] - ref[i]) / np.linalg.norm(ref[i])
         print(f"shot {i}: {back[i].shape} records vs FP64 oracle {e:.2e}")
-        assert e < 1e-5
+        assert e < 1e-4

@@ -1,0 +1,136 @@
+"""Parity of the HIP variable-density acoustic path (acoustic_rho_habc.py)
+with the FP32 restatement of the reference (oracle/ref_torch.rho_habc_*),
+computed in the reference's operation order, and with its FP64 evaluation."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref_torch
+from tests.cases import acoustic_case, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _rho_case(nshots=3, nt=300, multiple=False, seed=1):
+    from seistorch_amd.habc import generate_habc_coefficients_2d
+    c = acoustic_case(nz_i=60, nx_i=80, nshots=nshots, nt=nt, seed=seed)
+    if multiple:  # no top padding: drop the 50 top rows of the padded model
+        c.vp = np.ascontiguousarray(c.vp[50:])
+        c.nz = c.vp.shape[0]
+        c.src = c.src.copy()
+        c.src[:, 0] -= 50
+        c.rec = [r - np.array([50, 0]) for r in c.rec]
+    rng = np.random.default_rng(seed + 7)
+    # density: layers with a rough jump, 1000-2600 kg/m^3
+    z = np.arange(c.nz)[:, None] * np.ones((1, c.nx))
+    c.rho = (1000.0 + 8.0 * z + 300.0 * (rng.random((c.nz, c.nx)) > 0.7)).astype(np.float32)
+    c.b = generate_habc_coefficients_2d((c.nz, c.nx), 50, multiple=multiple).numpy()
+    c.multiple = multiple
+    return c
+
+
+def _masks(c):
+    from seistorch_amd.habc import bound_mask
+    return bound_mask(c.nz, c.nx, 50, "cuda", c.B, return_idx=True, multiple=c.multiple)
+
+
+@pytest.mark.parametrize("multiple", [False, True])
+@pytest.mark.parametrize("B", [2, 5])
+def test_step_matches_reference(multiple, B):
+    from seistorch_amd.equations2d import acoustic_rho_habc as ar
+    c = _rho_case(nshots=B, multiple=multiple)
+    rng = np.random.default_rng(B)
+    h1 = rng.standard_normal((B, c.nz, c.nx)).astype(np.float32)
+    h2 = rng.standard_normal((B, c.nz, c.nx)).astype(np.float32)
+    d = lambda a: torch.from_numpy(a).cuda()
+    y, _ = ar._time_step(d(c.vp), d(c.rho), d(h1), d(h2), c.dt, c.h, d(c.b), habcs=_masks(c))
+    t = lambda a, dt=torch.float32: torch.from_numpy(a).to(dt)
+    yr, _ = ref_torch.rho_habc_step(t(c.vp), t(c.rho), t(h1), t(h2), c.dt, c.h, t(c.b),
+                                    multiple=multiple)
+    y64, _ = ref_torch.rho_habc_step(*(t(a, torch.float64) for a in (c.vp, c.rho, h1, h2)), c.dt,
+                                     c.h, t(c.b, torch.float64), multiple=multiple)
+    yc = y.cpu()
+    bitwise = torch.equal(yc, yr)
+    e32 = (yc - yr).abs().max().item() / yr.abs().max().item()
+    e64 = rel_l2(yc.numpy(), y64.numpy())
+    print(f"rho step multiple={multiple} B={B}: vs FP32 reference order max {e32:.2e} "
+          f"(bitwise {bitwise}), vs FP64 {e64:.2e}")
+    assert e32 < 1e-6 and e64 < 1e-6
+
+
+def test_reverse_step_is_the_unblended_update():
+    """_time_step_backward: the update without the strips, then the ring
+    restore and the source (acoustic_rho_habc.py:59-89)."""
+    from seistorch_amd.equations2d import acoustic_rho_habc as ar
+    from seistorch_amd.equations2d.utils import save_boundaries
+    c = _rho_case(nshots=2)
+    rng = np.random.default_rng(3)
+    h1, h2, prev = (rng.standard_normal((2, c.nz, c.nx)).astype(np.float32) for _ in range(3))
+    d = lambda a: torch.from_numpy(a).cuda()
+    ring = save_boundaries(d(prev))
+
+    def src_func(y, v, sign):
+        y = y.clone()
+        y[:, 60, 70] += v
+        return y
+
+    y, _ = ar._time_step_backward(d(c.vp), d(c.rho), d(h1), d(h2), c.dt, c.h, d(c.b),
+                                  (ring, None), [["h1"], src_func, 0.25])
+    t = lambda a: torch.from_numpy(a.astype(np.float64))
+    C = c.dt ** 2 * c.h ** (-2)
+    ref = 2 * t(h1) - t(h2) + C * t(c.rho) * t(c.vp) ** 2 * ref_torch.rho_operator(t(c.rho), t(h1))
+    ref[:, 60, 70] += 0.25
+    inner = (slice(None), slice(50, -50), slice(50, -50))
+    e = rel_l2(y.cpu().numpy()[inner], ref.numpy()[inner])
+    ring_ok = torch.equal(y[:, 49, :], d(prev)[:, 49, :]) and \
+        torch.equal(y[:, :, -50], d(prev)[:, :, -50])
+    print(f"rho reverse step: interior vs FP64 {e:.2e}, ring restored {ring_ok}")
+    assert e < 1e-6 and ring_ok
+
+
+@pytest.mark.parametrize("multiple", [False, True])
+def test_fused_forward_records(multiple):
+    from seistorch_amd import synthetic
+    from seistorch_amd.equations2d import acoustic_rho_habc as ar
+    c = _rho_case(nshots=3, nt=400, multiple=multiple)
+    t = to_dev(c)
+    t["rec"] = torch.from_numpy(synthetic.rec_flat_index(c.rec, c.nz, c.nx)).cuda()
+    rec = ar.propagate(t["vp"], torch.from_numpy(c.rho).cuda(), torch.from_numpy(c.b).cuda(),
+                       t["w"], t["src"], t["rec"], c.B, c.dt, c.h, multiple=multiple)
+    r32 = torch.cat(ref_torch.rho_habc_forward(c.vp, c.rho, c.b, c.dt, c.h, c.w, c.src, c.rec,
+                                               multiple=multiple), 1)[..., 0]
+    r64 = torch.cat(ref_torch.rho_habc_forward(c.vp, c.rho, c.b, c.dt, c.h, c.w, c.src, c.rec,
+                                               dtype=torch.float64, multiple=multiple), 1)[..., 0]
+    got = rec.cpu()
+    e32, e64 = rel_l2(got.numpy(), r32.numpy()), rel_l2(got.numpy(), r64.numpy())
+    print(f"rho records multiple={multiple}: vs FP32 reference order {e32:.2e} "
+          f"(bitwise {torch.equal(got, r32)}), vs FP64 {e64:.2e}")
+    assert e32 < 1e-5 and e64 < 1e-5
+
+
+def test_constant_density_matches_the_acoustic_habc_records():
+    from seistorch_amd.equations2d import acoustic_habc as ah
+    from seistorch_amd.equations2d import acoustic_rho_habc as ar
+    c = _rho_case(nshots=3, nt=400)
+    t = to_dev(c)
+    b = torch.from_numpy(c.b).cuda()
+    rho = torch.full_like(t["vp"], 1800.0)
+    r_rho = ar.propagate(t["vp"], rho, b, t["w"], t["src"], t["rec"], c.B, c.dt, c.h)
+    r_ac = ah.propagate(t["vp"], b, t["w"], t["src"], t["rec"], c.B, c.dt, c.h)
+    e = rel_l2(r_rho.cpu().numpy(), r_ac.cpu().numpy())
+    print(f"constant density vs acoustic HABC records: {e:.2e}")
+    # two FP32 formulations of one exact update (equal in FP64 to 1e-13, see
+    # tests/test_oracle_rho.py): each carries ~5e-6 against FP64 after 400
+    # steps, so they differ by up to the sum
+    assert e < 2e-5
+
+
+def test_gradient_is_refused():
+    from seistorch_amd.equations2d import acoustic_rho_habc as ar
+    c = _rho_case(nshots=2, nt=20)
+    t = to_dev(c)
+    vp = t["vp"].clone().requires_grad_(True)
+    rec = ar.propagate(vp, torch.from_numpy(c.rho).cuda(), torch.from_numpy(c.b).cuda(), t["w"],
+                       t["src"], t["rec"], c.B, c.dt, c.h)
+    with pytest.raises(NotImplementedError):
+        rec.sum().backward()
