@@ -50,8 +50,8 @@ SIGNATURES = {
     "seis_rho_habc_step": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i64, _i64,
                                           ctypes.c_double, ctypes.c_double, _i32, _i32, _p]),
     "seis_rho_habc_forward": (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _p,
-                                             _i64, _i64, _i64, ctypes.c_double, ctypes.c_double,
-                                             _i32, _i32, _p]),
+                                             _p, _i64, _i64, _i64, ctypes.c_double,
+                                             ctypes.c_double, _i32, _i32, _p]),
     "seis_habc_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32, _p,
                                           _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32,
                                           _i32, _i32, _p]),

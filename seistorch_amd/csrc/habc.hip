@@ -258,16 +258,27 @@ __global__ __launch_bounds__(256) void k_habc_fwd(const float* __restrict__ vp,
 // acoustic_rho_habc.py:59-89).  One cell per thread for spt shots (a runtime
 // count): the face averages, the strip set-up and their divisions are formed
 // once per cell for all of them.
-__device__ __forceinline__ float buoyancy(const float* __restrict__ rho, int z, int x,
+// 1/rho at (z, x), zero outside the grid; from the precomputed plane when
+// the caller gives one (the same correctly rounded division)
+__device__ __forceinline__ float buoyancy(const float* __restrict__ rho,
+                                          const float* __restrict__ buoy, int z, int x,
                                           const Grid& g) {
-  return (z >= 0 && z < g.nz && x >= 0 && x < g.nx) ? 1.0f / rho[z * g.nx + x] : 0.0f;
+  if (z < 0 || z >= g.nz || x < 0 || x >= g.nx) return 0.0f;
+  const int j = z * g.nx + x;
+  return buoy ? buoy[j] : 1.0f / rho[j];
+}
+
+__global__ __launch_bounds__(256) void k_reciprocal(const float* __restrict__ a,
+                                                    float* __restrict__ r, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = 1.0f / a[i];
 }
 
 __global__ __launch_bounds__(256) void k_rho_habc_fwd(
     const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ bw,
-    const float* __restrict__ u, const float* __restrict__ p, float* __restrict__ out,
-    const int32_t* __restrict__ src, const float* __restrict__ wn, int B, int spt, HabcCtx hc,
-    float C, int blend, TileMap tm) {
+    const float* __restrict__ buoy, const float* __restrict__ u, const float* __restrict__ p,
+    float* __restrict__ out, const int32_t* __restrict__ src, const float* __restrict__ wn, int B,
+    int spt, HabcCtx hc, float C, int blend, TileMap tm) {
   int tx, tzi, grp;
   if (!tile_of(tm, tx, tzi, grp)) return;
   const Grid& g = hc.g;
@@ -276,22 +287,30 @@ __global__ __launch_bounds__(256) void k_rho_habc_fwd(
   if (x >= g.nx || z >= g.nz) return;
   const int i = z * g.nx + x;
   const float c = vp[i];
-  const float bc = 1.0f / rho[i];
-  const float bxp = (bc + buoyancy(rho, z, x + 1, g)) / 2.0f;
-  const float bxn = (buoyancy(rho, z, x - 1, g) + bc) / 2.0f;
-  const float bzp = (bc + buoyancy(rho, z + 1, x, g)) / 2.0f;
-  const float bzn = (buoyancy(rho, z - 1, x, g) + bc) / 2.0f;
+  const float bc = buoyancy(rho, buoy, z, x, g);
+  const float bxp = (bc + buoyancy(rho, buoy, z, x + 1, g)) / 2.0f;
+  const float bxn = (buoyancy(rho, buoy, z, x - 1, g) + bc) / 2.0f;
+  const float bzp = (bc + buoyancy(rho, buoy, z + 1, x, g)) / 2.0f;
+  const float bzn = (buoyancy(rho, buoy, z - 1, x, g) + bc) / 2.0f;
   const float coef = (C * rho[i]) * (c * c);
   const bool edge = blend && near_edge(z, x, g, kW);
-  const float b = edge ? bw[i] : 0.0f;
-  const OwCoef kc = ow_coef(c, hc);
-  const bool mt = m_top(z, x, hc), mb = m_bottom(z, x, hc), ml = m_left(z, x, hc),
-             mr = m_right(z, x, hc);
+  // the strip set-up only on the frame of strips
+  float b = 0.0f;
+  OwCoef kc{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  bool mt = false, mb = false, ml = false, mr = false;
   int d1 = -1, d2 = -1;
-  if (!hc.multiple && z < kW && x == z) { d1 = 0; d2 = 2; }
-  if (!hc.multiple && z < kW && x == g.nx - kW + z) { d1 = 0; d2 = 3; }
-  if (z >= g.nz - kW && x == z - (g.nz - kW)) { d1 = 1; d2 = 2; }
-  if (z >= g.nz - kW && x == g.nx - kW + (z - (g.nz - kW))) { d1 = 1; d2 = 3; }
+  if (edge) {
+    b = bw[i];
+    kc = ow_coef(c, hc);
+    mt = m_top(z, x, hc);
+    mb = m_bottom(z, x, hc);
+    ml = m_left(z, x, hc);
+    mr = m_right(z, x, hc);
+    if (!hc.multiple && z < kW && x == z) { d1 = 0; d2 = 2; }
+    if (!hc.multiple && z < kW && x == g.nx - kW + z) { d1 = 0; d2 = 3; }
+    if (z >= g.nz - kW && x == z - (g.nz - kW)) { d1 = 1; d2 = 2; }
+    if (z >= g.nz - kW && x == g.nx - kW + (z - (g.nz - kW))) { d1 = 1; d2 = 3; }
+  }
   const float w = src ? *wn : 0.0f;
   const int b0 = grp * spt, nb = min(spt, B - b0);
   for (int s = 0; s < nb; ++s) {
@@ -932,20 +951,22 @@ int seis_rho_habc_step(const float* vp, const float* rho, const float* b, const 
   const TileMap tm = make_tiles(hc.g, 1, 1, (int)((B + spt - 1) / spt));
   const float C = (float)(dt * dt * std::pow(h, -2.0));
   hipStream_t s = (hipStream_t)stream;
-  k_rho_habc_fwd<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, b, h1, h2, y, nullptr, nullptr,
-                                                      (int)B, spt, hc, C, blend ? 1 : 0, tm);
+  k_rho_habc_fwd<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, b, nullptr, h1, h2, y, nullptr,
+                                                      nullptr, (int)B, spt, hc, C, blend ? 1 : 0,
+                                                      tm);
   SEIS_CHECK_LAUNCH();
   return 0;
 }
 
 int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
                           const float* wavelet, int64_t nt, const int32_t* src_zx,
-                          const int64_t* rec_flat, int64_t R, float* rec_out, float* u_a,
-                          float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx, double dt,
-                          double h, int32_t multiple, int32_t shots_per_group, void* stream) {
+                          const int64_t* rec_flat, int64_t R, float* rec_out, float* buoy,
+                          float* u_a, float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx,
+                          double dt, double h, int32_t multiple, int32_t shots_per_group,
+                          void* stream) {
   if (int e = habc_check(B, nz, nx, (float)dt, (float)h, 1)) return e;
   SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1, got %d", shots_per_group);
-  SEIS_CHECK_ARG(vp && rho && b && wavelet && src_zx && u_a && u_b && u_c,
+  SEIS_CHECK_ARG(vp && rho && b && buoy && wavelet && src_zx && u_a && u_b && u_c,
                  "null pointer argument");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
@@ -965,11 +986,13 @@ int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
     k_habc_fill<<<nblk(field), 256, 0, s>>>(q, field);
     SEIS_CHECK_LAUNCH();
   }
+  k_reciprocal<<<nblk(g.plane), 256, 0, s>>>(rho, buoy, g.plane);
+  SEIS_CHECK_LAUNCH();
   for (int64_t n = 0; n < nt; ++n) {
     float* un = P[n % 3];
     float* up = P[(n + 2) % 3];
     float* out = P[(n + 1) % 3];
-    k_rho_habc_fwd<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, b, un, up, out, src_zx,
+    k_rho_habc_fwd<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, b, buoy, un, up, out, src_zx,
                                                         wavelet + n, (int)B, S, hc, C, 1, tm);
     SEIS_CHECK_LAUNCH();
     if (R > 0) {

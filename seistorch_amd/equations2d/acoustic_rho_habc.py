@@ -103,9 +103,11 @@ class RhoHabcLoop(torch.autograd.Function):
             S = min(B, DEFAULT_SHOTS_PER_THREAD)
         u = [torch.empty((B, nz, nx), device=vp.device, dtype=torch.float32) for _ in range(3)]
         rec = torch.empty((nt, R), device=vp.device, dtype=torch.float32)
+        buoy = torch.empty_like(vp)
         _lib.call("seis_rho_habc_forward", _lib.ptr(vp), _lib.ptr(rho), _lib.ptr(b),
                   _lib.ptr(wavelet), nt, _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec),
-                  *[_lib.ptr(t) for t in u], B, nz, nx, float(dt), float(h), int(multiple),
+                  _lib.ptr(buoy), *[_lib.ptr(t) for t in u], B, nz, nx, float(dt), float(h),
+                  int(multiple),
                   int(S), _lib.stream_of(vp))
         return rec
 
