@@ -114,3 +114,26 @@ def test_gradient_postprocess_argument_checks():
     with pytest.raises(RuntimeError, match="alias"):
         _lib.call("seis_smooth_gradient", f1, f2, f2, 10, 10, w3, 3, w3, 3, 1, None, None, None,
                   None)
+
+
+def test_variable_density_and_illumination_argument_checks():
+    """Host-side validation of the variable-density entry points and of the
+    forward's illumination buffer (no device call is reached)."""
+    import ctypes
+    from seistorch_amd import _lib
+    f1, f2 = ctypes.c_void_p(256), ctypes.c_void_p(512)
+    with pytest.raises(RuntimeError, match="at least"):
+        _lib.call("seis_rho_habc_step", f1, f1, f1, f1, f1, f2, 1, 50, 200, 1e-3, 10.0, 0, 1,
+                  None)
+    with pytest.raises(RuntimeError, match="null pointer"):
+        _lib.call("seis_rho_habc_step", f1, None, f1, f1, f1, f2, 1, 120, 120, 1e-3, 10.0, 0, 1,
+                  None)
+    with pytest.raises(RuntimeError, match="shots_per_group"):
+        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 10, f1, f1, 0, None, f1, f1, f1, f1,
+                  1, 120, 120, 1e-3, 10.0, 0, 0, None)
+    with pytest.raises(RuntimeError, match="nt must be"):
+        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 0, f1, f1, 0, None, f1, f1, f1, f1,
+                  1, 120, 120, 1e-3, 10.0, 0, 4, None)
+    with pytest.raises(RuntimeError, match="illum must not alias"):
+        _lib.call("seis_acoustic_forward", f1, f1, f1, 10, f1, None, 0, None, None, None, 0, f1,
+                  f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 1, 1, 1, None)
