@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 6
+#define SEIS_ABI_VERSION 7
 
 #ifdef __cplusplus
 extern "C" {
@@ -193,14 +193,15 @@ int seis_habc_step_vjp(const float* vp, const float* b, const float* h1, const f
                        const float* gy, float* gh1, float* gh2, float* gvp, int64_t B, int64_t nz,
                        int64_t nx, float dt, float h, int32_t multiple, void* stream);
 
-/* Whole HABC forward loop, as seis_acoustic_forward (same records, ring and
- * snapshot layouts) with three field buffers: on return u_a holds u^{nt},
- * u_b u^{nt-1}, u_c is scratch. */
+/* Whole HABC forward loop, as seis_acoustic_forward (same records, ring,
+ * snapshot and illum layouts) with three field buffers: on return u_a holds
+ * u^{nt}, u_b u^{nt-1}, u_c is scratch. */
 int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int64_t nt,
                       const int32_t* src_zx, const int64_t* rec_flat, int64_t R, float* rec_out,
-                      float* ring, float* snaps, int32_t snap_every, float* u_a, float* u_b,
-                      float* u_c, int64_t B, int64_t nz, int64_t nx, float dt, float h,
-                      int32_t npml, int32_t multiple, int32_t shots_per_group, void* stream);
+                      float* ring, float* snaps, int32_t snap_every, float* illum, float* u_a,
+                      float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx, float dt,
+                      float h, int32_t npml, int32_t multiple, int32_t shots_per_group,
+                      void* stream);
 
 /* Adjoint of seis_habc_forward with boundary-saving reconstruction
  * (acoustic_habc.py:223-268 + checkpoint_new.py, exact pairing): three

@@ -20,6 +20,7 @@
 // once per S shot-updates.  Neighbour reads hit L1/L2.
 #include "common.h"
 #include "seistorch_amd.h"
+#include "illum.h"
 #include "tiles.h"
 
 #include <algorithm>
@@ -346,28 +347,6 @@ __global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
     const int64_t j = t - b * ring_len;
     ring_slot[t] = u[b * g.plane + ring_cell(j, g)];
   }
-}
-
-// Source illumination of one time level (rnn.py:204-205): illum += sum over
-// shots of u^2, the shots summed in order into one FP32 value per cell, then
-// added, as torch.sum(u**2, 0) followed by +=.
-__global__ __launch_bounds__(256) void k_illum(const float* __restrict__ u, int B, int64_t plane,
-                                               float* __restrict__ illum) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= plane) return;
-  float acc = 0.0f;
-  for (int b = 0; b < B; ++b) {
-    const float v = u[b * plane + c];
-    acc = acc + v * v;
-  }
-  illum[c] = illum[c] + acc;
-}
-
-inline int illuminate(const float* u, int B, int64_t plane, float* illum, hipStream_t s) {
-  if (!illum) return 0;
-  k_illum<<<(unsigned)((plane + 255) / 256), 256, 0, s>>>(u, B, plane, illum);
-  SEIS_CHECK_LAUNCH();
-  return 0;
 }
 
 // One reverse iteration k (see file header): mu^k over the whole grid into

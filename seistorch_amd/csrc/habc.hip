@@ -22,6 +22,7 @@
 #include <cmath>
 
 #include "common.h"
+#include "illum.h"
 #include "seistorch_amd.h"
 #include "tiles.h"
 
@@ -880,11 +881,14 @@ int seis_habc_step_vjp(const float* vp, const float* b, const float* h1, const f
 
 int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int64_t nt,
                       const int32_t* src_zx, const int64_t* rec_flat, int64_t R, float* rec_out,
-                      float* ring, float* snaps, int32_t snap_every, float* u_a, float* u_b,
-                      float* u_c, int64_t B, int64_t nz, int64_t nx, float dt, float h,
-                      int32_t npml, int32_t multiple, int32_t shots_per_group, void* stream) {
+                      float* ring, float* snaps, int32_t snap_every, float* illum, float* u_a,
+                      float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx, float dt,
+                      float h, int32_t npml, int32_t multiple, int32_t shots_per_group,
+                      void* stream) {
   if (int e = habc_check(B, nz, nx, dt, h, shots_per_group)) return e;
   SEIS_CHECK_ARG(vp && b && wavelet && src_zx && u_a && u_b && u_c, "null pointer argument");
+  SEIS_CHECK_ARG(!illum || (illum != u_a && illum != u_b && illum != u_c),
+                 "illum must not alias a field buffer");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
   const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml) : 0;
@@ -914,6 +918,10 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
     k_habc_fill<<<nblk(field), 256, 0, s>>>(q, field);
     SEIS_CHECK_LAUNCH();
   }
+  if (illum) {
+    k_habc_fill<<<nblk(g.plane), 256, 0, s>>>(illum, g.plane);
+    SEIS_CHECK_LAUNCH();
+  }
   for (int64_t n = 0; n < nt; ++n) {
     float* un = P[n % 3];
     float* up = P[(n + 2) % 3];
@@ -922,6 +930,7 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
     SEIS_HABC_LAUNCH_V(k_habc_fwd_deep, S, VX, dim3((tmd.per + fr.per) * kXcds), s, vp, un,
                        up, out, src_zx, wavelet + n, (int)B, hc, k, dp, tmd, b, fr, ngf, G);
     SEIS_PROF(SEIS_KERNEL_HABC_FWD, s);
+    if (int e = illuminate(out, (int)B, g.plane, illum, s)) return e;
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
     if (work > 0) {

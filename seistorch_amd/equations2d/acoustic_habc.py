@@ -94,9 +94,11 @@ def _time_step_backward_multiple(*args, **kwargs):
 class HabcLoop(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vp, b, wavelet, src_zx, rec_flat, B, dt, h, multiple, shots_per_group,
-                snapshot_every):
+                snapshot_every, illum=None):
         from .acoustic import default_snapshot_every, ring_len
-        _lib.require_device(vp, b, wavelet, src_zx, rec_flat)
+        _lib.require_device(vp, b, wavelet, src_zx, rec_flat, illum)
+        if illum is not None and illum.shape != vp.shape:
+            raise RuntimeError("illum must have the model's shape")
         need_grad = ctx.needs_input_grad[0]
         # the HABC weight is zero on the interior, which the adjoint's reversal needs
         _check_acquisition(vp, b, wavelet, src_zx, rec_flat, B, need_grad)
@@ -118,8 +120,8 @@ class HabcLoop(torch.autograd.Function):
                 snaps = torch.empty((n_snap, 2, B, nz, nx), device=vp.device)
         _lib.call("seis_habc_forward", _lib.ptr(vp), _lib.ptr(b), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
-                  _lib.ptr(snaps), K, _lib.ptr(u[0]), _lib.ptr(u[1]), _lib.ptr(u[2]), B, nz, nx,
-                  dt, h, NPML, int(multiple), S, _lib.stream_of(vp))
+                  _lib.ptr(snaps), K, _lib.ptr(illum), _lib.ptr(u[0]), _lib.ptr(u[1]),
+                  _lib.ptr(u[2]), B, nz, nx, dt, h, NPML, int(multiple), S, _lib.stream_of(vp))
         if need_grad:
             ctx.save_for_backward(vp, b, wavelet, src_zx, rec_flat)
             ctx.state = (ring, snaps, u[0], u[1])
@@ -146,11 +148,14 @@ class HabcLoop(torch.autograd.Function):
                   K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(lam[0]),
                   _lib.ptr(lam[1]), _lib.ptr(lam[2]), _lib.ptr(gpart), _lib.ptr(grad), B, nz, nx,
                   dt, h, NPML, multiple, S, _lib.stream_of(vp))
-        return (grad,) + (None,) * 10
+        return (grad,) + (None,) * 11
 
 
 def propagate(vp, b, wavelet, src_zx, rec_flat, B, dt, h, multiple=False, shots_per_group=None,
-              snapshot_every=None):
-    """Records (nt, R) of the HABC loop; differentiable w.r.t. vp."""
+              snapshot_every=None, illum=None):
+    """Records (nt, R) of the HABC loop; differentiable w.r.t. vp.  illum:
+    None, or an (nz, nx) float32 tensor filled with the source illumination
+    (rnn.py:127-128, 204-205)."""
     return HabcLoop.apply(vp, b, wavelet.contiguous(), src_zx.contiguous(), rec_flat.contiguous(),
-                          int(B), _f(dt), _f(h), bool(multiple), shots_per_group, snapshot_every)
+                          int(B), _f(dt), _f(h), bool(multiple), shots_per_group, snapshot_every,
+                          illum)

@@ -49,6 +49,9 @@ class Problem:
     band: object = "all"     # multiscale frequency spec (seistorch/signal.py:25-37)
     filter_ord: int = 3      # Butterworth order (template.yml:44)
     _fobs: tuple = None      # (band, filtered observed records)
+    # the configuration's `equation:` (model.py:64-77): "acoustic" (PML, d is
+    # the damping) or "acoustic_habc" (d is the HABC weight field)
+    equation: str = "acoustic"
 
     def interior(self, v):
         return v[PAD:v.shape[0] - PAD, PAD:v.shape[1] - PAD]
@@ -56,10 +59,19 @@ class Problem:
     def pad(self, vi):
         return F.pad(vi[None, None], (PAD, PAD, PAD, PAD), mode="replicate")[0, 0].contiguous()
 
+    def _propagate(self, vp, illum=None):
+        if self.equation == "acoustic":
+            return ac.propagate(vp, self.d, self.wavelet, self.src, self.rec, self.B, self.dt,
+                                self.h, illum=illum)
+        if self.equation == "acoustic_habc":
+            from .equations2d import acoustic_habc as ah
+            return ah.propagate(vp, self.d, self.wavelet, self.src, self.rec, self.B, self.dt,
+                                self.h, illum=illum)
+        raise ValueError(f"unsupported equation {self.equation!r}")
+
     def records(self, vp):
         with torch.no_grad():
-            return ac.propagate(vp, self.d, self.wavelet, self.src, self.rec, self.B, self.dt,
-                                self.h)
+            return self._propagate(vp)
 
     def _filter(self, rec):
         """Records filtered to the current band (seistorch_dist.py:222-225:
@@ -87,8 +99,7 @@ class Problem:
         """(misfit, interior gradient); illum: optional (nz, nx) buffer that
         receives the forward's source illumination."""
         v = vp.detach().clone().requires_grad_(True)
-        rec = ac.propagate(v, self.d, self.wavelet, self.src, self.rec, self.B, self.dt, self.h,
-                           illum=illum)
+        rec = self._propagate(v, illum=illum)
         loss = torch.sum((self._filter(rec) - self.observed()) ** 2)
         loss.backward()
         g = self.interior(v.grad).clone()

@@ -81,3 +81,18 @@ def test_multiscale_loop_with_smoothing_and_cut():
                                   source_illumination=True, log=print)
     assert hist.misfit[-1] < hist.misfit[0]
     assert torch.equal(p.interior(vp)[:8], vi0[:8])
+
+
+def test_habc_problem_in_the_loop():
+    """equation: acoustic_habc in the FWI loop (its records, gradient and
+    illumination), with smoothing and illumination preconditioning."""
+    from seistorch_amd import fwi
+    from seistorch_amd.habc import generate_habc_coefficients_2d
+    p = _small_problem()
+    p.equation = "acoustic_habc"
+    p.d = generate_habc_coefficients_2d(tuple(p.vp_true.shape), 50).cuda()
+    p.obs = p.records(p.vp_true)
+    sm = {"counts": 1, "sigma": {"z": 1.0, "x": 1.0}, "radius": {"z": 2, "x": 2}}
+    vp, hist = fwi.run_multiscale(p, ["all"], epochs=3, lr=10.0, smooth=sm,
+                                  source_illumination=True, log=print)
+    assert hist.misfit[-1] < hist.misfit[0]

@@ -71,3 +71,33 @@ def test_illumination_preconditioned_gradient():
     post.precondition()
     # cells the wavefield never reached hold 0 / 0, as in the reference
     assert torch.allclose(vp.grad, g0 / illum, rtol=0, atol=0, equal_nan=True)
+
+
+def test_habc_illumination_matches_the_reference_loop():
+    """The HABC forward's illumination against the reference loop on the
+    per-step HABC function, FP32 sums in the reference's order."""
+    from seistorch_amd.equations2d import acoustic_habc as ah
+    from seistorch_amd.habc import bound_mask, generate_habc_coefficients_2d
+    c = acoustic_case(nshots=3, nt=150)
+    t = to_dev(c)
+    b = generate_habc_coefficients_2d((c.nz, c.nx), 50).cuda()
+    illum = torch.empty((c.nz, c.nx), device="cuda")
+    rec = ah.propagate(t["vp"], b, t["w"], t["src"], t["rec"], c.B, c.dt, c.h, illum=illum)
+    rec0 = ah.propagate(t["vp"], b, t["w"], t["src"], t["rec"], c.B, c.dt, c.h)
+    assert torch.equal(rec, rec0)
+    masks = bound_mask(c.nz, c.nx, 50, "cuda", c.B, return_idx=True)
+    smask = torch.zeros((c.B, c.nz, c.nx), device="cuda")
+    smask[torch.arange(c.B), t["src"][:, 0].long(), t["src"][:, 1].long()] = 1.0
+    h1 = torch.zeros_like(smask)
+    h2 = torch.zeros_like(smask)
+    P = torch.zeros((c.nz, c.nx), device="cuda")
+    for n in range(c.nt):
+        h1, h2 = ah._time_step(t["vp"], h1, h2, c.dt, c.h, b, habcs=masks)
+        h1 = h1 + smask * t["w"][n]
+        s = torch.zeros_like(P)
+        for k in range(c.B):
+            s = s + h1[k] * h1[k]
+        P = P + s
+    e = rel_l2(illum.cpu().numpy(), P.cpu().numpy())
+    print(f"HABC illumination vs the per-step loop: {e:.2e} (bitwise {torch.equal(illum, P)})")
+    assert e < 1e-6
