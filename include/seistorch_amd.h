@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 7
+#define SEIS_ABI_VERSION 8
 
 #ifdef __cplusplus
 extern "C" {
@@ -231,16 +231,17 @@ int seis_rho_habc_step(const float* vp, const float* rho, const float* b, const 
  * acoustic_rho_habc.py:31-57): nt steps from zero fields, wavelet[n] added at
  * src_zx[b] after step n, receivers rec_flat[r] recorded into
  * rec_out[n*R + r]; buoy: (nz, nx) work buffer (receives 1/rho, formed once
- * per call); u_a, u_b, u_c: (B, nz, nx) rotating buffers (u^{nt} ends
+ * per call); illum: NULL or the source illumination as in
+ * seis_acoustic_forward; u_a, u_b, u_c: (B, nz, nx) rotating buffers (u^{nt} ends
  * in u_a, u^{nt-1} in u_b); shots_per_group >= 1 shots share one thread
  * (the per-cell set-up runs once for them).  Forward only: the gradient with
  * respect to vp and rho has no HIP adjoint yet. */
 int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
                           const float* wavelet, int64_t nt, const int32_t* src_zx,
                           const int64_t* rec_flat, int64_t R, float* rec_out, float* buoy,
-                          float* u_a, float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx,
-                          double dt, double h, int32_t multiple, int32_t shots_per_group,
-                          void* stream);
+                          float* illum, float* u_a, float* u_b, float* u_c, int64_t B,
+                          int64_t nz, int64_t nx, double dt, double h, int32_t multiple,
+                          int32_t shots_per_group, void* stream);
 
 /* ---------------- elastic, isotropic velocity-stress, PML ---------------- */
 /* Fields are passed as an array of five (B, nz, nx) pointers in the order

@@ -51,7 +51,7 @@ SIGNATURES = {
     "seis_rho_habc_step": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i64, _i64,
                                           ctypes.c_double, ctypes.c_double, _i32, _i32, _p]),
     "seis_rho_habc_forward": (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _p,
-                                             _p, _i64, _i64, _i64, ctypes.c_double,
+                                             _p, _p, _i64, _i64, _i64, ctypes.c_double,
                                              ctypes.c_double, _i32, _i32, _p]),
     "seis_habc_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32, _p,
                                           _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32,
@@ -93,7 +93,7 @@ SIGNATURES = {
     "seis_unet_fill": (ctypes.c_int, [_p, _i64, _f, _p]),
 }
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 BATCH_COLS = 22  # SEIS_BATCH_COLS: int64 columns per instance of the batch table
 
 _lock = threading.Lock()

@@ -970,13 +970,15 @@ int seis_rho_habc_step(const float* vp, const float* rho, const float* b, const 
 int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
                           const float* wavelet, int64_t nt, const int32_t* src_zx,
                           const int64_t* rec_flat, int64_t R, float* rec_out, float* buoy,
-                          float* u_a, float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx,
-                          double dt, double h, int32_t multiple, int32_t shots_per_group,
-                          void* stream) {
+                          float* illum, float* u_a, float* u_b, float* u_c, int64_t B,
+                          int64_t nz, int64_t nx, double dt, double h, int32_t multiple,
+                          int32_t shots_per_group, void* stream) {
   if (int e = habc_check(B, nz, nx, (float)dt, (float)h, 1)) return e;
   SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1, got %d", shots_per_group);
   SEIS_CHECK_ARG(vp && rho && b && buoy && wavelet && src_zx && u_a && u_b && u_c,
                  "null pointer argument");
+  SEIS_CHECK_ARG(!illum || (illum != u_a && illum != u_b && illum != u_c && illum != buoy),
+                 "illum must not alias a field buffer");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
   const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, (float)dt, (float)h};
@@ -997,6 +999,10 @@ int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
   }
   k_reciprocal<<<nblk(g.plane), 256, 0, s>>>(rho, buoy, g.plane);
   SEIS_CHECK_LAUNCH();
+  if (illum) {
+    k_habc_fill<<<nblk(g.plane), 256, 0, s>>>(illum, g.plane);
+    SEIS_CHECK_LAUNCH();
+  }
   for (int64_t n = 0; n < nt; ++n) {
     float* un = P[n % 3];
     float* up = P[(n + 2) % 3];
@@ -1004,6 +1010,7 @@ int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
     k_rho_habc_fwd<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, b, buoy, un, up, out, src_zx,
                                                         wavelet + n, (int)B, S, hc, C, 1, tm);
     SEIS_CHECK_LAUNCH();
+    if (int e = illuminate(out, (int)B, g.plane, illum, s)) return e;
     if (R > 0) {
       k_habc_record<<<nblk(R), 256, 0, s>>>(out, rec_flat, R, rec_out + n * R, nullptr, 0,
                                             (int)B, g);

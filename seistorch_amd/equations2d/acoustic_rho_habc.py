@@ -91,8 +91,10 @@ def _time_step_backward_multiple(*args, **kwargs):
 
 class RhoHabcLoop(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, vp, rho, b, wavelet, src_zx, rec_flat, B, dt, h, multiple, S):
-        _lib.require_device(vp, rho, b, wavelet, src_zx, rec_flat)
+    def forward(ctx, vp, rho, b, wavelet, src_zx, rec_flat, B, dt, h, multiple, S, illum=None):
+        _lib.require_device(vp, rho, b, wavelet, src_zx, rec_flat, illum)
+        if illum is not None and illum.shape != vp.shape:
+            raise RuntimeError("illum must have the model's shape")
         _check_acquisition(vp, b, wavelet, src_zx, rec_flat, B, False)
         if rho.shape != vp.shape:
             raise RuntimeError("rho must have the model's shape")
@@ -106,7 +108,8 @@ class RhoHabcLoop(torch.autograd.Function):
         buoy = torch.empty_like(vp)
         _lib.call("seis_rho_habc_forward", _lib.ptr(vp), _lib.ptr(rho), _lib.ptr(b),
                   _lib.ptr(wavelet), nt, _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec),
-                  _lib.ptr(buoy), *[_lib.ptr(t) for t in u], B, nz, nx, float(dt), float(h),
+                  _lib.ptr(buoy), _lib.ptr(illum), *[_lib.ptr(t) for t in u], B, nz, nx,
+                  float(dt), float(h),
                   int(multiple),
                   int(S), _lib.stream_of(vp))
         return rec
@@ -117,10 +120,11 @@ class RhoHabcLoop(torch.autograd.Function):
 
 
 def propagate(vp, rho, b, wavelet, src_zx, rec_flat, B, dt, h, multiple=False,
-              shots_per_group=None):
+              shots_per_group=None, illum=None):
     """Records (nt, R) of B shots of the variable-density HABC loop (forward
     only).  dt and h are taken as Python floats (double), as the reference's
-    configuration scalars are."""
+    configuration scalars are.  illum: None, or an (nz, nx) float32 tensor
+    filled with the source illumination (rnn.py:127-128, 204-205)."""
     return RhoHabcLoop.apply(vp, rho, b, wavelet.contiguous(), src_zx.contiguous(),
                              rec_flat.contiguous(), int(B), float(dt), float(h), bool(multiple),
-                             shots_per_group)
+                             shots_per_group, illum)

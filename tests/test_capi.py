@@ -129,11 +129,11 @@ def test_variable_density_and_illumination_argument_checks():
         _lib.call("seis_rho_habc_step", f1, None, f1, f1, f1, f2, 1, 120, 120, 1e-3, 10.0, 0, 1,
                   None)
     with pytest.raises(RuntimeError, match="shots_per_group"):
-        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 10, f1, f1, 0, None, f1, f1, f1, f1,
-                  1, 120, 120, 1e-3, 10.0, 0, 0, None)
+        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 10, f1, f1, 0, None, f1, None, f1,
+                  f1, f1, 1, 120, 120, 1e-3, 10.0, 0, 0, None)
     with pytest.raises(RuntimeError, match="nt must be"):
-        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 0, f1, f1, 0, None, f1, f1, f1, f1,
-                  1, 120, 120, 1e-3, 10.0, 0, 4, None)
+        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 0, f1, f1, 0, None, f1, None, f1,
+                  f1, f1, 1, 120, 120, 1e-3, 10.0, 0, 4, None)
     with pytest.raises(RuntimeError, match="illum must not alias"):
         _lib.call("seis_acoustic_forward", f1, f1, f1, 10, f1, None, 0, None, None, None, 0, f1,
                   f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 1, 1, 1, None)

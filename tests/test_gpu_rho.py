@@ -134,3 +134,32 @@ def test_gradient_is_refused():
                        t["src"], t["rec"], c.B, c.dt, c.h)
     with pytest.raises(NotImplementedError):
         rec.sum().backward()
+
+
+def test_illumination_matches_the_reference_loop():
+    """The variable-density forward's source illumination against the
+    reference loop on the per-step function (rnn.py:204-205 order)."""
+    from seistorch_amd.equations2d import acoustic_rho_habc as ar
+    c = _rho_case(nshots=3, nt=120)
+    t = to_dev(c)
+    d = lambda a: torch.from_numpy(a).cuda()
+    illum = torch.empty((c.nz, c.nx), device="cuda")
+    rec = ar.propagate(t["vp"], d(c.rho), d(c.b), t["w"], t["src"], t["rec"], c.B, c.dt, c.h,
+                       illum=illum)
+    rec0 = ar.propagate(t["vp"], d(c.rho), d(c.b), t["w"], t["src"], t["rec"], c.B, c.dt, c.h)
+    assert torch.equal(rec, rec0)
+    smask = torch.zeros((c.B, c.nz, c.nx), device="cuda")
+    smask[torch.arange(c.B), t["src"][:, 0].long(), t["src"][:, 1].long()] = 1.0
+    h1 = torch.zeros_like(smask)
+    h2 = torch.zeros_like(smask)
+    P = torch.zeros((c.nz, c.nx), device="cuda")
+    for n in range(c.nt):
+        h1, h2 = ar._time_step(t["vp"], d(c.rho), h1, h2, c.dt, c.h, d(c.b), habcs=_masks(c))
+        h1 = h1 + smask * t["w"][n]
+        s = torch.zeros_like(P)
+        for k in range(c.B):
+            s = s + h1[k] * h1[k]
+        P = P + s
+    e = rel_l2(illum.cpu().numpy(), P.cpu().numpy())
+    print(f"rho illumination vs the per-step loop: {e:.2e} (bitwise {torch.equal(illum, P)})")
+    assert e < 1e-6
