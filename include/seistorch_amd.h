@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 8
+#define SEIS_ABI_VERSION 9
 
 #ifdef __cplusplus
 extern "C" {
@@ -292,13 +292,17 @@ int seis_elastic_chain(const float* grad_coef, int32_t groups, const float* vp, 
  * (5 x B x nz x nx each).  fields are zeroed first and hold state nt on
  * return.  If scratch (five more (B, nz, nx) buffers) is not NULL, each step
  * runs as one fused pass (both stages, ping-ponging between fields and
- * scratch) with the same results. */
+ * scratch) with the same results.  illum (if not NULL): (nz, nx), receives
+ * the source illumination sum_n sum_b of the squared post-source field
+ * illum_field (0..4 = vx, vz, txx, tzz, txz; the reference uses its last
+ * source type, rnn.py:204-205). */
 int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
                          const int32_t* src_zx, int32_t src_mask, const int64_t* rec_flat,
                          int64_t R, const int32_t* rec_field, int32_t nch, float* rec_out,
-                         float* ring, float* snaps, int32_t snap_every, float* const* fields,
-                         float* const* scratch, int64_t B, int64_t nz, int64_t nx, float dt,
-                         float h, int32_t npml, int32_t shots_per_group, void* stream);
+                         float* ring, float* snaps, int32_t snap_every, float* illum,
+                         int32_t illum_field, float* const* fields, float* const* scratch,
+                         int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                         int32_t shots_per_group, void* stream);
 
 /* Adjoint of seis_elastic_forward with boundary-saving reconstruction
  * (replaces seistorch/checkpoint.py:108-230 and elastic.py:39-122 with the

@@ -24,6 +24,7 @@
 // operation order (k_el_coef), and the stage kernels round op by op in the
 // reference's expression order, so the forward matches its FP32 path.
 #include "common.h"
+#include "illum.h"
 #include "seistorch_amd.h"
 #include "tiles.h"
 
@@ -1172,11 +1173,14 @@ int seis_elastic_chain(const float* grad_coef, int32_t groups, const float* vp, 
 int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
                          const int32_t* src_zx, int32_t src_mask, const int64_t* rec_flat,
                          int64_t R, const int32_t* rec_field, int32_t nch, float* rec_out,
-                         float* ring, float* snaps, int32_t snap_every, float* const* fields,
-                         float* const* scratch, int64_t B, int64_t nz, int64_t nx, float dt,
-                         float h, int32_t npml, int32_t shots_per_group, void* stream) {
+                         float* ring, float* snaps, int32_t snap_every, float* illum,
+                         int32_t illum_field, float* const* fields, float* const* scratch,
+                         int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
+                         int32_t shots_per_group, void* stream) {
   if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
   SEIS_CHECK_ARG(coef && wavelet && src_zx && fields, "null pointer argument");
+  SEIS_CHECK_ARG(!illum || (illum_field >= 0 && illum_field < 5),
+                 "illum_field %d outside [0, 5)", illum_field);
   for (int f = 0; f < 5; ++f) SEIS_CHECK_ARG(fields[f], "null field pointer");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && nch >= 1 && (R == 0 || (rec_flat && rec_field && rec_out)),
@@ -1205,6 +1209,8 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
   if (scratch && nt % 2 == 1) { Fields t = F; F = Fs; Fs = t; }
   for (int f = 0; f < 5; ++f)
     if (int e = zero(F.f[f], field, s)) return e;
+  if (illum)
+    if (int e = zero(illum, g.plane, s)) return e;
   for (int64_t n = 0; n < nt; ++n) {
     SEIS_PROF(SEIS_KERNEL_ELASTIC_FWD, s);
     if (scratch) {
@@ -1218,6 +1224,8 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
     SEIS_PROF(SEIS_KERNEL_ELASTIC_FWD, s);
     k_el_source<<<blocks_for(B), 256, 0, s>>>(F, src_zx, wavelet + n, 1.0f, src_mask, (int)B, g);
     SEIS_CHECK_LAUNCH();
+    if (illum)
+      if (int e = illuminate(F.f[illum_field], (int)B, g.plane, illum, s)) return e;
     float* slot = (ring && n + 1 < nt) ? ring + n * 5 * B * ring_len : nullptr;
     const int64_t work = R * nch + (slot ? 5 * B * ring_len : 0);
     if (work > 0) {

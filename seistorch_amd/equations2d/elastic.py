@@ -175,8 +175,10 @@ class ElasticLoop(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, vp, vs, rho, d, wavelet, src_zx, rec_flat, B, dt, h, src_type, rec_type,
-                shots_per_group, snapshot_every, fused=True):
-        _lib.require_device(vp, vs, rho, d, wavelet, src_zx, rec_flat)
+                shots_per_group, snapshot_every, fused=True, illum=None):
+        _lib.require_device(vp, vs, rho, d, wavelet, src_zx, rec_flat, illum)
+        if illum is not None and illum.shape != vp.shape:
+            raise RuntimeError("illum must have the model's shape")
         from .acoustic import _check_acquisition
         need_grad = any(ctx.needs_input_grad[:3])
         _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad)
@@ -204,7 +206,8 @@ class ElasticLoop(torch.autograd.Function):
         scratch = [torch.empty_like(fields[0]) for _ in range(5)] if fused else None
         _lib.call("seis_elastic_forward", _lib.ptr(cf), _lib.ptr(wavelet), nt, _lib.ptr(src_zx),
                   _mask(src_type), _lib.ptr(rec_flat), R, _lib.ptr(rf), nch, _lib.ptr(rec),
-                  _lib.ptr(ring), _lib.ptr(snaps), K, _lib.ptr_array(fields),
+                  _lib.ptr(ring), _lib.ptr(snaps), K, _lib.ptr(illum),
+                  FIELDS.index(src_type[-1]), _lib.ptr_array(fields),
                   _lib.ptr_array(scratch) if fused else None, B, nz, nx, dt, h, NPML, S,
                   _lib.stream_of(vp))
         del scratch  # ping-pong buffers of the fused steps
@@ -241,14 +244,18 @@ class ElasticLoop(torch.autograd.Function):
         _lib.call("seis_elastic_chain", _lib.ptr(gco), G, _lib.ptr(vp), _lib.ptr(vs),
                   _lib.ptr(rho), _lib.ptr(gvp), _lib.ptr(gvs), _lib.ptr(grho), nz, nx, dt, h,
                   _lib.stream_of(vp))
-        return (gvp, gvs, grho) + (None,) * 12
+        return (gvp, gvs, grho) + (None,) * 13
 
 
 def propagate(vp, vs, rho, d, wavelet, src_zx, rec_flat, B, dt, h, src_type=("txx", "tzz"),
-              rec_type=("vx", "vz"), shots_per_group=None, snapshot_every=None, fused=True):
+              rec_type=("vx", "vz"), shots_per_group=None, snapshot_every=None, fused=True,
+              illum=None):
     """Records (nt, R, nch) of B shots; differentiable w.r.t. vp, vs, rho.
     fused: each forward step and each reverse step of the adjoint in one pass
-    (five more field buffers)."""
+    (five more field buffers).  illum: None, or an (nz, nx) float32 tensor
+    filled with the source illumination of the last source type's field
+    (rnn.py:204-205)."""
     return ElasticLoop.apply(vp, vs, rho, d, wavelet.contiguous(), src_zx.contiguous(),
                              rec_flat.contiguous(), int(B), _f(dt), _f(h), tuple(src_type),
-                             tuple(rec_type), shots_per_group, snapshot_every, bool(fused))
+                             tuple(rec_type), shots_per_group, snapshot_every, bool(fused),
+                             illum)

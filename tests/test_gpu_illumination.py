@@ -101,3 +101,38 @@ def test_habc_illumination_matches_the_reference_loop():
     e = rel_l2(illum.cpu().numpy(), P.cpu().numpy())
     print(f"HABC illumination vs the per-step loop: {e:.2e} (bitwise {torch.equal(illum, P)})")
     assert e < 1e-6
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_elastic_illumination_matches_the_reference_loop(fused):
+    """The elastic forward's illumination of the last source type's field
+    (rnn.py:197-205) against the reference loop on the per-step function."""
+    from seistorch_amd.equations2d import elastic as el
+    from tests.cases import elastic_case, elastic_dev
+    c = elastic_case(nshots=2, nt=120)
+    t = elastic_dev(c)
+    src_type = ("txx", "tzz")
+    illum = torch.empty((c.nz, c.nx), device="cuda")
+    rec = el.propagate(t["vp"], t["vs"], t["rho"], t["d"], t["w"], t["src"], t["rec"], c.B,
+                       c.dt, c.h, src_type=src_type, fused=fused, illum=illum)
+    rec0 = el.propagate(t["vp"], t["vs"], t["rho"], t["d"], t["w"], t["src"], t["rec"], c.B,
+                        c.dt, c.h, src_type=src_type, fused=fused)
+    assert torch.equal(rec, rec0)
+    smask = torch.zeros((c.B, c.nz, c.nx), device="cuda")
+    smask[torch.arange(c.B), t["src"][:, 0].long(), t["src"][:, 1].long()] = 1.0
+    f = [torch.zeros_like(smask) for _ in range(5)]
+    P = torch.zeros((c.nz, c.nx), device="cuda")
+    k = el.FIELDS.index(src_type[-1])
+    for n in range(c.nt):
+        f = list(el._time_step(t["vp"], t["vs"], t["rho"], *f, c.dt, c.h, t["d"]))
+        for st in src_type:
+            j = el.FIELDS.index(st)
+            f[j] = f[j] + smask * t["w"][n]
+        s = torch.zeros_like(P)
+        for b in range(c.B):
+            s = s + f[k][b] * f[k][b]
+        P = P + s
+    e = rel_l2(illum.cpu().numpy(), P.cpu().numpy())
+    print(f"elastic illumination fused={fused} vs the per-step loop: {e:.2e} "
+          f"(bitwise {torch.equal(illum, P)})")
+    assert e < 1e-6
