@@ -300,3 +300,102 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
     finally:
         p.band = band0
     return vp, hist
+
+
+# ------------------------------------------------ configuration-driven runs
+
+def problem_from_config(cfg, device="cuda", shots=None):
+    """A Problem from the reference's configuration keys (template.yml;
+    seistorch_dist.py:90-160): ``equation`` ("acoustic" with a PML boundary
+    or "acoustic_habc"), geom.truePath / initPath ``vp`` (.npy (nz, nx)
+    models of the unpadded grid), geom.sources / receivers (.pkl),
+    geom.obsPath (.npy records, one (nt, nr, 1) array per shot), the Ricker
+    wavelet from geom.fm / dt / nt / wavelet_delay / wavelet_inverse or a
+    geom.wavelet .npy (setup.py:232-252), geom.h, geom.boundary.width and
+    training.filter_ord.  The fused path's ring sits at index 49, so the
+    boundary width must be 50; multiples (no top boundary) are not supported
+    here.  shots: the shot numbers to use (all by default)."""
+    from . import io as sio
+    from .habc import generate_habc_coefficients_2d
+    from .pml import generate_pml_coefficients_2d
+    from .synthetic import rec_flat_index
+    from .utils import pad_model, ricker_wave
+    geom = cfg["geom"]
+    width = int(geom["boundary"]["width"])
+    if width != PAD:
+        raise ValueError(f"boundary width {width}: the fused path needs {PAD}")
+    if geom.get("multiple", False):
+        raise NotImplementedError("multiples (free surface without top boundary) are not "
+                                  "supported by problem_from_config")
+    equation = cfg.get("equation", "acoustic")
+    if equation not in ("acoustic", "acoustic_habc"):
+        raise ValueError(f"unsupported equation {equation!r}")
+    dt, h, nt = float(geom["dt"]), float(geom["h"]), int(geom["nt"])
+
+    def model(key):
+        path = (geom.get(key) or {}).get("vp")
+        return None if not path else torch.from_numpy(
+            pad_model(sio.load_model(path), width).astype("float32")).to(device)
+
+    vp_init = model("initPath")
+    vp_true = model("truePath")
+    if vp_init is None:
+        raise ValueError("geom.initPath.vp is required")
+    if vp_true is None:
+        vp_true = vp_init.clone()
+    nz, nx = vp_init.shape
+    if geom.get("wavelet"):
+        w = torch.from_numpy(sio.load_npy(geom["wavelet"]).astype("float32"))
+    else:
+        w = ricker_wave(geom["fm"], dt, nt, geom.get("wavelet_delay", 80),
+                        inverse=bool(geom.get("wavelet_inverse", False)))
+    if w.shape[0] != nt:
+        raise ValueError(f"wavelet has {w.shape[0]} samples, geom.nt is {nt}")
+    src_list = sio.read_pkl(geom["sources"])
+    rec_list = sio.read_pkl(geom["receivers"])
+    shots = list(range(len(src_list))) if shots is None else list(shots)
+    src, rec = sio.setup_acquisition(src_list, rec_list, width, False, shots)
+    if equation == "acoustic":
+        d = generate_pml_coefficients_2d((nz, nx), width)
+    else:
+        d = generate_habc_coefficients_2d((nz, nx), width)
+    p = Problem(vp_true, vp_init, d.to(device), w.to(device),
+                torch.from_numpy(src).to(device),
+                torch.from_numpy(rec_flat_index(rec, nz, nx)).to(device), len(shots), dt, h,
+                filter_ord=int(cfg.get("training", {}).get("filter_ord", 3)), equation=equation)
+    if geom.get("obsPath"):
+        records = sio.SeisRecord(cfg)
+        records.setup("inversion")
+        obs = sio.stack_records([records[i] for i in shots])[..., 0]
+        if obs.shape != (nt, len(p.rec)):
+            raise ValueError(f"observed records {obs.shape} do not match (nt, receivers) "
+                             f"{(nt, len(p.rec))}")
+        p.obs = torch.from_numpy(obs).to(device)
+    return p
+
+
+def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smooth=False,
+                    grad_cut=False, source_illumination=False, log=None):
+    """The multiscale FWI loop of seistorch_dist.py:166-303 driven by a
+    configuration: geom.multiscale bands, training.N_epochs, lr (a number or
+    {"vp": lr}), scale_decay, lr_decay; the command-line switches
+    --grad-smooth (training.smooth), --grad-cut (geom.seabed, an .npy mask of
+    the unpadded grid) and --source-illumination.  Returns (Problem, vp,
+    History)."""
+    from . import io as sio
+    p = problem_from_config(cfg, device, shots)
+    if p.obs is None:
+        raise ValueError("geom.obsPath is required for an inversion")
+    tr = cfg.get("training", {})
+    lr = tr.get("lr", 5.0)
+    lr = float(lr["vp"] if isinstance(lr, dict) else lr)
+    seabed = None
+    if grad_cut:
+        seabed = torch.from_numpy(sio.load_model(cfg["geom"]["seabed"])).to(device)
+    vp, hist = run_multiscale(p, cfg["geom"].get("multiscale") or ["all"], int(tr["N_epochs"]),
+                              lr=lr, optimizer=optimizer,
+                              scale_decay=float(tr.get("scale_decay", 0.85)),
+                              lr_decay=float(tr.get("lr_decay", 0.995)),
+                              smooth=tr.get("smooth") if grad_smooth else None, seabed=seabed,
+                              source_illumination=source_illumination, log=log)
+    return p, vp, hist
