@@ -399,3 +399,121 @@ def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smoot
                               smooth=tr.get("smooth") if grad_smooth else None, seabed=seabed,
                               source_illumination=source_illumination, log=log)
     return p, vp, hist
+
+
+# ---------------------------------------------------------- elastic FWI
+
+ELASTIC_BOUNDS = {"vp": (1400.0, 6000.0), "vs": (0.0, 3500.0), "rho": (1000.0, 3000.0)}
+
+
+@dataclass
+class ElasticProblem:
+    """Elastic FWI on one GPU (seistorch_dist.py:166-303 with
+    equation: elastic): models_true / models_init hold padded (nz, nx)
+    'vp', 'vs', 'rho'; invlist names the inverted ones (geom.invlist);
+    records (nt, R, nch) of rec_type from sources on src_type."""
+    models_true: dict
+    models_init: dict
+    d: torch.Tensor
+    wavelet: torch.Tensor
+    src: torch.Tensor
+    rec: torch.Tensor
+    B: int
+    dt: float
+    h: float
+    src_type: tuple = ("txx", "tzz")
+    rec_type: tuple = ("vx", "vz")
+    invlist: tuple = ("vp", "vs")
+    obs: torch.Tensor = None
+    mute: int = 5
+    band: object = "all"
+    filter_ord: int = 3
+    _fobs: tuple = None
+
+    interior = Problem.interior
+    pad = Problem.pad
+    _filter = Problem._filter
+    observed = Problem.observed
+
+    def _propagate(self, m):
+        from .equations2d import elastic as el
+        return el.propagate(m["vp"], m["vs"], m["rho"], self.d, self.wavelet, self.src, self.rec,
+                            self.B, self.dt, self.h, src_type=self.src_type,
+                            rec_type=self.rec_type)
+
+    def records(self, m):
+        with torch.no_grad():
+            return self._propagate(m)
+
+    def gradient(self, m):
+        """(misfit, {name: interior gradient} for the inverted parameters)."""
+        mm = {k: (v.detach().clone().requires_grad_(k in self.invlist)) for k, v in m.items()}
+        loss = torch.sum((self._filter(self._propagate(mm)) - self.observed()) ** 2)
+        loss.backward()
+        g = {}
+        for k in self.invlist:
+            gk = self.interior(mm[k].grad).clone()
+            gk[:self.mute] = 0.0
+            g[k] = gk
+        return float(loss), g
+
+    def rmse(self, m, name="vp"):
+        return float(torch.sqrt(torch.mean((self.interior(m[name])
+                                            - self.interior(self.models_true[name])) ** 2)))
+
+
+def run_multiscale_elastic(p, scales, epochs, lr, optimizer="adam", scale_decay=0.85,
+                           lr_decay=0.995, grad_clamp=True, log=None):
+    """run_multiscale for the elastic equation: one fresh optimiser per scale
+    over the inverted parameters, each with its own learning rate (lr: a
+    number or {name: lr}, the reference's --lr vp=.. vs=..), scaled by
+    scale_decay**scale and decayed by lr_decay per epoch; Adam (eps 1e-22) or
+    max-normalised steepest descent per parameter; each parameter clamped to
+    ELASTIC_BOUNDS after the update.  Returns (models, History)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    inv = list(p.invlist)
+    lrs = {k: float(lr[k] if isinstance(lr, dict) else lr) for k in inv}
+    cur_int = {k: p.interior(p.models_init[k]).clone().contiguous() for k in inv}
+    hist = History()
+    band0 = p.band
+
+    def models_of(xs):
+        m = dict(p.models_init)
+        for k in inv:
+            m[k] = p.pad(xs[k].detach())
+        return m
+
+    try:
+        for si, band in enumerate(scales):
+            p.band = band
+            xs = {k: torch.nn.Parameter(cur_int[k].clone()) for k in inv}
+            groups = [{"params": [xs[k]], "lr": lrs[k] * scale_decay ** si} for k in inv]
+            opt = torch.optim.Adam(groups, eps=1e-22) if optimizer == "adam" else None
+            sched = torch.optim.lr_scheduler.ExponentialLR(opt, lr_decay) if opt else None
+            step = {k: lrs[k] * scale_decay ** si for k in inv}
+            for ep in range(epochs):
+                loss, g = p.gradient(models_of(xs))
+                if opt is not None:
+                    opt.zero_grad()
+                    for k in inv:
+                        xs[k].grad = g[k].contiguous()
+                    opt.step()
+                    sched.step()
+                else:
+                    with torch.no_grad():
+                        for k in inv:
+                            xs[k] -= step[k] * _gd_direction(g[k], grad_clamp)
+                            step[k] *= lr_decay
+                with torch.no_grad():
+                    for k in inv:
+                        xs[k].clamp_(*ELASTIC_BOUNDS[k])
+                torch.cuda.synchronize()
+                hist.add(time.perf_counter() - t0, loss, p.rmse(models_of(xs)), f"scale{si}")
+                if log:
+                    log(f"[elastic multiscale] band {band} epoch {ep + 1}: misfit {loss:.4e} "
+                        f"vp rmse {hist.rmse[-1]:.1f}")
+            cur_int = {k: xs[k].detach().contiguous() for k in inv}
+    finally:
+        p.band = band0
+    return models_of(cur_int), hist
