@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 9
+#define SEIS_ABI_VERSION 10
 
 #ifdef __cplusplus
 extern "C" {
@@ -96,14 +96,18 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
  * illum: NULL, or an (nz, nx) buffer that receives the source illumination
  * sum_n sum_b (u_b^{n+1})^2 over the post-source fields (rnn.py:127-128,
  * 204-205: zeroed per forward, the shots summed in order per time level and
- * then added), the divisor of seistorch/process.py:114-117. */
+ * then added), the divisor of seistorch/process.py:114-117.  illum_part:
+ * NULL, or ceil(B/shots_per_group) more (nz, nx) planes with which the
+ * two-step launches accumulate the illumination themselves (per shot group,
+ * summed at the end: the same sum in another order) instead of a pass over
+ * every shot's field per time level. */
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
-                          float* illum, float* u_a, float* u_b, float* u_c, float* u_d,
-                          int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                          int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
-                          void* stream);
+                          float* illum, float* illum_part, float* u_a, float* u_b, float* u_c,
+                          float* u_d, int64_t B, int64_t nz, int64_t nx, float dt, float h,
+                          int32_t npml, int32_t shots_per_group, int32_t rows_per_thread,
+                          int32_t vec_x, void* stream);
 
 /* Adjoint of seis_acoustic_forward (replaces the boundary-saving backward of
  * seistorch/checkpoint_new.py:146-217 and acoustic.py:88-137, with the exact

@@ -31,8 +31,8 @@ SIGNATURES = {
     "seis_acoustic_step_vjp": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64,
                                               _f, _f, _p]),
     "seis_acoustic_forward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
-                                             _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32,
-                                             _i32, _i32, _i32, _p]),
+                                             _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
+                                             _i32, _i32, _i32, _i32, _p]),
     "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
                                               _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64,
                                               _i64, _i64, _f, _f, _i32, _i32, _i32, _i32, _p]),
@@ -93,7 +93,7 @@ SIGNATURES = {
     "seis_unet_fill": (ctypes.c_int, [_p, _i64, _f, _p]),
 }
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 BATCH_COLS = 22  # SEIS_BATCH_COLS: int64 columns per instance of the batch table
 
 _lock = threading.Lock()

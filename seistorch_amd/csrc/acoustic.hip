@@ -216,12 +216,17 @@ __device__ __forceinline__ float step_cell(const float* __restrict__ vp,
 // No LDS, no barriers; each field crosses HBM once per two steps.
 constexpr int kFwd2Out = kLanes - 2;
 
-template <int S, int RZ, int VX>
-__global__ __launch_bounds__(kBlock) void k_fwd2(
+//
+// ILL (the source illumination of rnn.py:204-205): each inner lane also sums
+// the squares of both new levels over its S shots and adds them to its
+// shot group's plane of ipart, (ipart + sum_s (u^{n+1})^2) + sum_s (u^{n+2})^2
+// (the host adds the groups' planes at the end).
+template <int S, int RZ, int VX, bool ILL>
+__device__ __forceinline__ void fwd2_tile(
     const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ ua,
     const float* __restrict__ ub, float* __restrict__ uc, float* __restrict__ ud,
-    const int32_t* __restrict__ src, const float* __restrict__ wn, int B, Grid g,
-    seis::TimeConsts k, TileMap tm) {
+    const int32_t* __restrict__ src, const float* __restrict__ wn, int B, const Grid& g,
+    const seis::TimeConsts& k, const TileMap& tm, float* __restrict__ ipart) {
   int tx, tzi, grp;
   if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
   const int lane = (int)(threadIdx.x % kLanes);
@@ -253,6 +258,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd2(
   }
   const int b0 = grp * S;
   const int nb = min(S, B - b0);
+  float il1[ILL ? RZ : 1][VX], il2[ILL ? RZ : 1][VX];
+  if constexpr (ILL) {
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      vzero<VX>(il1[r]);
+      vzero<VX>(il2[r]);
+    }
+  }
   for (int s = 0; s < nb; ++s) {
     const int b = b0 + s;
     const int64_t off = (int64_t)b * g.plane;
@@ -304,6 +317,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd2(
       for (int r = 0; r < RZ; ++r) {
         const int z = zb + r;
         if (z < g.nz) vstore<VX>(uc + off + z * g.nx + x0, u1[r + 1]);
+        if constexpr (ILL) {
+#pragma unroll
+          for (int e = 0; e < VX; ++e) il1[r][e] = il1[r][e] + u1[r + 1][e] * u1[r + 1][e];
+        }
       }
     }
     // step 2 on rows zb .. zb+RZ-1 (inner lanes)
@@ -324,10 +341,44 @@ __global__ __launch_bounds__(kBlock) void k_fwd2(
                                   c2[q][e], L);
         if (z == sz && x0 + e == sx) v += w1;
         out[e] = v;
+        if constexpr (ILL) il2[r][e] = il2[r][e] + v * v;
       }
       vstore<VX>(ud + off + z * g.nx + x0, out);
     }
   }
+  if constexpr (ILL) {
+    if (!inner || !xin) return;
+    float* P = ipart + (int64_t)grp * g.plane;
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int z = zb + r;
+      if (z >= g.nz) continue;
+      float v[VX];
+      vload<VX>(P + z * g.nx + x0, v);
+#pragma unroll
+      for (int e = 0; e < VX; ++e) v[e] = (v[e] + il1[r][e]) + il2[r][e];
+      vstore<VX>(P + z * g.nx + x0, v);
+    }
+  }
+}
+
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_fwd2(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ ua,
+    const float* __restrict__ ub, float* __restrict__ uc, float* __restrict__ ud,
+    const int32_t* __restrict__ src, const float* __restrict__ wn, int B, Grid g,
+    seis::TimeConsts k, TileMap tm) {
+  fwd2_tile<S, RZ, VX, false>(vp, d, ua, ub, uc, ud, src, wn, B, g, k, tm, nullptr);
+}
+
+// k_fwd2 that also accumulates the source illumination into ipart
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_fwd2_illum(
+    const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ ua,
+    const float* __restrict__ ub, float* __restrict__ uc, float* __restrict__ ud,
+    const int32_t* __restrict__ src, const float* __restrict__ wn, int B, Grid g,
+    seis::TimeConsts k, TileMap tm, float* __restrict__ ipart) {
+  fwd2_tile<S, RZ, VX, true>(vp, d, ua, ub, uc, ud, src, wn, B, g, k, tm, ipart);
 }
 
 // Receivers of the new field into one record row, and its ring into one slot.
@@ -1010,6 +1061,7 @@ inline bool valid_shape(int S, int RZ, int VX) {
 SEIS_DEFINE_LAUNCH(launch_fwd, k_fwd)
 SEIS_DEFINE_LAUNCH(launch_bwd, k_bwd)
 SEIS_DEFINE_LAUNCH(launch_fwd2, k_fwd2)
+SEIS_DEFINE_LAUNCH(launch_fwd2_illum, k_fwd2_illum)
 SEIS_DEFINE_LAUNCH(launch_bwd2, k_bwd2)
 SEIS_DEFINE_LAUNCH(launch_fwd_batch, k_fwd_batch)
 SEIS_DEFINE_LAUNCH(launch_bwd_batch, k_bwd_batch)
@@ -1044,8 +1096,8 @@ int check_grid(int64_t B, int64_t nz, int64_t nx, float dt, float h) {
 // in the one-step loop; u^{nt} and u^{nt-1} end in u_a and u_b.
 int forward_two_step(const float* vp, const float* d, const float* wavelet, int64_t nt,
                      const int32_t* src_zx, const int64_t* rec_flat, int64_t R, float* rec_out,
-                     float* ring, float* snaps, int32_t snap_every, float* illum, float* u_a,
-                     float* u_b,
+                     float* ring, float* snaps, int32_t snap_every, float* illum, float* ipart,
+                     float* u_a, float* u_b,
                      float* u_c, float* u_d, int B, const Grid& g, const seis::TimeConsts& k,
                      int S, int RZ, int VX, const TileMap& tm, int64_t ring_len, int64_t aux,
                      hipStream_t s) {
@@ -1073,8 +1125,10 @@ int forward_two_step(const float* vp, const float* d, const float* wavelet, int6
   TileMap tm2 = tm;
   tm2.ntx = (g.nx + kFwd2Out * VX - 1) / (kFwd2Out * VX);
   tm2.per = (int)(((int64_t)tm2.ntx * tm2.ntz * tm2.groups + kXcds - 1) / kXcds);
-  auto record = [&](const float* u, int64_t n) -> int {  // u = u^{n+1}
-    if (int e = illuminate(u, B, g.plane, illum, s)) return e;
+  // with ipart the two-step launches accumulate the illumination themselves
+  auto record = [&](const float* u, int64_t n, bool ill) -> int {  // u = u^{n+1}
+    if (ill)
+      if (int e = illuminate(u, B, g.plane, illum, s)) return e;
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
     if (work > 0 && aux > 0) {
@@ -1091,19 +1145,32 @@ int forward_two_step(const float* vp, const float* d, const float* wavelet, int6
     }
     return 0;
   };
+  const bool fold = illum && ipart;
+  if (fold)
+    if (int e = fill(ipart, (int64_t)tm.groups * g.plane, 0.0f, s)) return e;
   int64_t n = 0;
   for (; n + 1 < nt; n += 2) {
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD2, s);
-    if (int e = launch_fwd2(S, RZ, VX, (int64_t)tm2.per * kXcds, s, vp, d, cur, prev, n1, n2,
-                            src_zx, wavelet + n, B, g, k, tm2))
+    if (fold) {
+      if (int e = launch_fwd2_illum(S, RZ, VX, (int64_t)tm2.per * kXcds, s, vp, d, cur, prev, n1,
+                                    n2, src_zx, wavelet + n, B, g, k, tm2, ipart))
+        return e;
+    } else if (int e = launch_fwd2(S, RZ, VX, (int64_t)tm2.per * kXcds, s, vp, d, cur, prev, n1,
+                                   n2, src_zx, wavelet + n, B, g, k, tm2)) {
       return e;
+    }
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD2, s);
-    if (int e = record(n1, n)) return e;
-    if (int e = record(n2, n + 1)) return e;
+    if (int e = record(n1, n, !fold)) return e;
+    if (int e = record(n2, n + 1, !fold)) return e;
     if (int e = snapshot(cur, n1, n)) return e;
     if (int e = snapshot(n1, n2, n + 1)) return e;
     float* t0 = cur; float* t1 = prev;
     cur = n2; prev = n1; n1 = t0; n2 = t1;
+  }
+  if (fold) {  // the groups' planes, then the odd tail's level below
+    k_sum_groups<<<(unsigned)((g.plane + 255) / 256), 256, 0, s>>>(ipart, tm.groups, illum,
+                                                                    g.plane);
+    SEIS_CHECK_LAUNCH();
   }
   if (n < nt) {  // odd nt: one more single step, u^{nt} written over u^{nt-2}
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
@@ -1111,7 +1178,7 @@ int forward_two_step(const float* vp, const float* d, const float* wavelet, int6
                            wavelet + n, B, g, k, tm))
       return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
-    if (int e = record(prev, n)) return e;
+    if (int e = record(prev, n, true)) return e;
     if (int e = snapshot(cur, prev, n)) return e;
     float* t = cur; cur = prev; prev = t;
   }
@@ -1414,10 +1481,10 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
-                          float* illum, float* u_a, float* u_b, float* u_c, float* u_d,
-                          int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                          int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
-                          void* stream) {
+                          float* illum, float* illum_part, float* u_a, float* u_b, float* u_c,
+                          float* u_d, int64_t B, int64_t nz, int64_t nx, float dt, float h,
+                          int32_t npml, int32_t shots_per_group, int32_t rows_per_thread,
+                          int32_t vec_x, void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
   SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, vec_x),
                  "launch shape (shots_per_group %d, rows_per_thread %d, vec_x %d) not in "
@@ -1448,6 +1515,9 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   if (illum) {
     SEIS_CHECK_ARG(illum != u_a && illum != u_b && illum != u_c && illum != u_d,
                    "illum must not alias a field buffer");
+    SEIS_CHECK_ARG(!illum_part || (illum_part != illum && illum_part != u_a &&
+                                   illum_part != u_b && illum_part != u_c && illum_part != u_d),
+                   "illum_part must not alias illum or a field buffer");
     if (int e = fill(illum, g.plane, 0.0f, s)) return e;
   }
   if (u_c && u_d) {
@@ -1456,7 +1526,8 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
     SEIS_CHECK_ARG(u_c != u_a && u_c != u_b && u_d != u_a && u_d != u_b && u_c != u_d,
                    "the four field buffers must be distinct");
     return forward_two_step(vp, d, wavelet, nt, src_zx, rec_flat, R, rec_out, ring, snaps,
-                            snap_every, illum, u_a, u_b, u_c, u_d, (int)B, g, k, S, rows_per_thread,
+                            snap_every, illum, illum ? illum_part : nullptr, u_a, u_b, u_c, u_d,
+                            (int)B, g, k, S, rows_per_thread,
                             vec_x, tm, ring_len, aux, s);
   }
   float* cur = (nt % 2 == 0) ? u_a : u_b;

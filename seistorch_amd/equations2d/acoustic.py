@@ -248,9 +248,14 @@ class AcousticLoop(torch.autograd.Function):
         if two_step and nt >= 2:
             u_c = torch.empty_like(u_a)
             u_d = torch.empty_like(u_a)
+        # with two steps per launch the launches sum the illumination per shot group
+        ipart = None
+        if illum is not None and u_c is not None:
+            ipart = torch.empty(((B + S - 1) // S, nz, nx), device=vp.device, dtype=torch.float32)
         _lib.call("seis_acoustic_forward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
-                  _lib.ptr(snaps), K, _lib.ptr(illum), _lib.ptr(u_a), _lib.ptr(u_b),
+                  _lib.ptr(snaps), K, _lib.ptr(illum), _lib.ptr(ipart), _lib.ptr(u_a),
+                  _lib.ptr(u_b),
                   _lib.ptr(u_c), _lib.ptr(u_d), B, nz, nx, dt, h, NPML, S, rz, vx,
                   _lib.stream_of(vp))
         del u_c, u_d  # scratch of the two-step launches

@@ -41,7 +41,8 @@ def test_ring_len_and_error_path():
         _lib.call("seis_acoustic_step", None, None, None, None, None, 0, 10, 10, 1e-3, 10.0, None)
     with pytest.raises(RuntimeError, match="launch shape"):
         _lib.call("seis_acoustic_forward", None, None, None, 10, None, None, 0, None, None, None,
-                  0, None, None, None, None, None, 1, 120, 120, 1e-3, 10.0, 49, 3, 2, 4, None)
+                  0, None, None, None, None, None, None, 1, 120, 120, 1e-3, 10.0, 49, 3, 2, 4,
+                  None)
     with pytest.raises(RuntimeError, match="null pointer"):
         _lib.call("seis_acoustic_step", None, None, None, None, None, 1, 10, 10, 1e-3, 10.0, None)
 
@@ -136,4 +137,4 @@ def test_variable_density_and_illumination_argument_checks():
                   f1, f1, 1, 120, 120, 1e-3, 10.0, 0, 4, None)
     with pytest.raises(RuntimeError, match="illum must not alias"):
         _lib.call("seis_acoustic_forward", f1, f1, f1, 10, f1, None, 0, None, None, None, 0, f1,
-                  f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 1, 1, 1, None)
+                  None, f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 1, 1, 1, None)

@@ -52,7 +52,14 @@ def test_illumination_matches_the_reference_loop(nt, two_step, S):
     e64 = rel_l2(illum.cpu().numpy(), ref64)
     print(f"illumination nt={nt} two_step={two_step} S={S}: vs FP32 reference loop {e32:.2e} "
           f"(bitwise {bitwise}), vs FP64 oracle {e64:.2e}")
-    assert e32 < 1e-6 and e64 < 1e-5
+    if two_step and nt >= 2:
+        # the two-step launches sum per shot group and the groups at the end:
+        # the same nt*B positive terms in another order, so within the
+        # worst-case relative error of such a sum in FP32
+        assert e32 <= nt * c.B * 2.0 ** -24
+    else:
+        assert bitwise
+    assert e64 < 1e-5
 
 
 def test_illumination_preconditioned_gradient():
