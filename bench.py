@@ -487,6 +487,20 @@ def run_rho(args, dev):
                      "bytes_per_cell_update": bytes_per_cell},
         "checksum": float(rec.double().abs().sum()),
     }
+    if not args.no_cpu_baseline:
+        from oracle import ref_torch
+        steps = 40
+        c_np, r_np, b_np = P["vp_true"].cpu().numpy(), rho.cpu().numpy(), b.cpu().numpy()
+        t1 = time.perf_counter()
+        ref_torch.rho_habc_forward(c_np, r_np, b_np, P["dt"], P["h"], P["w_np"][:steps],
+                                   P["src_np"][:1], P["rec_np"][:1])
+        cel = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(nz * nx * steps / cel / 1e9, 5),
+                               "unit": "Gcell-updates/s", "cores": torch.get_num_threads(),
+                               "kind": "port",
+                               "sample": f"1 shot, {nz}x{nx} padded grid, {steps} of {nt} steps, "
+                                         f"oracle/ref_torch.rho_habc_forward (FP32 torch on the "
+                                         f"host), {cel:.1f} s"}
     print(json.dumps(out), flush=True)
 
 
