@@ -306,8 +306,9 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
 
 def problem_from_config(cfg, device="cuda", shots=None):
     """A Problem from the reference's configuration keys (template.yml;
-    seistorch_dist.py:90-160): ``equation`` ("acoustic" with a PML boundary
-    or "acoustic_habc"), geom.truePath / initPath ``vp`` (.npy (nz, nx)
+    seistorch_dist.py:90-160): ``equation`` ("acoustic" with a PML boundary,
+    "acoustic_habc", or "elastic": an ElasticProblem with vp, vs and rho,
+    geom.invlist, source_type and receiver_type), geom.truePath / initPath ``vp`` (.npy (nz, nx)
     models of the unpadded grid), geom.sources / receivers (.pkl),
     geom.obsPath (.npy records, one (nt, nr, 1) array per shot), the Ricker
     wavelet from geom.fm / dt / nt / wavelet_delay / wavelet_inverse or a
@@ -328,12 +329,12 @@ def problem_from_config(cfg, device="cuda", shots=None):
         raise NotImplementedError("multiples (free surface without top boundary) are not "
                                   "supported by problem_from_config")
     equation = cfg.get("equation", "acoustic")
-    if equation not in ("acoustic", "acoustic_habc"):
+    if equation not in ("acoustic", "acoustic_habc", "elastic"):
         raise ValueError(f"unsupported equation {equation!r}")
     dt, h, nt = float(geom["dt"]), float(geom["h"]), int(geom["nt"])
 
-    def model(key):
-        path = (geom.get(key) or {}).get("vp")
+    def model(key, name="vp"):
+        path = (geom.get(key) or {}).get(name)
         return None if not path else torch.from_numpy(
             pad_model(sio.load_model(path), width).astype("float32")).to(device)
 
@@ -355,22 +356,41 @@ def problem_from_config(cfg, device="cuda", shots=None):
     rec_list = sio.read_pkl(geom["receivers"])
     shots = list(range(len(src_list))) if shots is None else list(shots)
     src, rec = sio.setup_acquisition(src_list, rec_list, width, False, shots)
-    if equation == "acoustic":
-        d = generate_pml_coefficients_2d((nz, nx), width)
-    else:
+    if equation == "acoustic_habc":
         d = generate_habc_coefficients_2d((nz, nx), width)
-    p = Problem(vp_true, vp_init, d.to(device), w.to(device),
-                torch.from_numpy(src).to(device),
-                torch.from_numpy(rec_flat_index(rec, nz, nx)).to(device), len(shots), dt, h,
-                filter_ord=int(cfg.get("training", {}).get("filter_ord", 3)), equation=equation)
+    else:
+        d = generate_pml_coefficients_2d((nz, nx), width)
+    common = (d.to(device), w.to(device), torch.from_numpy(src).to(device),
+              torch.from_numpy(rec_flat_index(rec, nz, nx)).to(device), len(shots), dt, h)
+    filter_ord = int(cfg.get("training", {}).get("filter_ord", 3))
+    if equation == "elastic":
+        true, init = {}, {}
+        for k in ("vp", "vs", "rho"):
+            init[k] = model("initPath", k)
+            true[k] = model("truePath", k)
+            if init[k] is None:
+                raise ValueError(f"geom.initPath.{k} is required for the elastic equation")
+            if true[k] is None:
+                true[k] = init[k].clone()
+        inv = geom.get("invlist") or {"vp": True, "vs": True}
+        p = ElasticProblem(true, init, *common,
+                           src_type=tuple(geom.get("source_type") or ("txx", "tzz")),
+                           rec_type=tuple(geom.get("receiver_type") or ("vx", "vz")),
+                           invlist=tuple(k for k in ("vp", "vs", "rho") if inv.get(k)),
+                           filter_ord=filter_ord)
+        nch = len(p.rec_type)
+    else:
+        p = Problem(vp_true, vp_init, *common, filter_ord=filter_ord, equation=equation)
+        nch = 1
     if geom.get("obsPath"):
         records = sio.SeisRecord(cfg)
         records.setup("inversion")
-        obs = sio.stack_records([records[i] for i in shots])[..., 0]
-        if obs.shape != (nt, len(p.rec)):
-            raise ValueError(f"observed records {obs.shape} do not match (nt, receivers) "
-                             f"{(nt, len(p.rec))}")
-        p.obs = torch.from_numpy(obs).to(device)
+        obs = sio.stack_records([records[i] for i in shots])
+        want = (nt, len(p.rec), nch)
+        if obs.shape != want:
+            raise ValueError(f"observed records {obs.shape} do not match (nt, receivers, "
+                             f"channels) {want}")
+        p.obs = torch.from_numpy(obs if nch > 1 else obs[..., 0]).to(device)
     return p
 
 
@@ -380,20 +400,30 @@ def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smoot
     configuration: geom.multiscale bands, training.N_epochs, lr (a number or
     {"vp": lr}), scale_decay, lr_decay; the command-line switches
     --grad-smooth (training.smooth), --grad-cut (geom.seabed, an .npy mask of
-    the unpadded grid) and --source-illumination.  Returns (Problem, vp,
-    History)."""
+    the unpadded grid) and --source-illumination (acoustic equations).
+    Returns (Problem, vp, History); for the elastic equation (ElasticProblem,
+    {name: padded model}, History)."""
     from . import io as sio
     p = problem_from_config(cfg, device, shots)
     if p.obs is None:
         raise ValueError("geom.obsPath is required for an inversion")
     tr = cfg.get("training", {})
     lr = tr.get("lr", 5.0)
+    scales = cfg["geom"].get("multiscale") or ["all"]
+    if isinstance(p, ElasticProblem):
+        if grad_smooth or grad_cut or source_illumination:
+            raise NotImplementedError("gradient smoothing, cut and illumination are not wired "
+                                      "into the elastic loop")
+        models, hist = run_multiscale_elastic(p, scales, int(tr["N_epochs"]), lr,
+                                              optimizer=optimizer,
+                                              scale_decay=float(tr.get("scale_decay", 0.85)),
+                                              lr_decay=float(tr.get("lr_decay", 0.995)), log=log)
+        return p, models, hist
     lr = float(lr["vp"] if isinstance(lr, dict) else lr)
     seabed = None
     if grad_cut:
         seabed = torch.from_numpy(sio.load_model(cfg["geom"]["seabed"])).to(device)
-    vp, hist = run_multiscale(p, cfg["geom"].get("multiscale") or ["all"], int(tr["N_epochs"]),
-                              lr=lr, optimizer=optimizer,
+    vp, hist = run_multiscale(p, scales, int(tr["N_epochs"]), lr=lr, optimizer=optimizer,
                               scale_decay=float(tr.get("scale_decay", 0.85)),
                               lr_decay=float(tr.get("lr_decay", 0.995)),
                               smooth=tr.get("smooth") if grad_smooth else None, seabed=seabed,

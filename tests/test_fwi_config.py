@@ -57,6 +57,31 @@ def test_problem_from_config_rejects(tmp_path):
     with pytest.raises(ValueError, match="records|wavelet"):
         fwi.problem_from_config(cfg, device="cpu")
     cfg["geom"]["nt"] = 50
-    cfg["equation"] = "elastic"
+    cfg["equation"] = "viscoelastic"
     with pytest.raises(ValueError, match="equation"):
         fwi.problem_from_config(cfg, device="cpu")
+    cfg["equation"] = "elastic"
+    with pytest.raises(ValueError, match="initPath.vs"):
+        fwi.problem_from_config(cfg, device="cpu")
+
+
+def test_elastic_problem_from_config(tmp_path):
+    cfg, _ = _write_case(tmp_path)
+    nz, nx, nt = 30, 40, 50
+    for k, v in (("vs", 1200.0), ("rho", 2000.0)):
+        np.save(tmp_path / f"init_{k}.npy", np.full((nz, nx), v))
+        cfg["geom"]["initPath"][k] = str(tmp_path / f"init_{k}.npy")
+    cfg["equation"] = "elastic"
+    cfg["geom"]["invlist"] = {"vp": True, "vs": False, "rho": True}
+    cfg["geom"]["source_type"] = ["vz"]
+    cfg["geom"]["receiver_type"] = ["vx", "vz"]
+    obs = np.empty(2, dtype=np.ndarray)
+    for i in range(2):
+        obs[i] = np.full((nt, nx // 2, 2), float(i), np.float32)
+    np.save(tmp_path / "obs.npy", obs)
+    p = fwi.problem_from_config(cfg, device="cpu")
+    assert isinstance(p, fwi.ElasticProblem)
+    assert p.invlist == ("vp", "rho") and p.src_type == ("vz",) and p.rec_type == ("vx", "vz")
+    assert tuple(p.obs.shape) == (nt, nx, 2) and float(p.obs[0, nx // 2, 1]) == 1.0
+    assert float(p.models_init["vs"][60, 70]) == 1200.0
+    assert float(p.models_true["vs"][60, 70]) == 1200.0  # no truePath.vs: the initial model

@@ -181,3 +181,52 @@ def test_elastic_multiscale_loop(optimizer):
         ms = [m for m, k in zip(hist.misfit, hist.kind) if k == f"scale{si}"]
         assert ms[-1] < ms[0]
     assert torch.equal(models["rho"], true["rho"])
+
+
+def test_run_from_config_elastic(tmp_path):
+    """equation: elastic from files: vp, vs, rho models, geometry, two-component
+    observed records; vp and vs inverted with their own learning rates."""
+    import numpy as np
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from seistorch_amd import fwi, synthetic
+    from seistorch_amd import io as sio
+    from tests.cases import elastic_case
+    c = elastic_case(nshots=3, nt=400)
+    nz_i, nx_i = c.nz - 100, c.nx - 100
+    paths = {}
+    for k in ("vp", "vs", "rho"):
+        a = getattr(c, k)[50:-50, 50:-50].astype(np.float64)
+        np.save(tmp_path / f"true_{k}.npy", a)
+        s = gaussian_filter(a, 4.0) if k != "rho" else a
+        if k == "vs":
+            s[:6] = 0.0
+        np.save(tmp_path / f"init_{k}.npy", s)
+        paths[k] = (str(tmp_path / f"true_{k}.npy"), str(tmp_path / f"init_{k}.npy"))
+    # the case's acquisition in grid units of the unpadded model, [x, z]
+    sources = [[int(x) - 50, int(z) - 50] for z, x in c.src]
+    receivers = [[[int(x) - 50 for x in r[:, 1]], [int(z) - 50 for z in r[:, 0]]] for r in c.rec]
+    sio.write_pkl(tmp_path / "s.pkl", sources)
+    sio.write_pkl(tmp_path / "r.pkl", receivers)
+    cfg = {"equation": "elastic",
+           "geom": {"truePath": {k: v[0] for k, v in paths.items()},
+                    "initPath": {k: v[1] for k, v in paths.items()},
+                    "sources": str(tmp_path / "s.pkl"), "receivers": str(tmp_path / "r.pkl"),
+                    "obsPath": str(tmp_path / "obs.npy"), "wavelet": None, "fm": 20.0,
+                    "dt": c.dt, "nt": c.nt, "h": c.h, "wavelet_delay": 40,
+                    "wavelet_inverse": False, "multiple": False,
+                    "boundary": {"type": "pml", "width": 50}, "multiscale": ["all"],
+                    "invlist": {"vp": True, "vs": True, "rho": False},
+                    "source_type": ["txx", "tzz"], "receiver_type": ["vx", "vz"]},
+           "training": {"N_epochs": 3, "lr": {"vp": 10.0, "vs": 6.0}}}
+    # no records yet: model them from the true models
+    p = fwi.problem_from_config({**cfg, "geom": {**cfg["geom"], "obsPath": None}})
+    obs = p.records(p.models_true).cpu().numpy()
+    store = sio.SeisRecord(cfg)
+    store.setup("forward")
+    for i, shot in enumerate(sio.split_records(obs, [len(r[0]) for r in receivers])):
+        store[i] = shot
+    store.write()
+    p, models, hist = fwi.run_from_config(cfg, log=print)
+    assert hist.misfit[-1] < hist.misfit[0]
+    assert torch.equal(models["rho"], p.models_init["rho"])
