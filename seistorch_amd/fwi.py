@@ -400,7 +400,7 @@ def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smoot
     configuration: geom.multiscale bands, training.N_epochs, lr (a number or
     {"vp": lr}), scale_decay, lr_decay; the command-line switches
     --grad-smooth (training.smooth), --grad-cut (geom.seabed, an .npy mask of
-    the unpadded grid) and --source-illumination (acoustic equations).
+    the unpadded grid) and --source-illumination.
     Returns (Problem, vp, History); for the elastic equation (ElasticProblem,
     {name: padded model}, History)."""
     from . import io as sio
@@ -410,19 +410,19 @@ def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smoot
     tr = cfg.get("training", {})
     lr = tr.get("lr", 5.0)
     scales = cfg["geom"].get("multiscale") or ["all"]
-    if isinstance(p, ElasticProblem):
-        if grad_smooth or grad_cut or source_illumination:
-            raise NotImplementedError("gradient smoothing, cut and illumination are not wired "
-                                      "into the elastic loop")
-        models, hist = run_multiscale_elastic(p, scales, int(tr["N_epochs"]), lr,
-                                              optimizer=optimizer,
-                                              scale_decay=float(tr.get("scale_decay", 0.85)),
-                                              lr_decay=float(tr.get("lr_decay", 0.995)), log=log)
-        return p, models, hist
-    lr = float(lr["vp"] if isinstance(lr, dict) else lr)
     seabed = None
     if grad_cut:
         seabed = torch.from_numpy(sio.load_model(cfg["geom"]["seabed"])).to(device)
+    if isinstance(p, ElasticProblem):
+        models, hist = run_multiscale_elastic(p, scales, int(tr["N_epochs"]), lr,
+                                              optimizer=optimizer,
+                                              scale_decay=float(tr.get("scale_decay", 0.85)),
+                                              lr_decay=float(tr.get("lr_decay", 0.995)),
+                                              smooth=tr.get("smooth") if grad_smooth else None,
+                                              seabed=seabed,
+                                              source_illumination=source_illumination, log=log)
+        return p, models, hist
+    lr = float(lr["vp"] if isinstance(lr, dict) else lr)
     vp, hist = run_multiscale(p, scales, int(tr["N_epochs"]), lr=lr, optimizer=optimizer,
                               scale_decay=float(tr.get("scale_decay", 0.85)),
                               lr_decay=float(tr.get("lr_decay", 0.995)),
@@ -465,20 +465,21 @@ class ElasticProblem:
     _filter = Problem._filter
     observed = Problem.observed
 
-    def _propagate(self, m):
+    def _propagate(self, m, illum=None):
         from .equations2d import elastic as el
         return el.propagate(m["vp"], m["vs"], m["rho"], self.d, self.wavelet, self.src, self.rec,
                             self.B, self.dt, self.h, src_type=self.src_type,
-                            rec_type=self.rec_type)
+                            rec_type=self.rec_type, illum=illum)
 
     def records(self, m):
         with torch.no_grad():
             return self._propagate(m)
 
-    def gradient(self, m):
-        """(misfit, {name: interior gradient} for the inverted parameters)."""
+    def gradient(self, m, illum=None):
+        """(misfit, {name: interior gradient} for the inverted parameters);
+        illum: optional (nz, nx) buffer for the forward's source illumination."""
         mm = {k: (v.detach().clone().requires_grad_(k in self.invlist)) for k, v in m.items()}
-        loss = torch.sum((self._filter(self._propagate(mm)) - self.observed()) ** 2)
+        loss = torch.sum((self._filter(self._propagate(mm, illum)) - self.observed()) ** 2)
         loss.backward()
         g = {}
         for k in self.invlist:
@@ -493,13 +494,17 @@ class ElasticProblem:
 
 
 def run_multiscale_elastic(p, scales, epochs, lr, optimizer="adam", scale_decay=0.85,
-                           lr_decay=0.995, grad_clamp=True, log=None):
+                           lr_decay=0.995, grad_clamp=True, smooth=None, seabed=None,
+                           source_illumination=False, log=None):
     """run_multiscale for the elastic equation: one fresh optimiser per scale
     over the inverted parameters, each with its own learning rate (lr: a
     number or {name: lr}, the reference's --lr vp=.. vs=..), scaled by
     scale_decay**scale and decayed by lr_decay per epoch; Adam (eps 1e-22) or
     max-normalised steepest descent per parameter; each parameter clamped to
-    ELASTIC_BOUNDS after the update.  Returns (models, History)."""
+    ELASTIC_BOUNDS after the update; smooth, seabed and source_illumination
+    post-process every inverted parameter's gradient as in run_multiscale.
+    Returns (models, History)."""
+    from . import process
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     inv = list(p.invlist)
@@ -523,7 +528,15 @@ def run_multiscale_elastic(p, scales, epochs, lr, optimizer="adam", scale_decay=
             sched = torch.optim.lr_scheduler.ExponentialLR(opt, lr_decay) if opt else None
             step = {k: lrs[k] * scale_decay ** si for k in inv}
             for ep in range(epochs):
-                loss, g = p.gradient(models_of(xs))
+                illum = torch.empty_like(p.models_init["vp"]) if source_illumination else None
+                loss, g = p.gradient(models_of(xs), illum)
+                if smooth is not None or seabed is not None or illum is not None:
+                    il = p.interior(illum).contiguous() if illum is not None else None
+                    for k in inv:
+                        g[k] = process.smooth_gradient(
+                            g[k].contiguous(), smooth,
+                            pre_mask=seabed if smooth is not None else None, post_mask=seabed,
+                            illum=il)
                 if opt is not None:
                     opt.zero_grad()
                     for k in inv:

@@ -218,7 +218,11 @@ def test_run_from_config_elastic(tmp_path):
                     "boundary": {"type": "pml", "width": 50}, "multiscale": ["all"],
                     "invlist": {"vp": True, "vs": True, "rho": False},
                     "source_type": ["txx", "tzz"], "receiver_type": ["vx", "vz"]},
-           "training": {"N_epochs": 3, "lr": {"vp": 10.0, "vs": 6.0}}}
+           "training": {"N_epochs": 3, "lr": {"vp": 10.0, "vs": 6.0},
+                        "smooth": {"counts": 1, "sigma": {"z": 1.0, "x": 1.0},
+                                   "radius": {"z": 2, "x": 2}}}}
+    np.save(tmp_path / "seabed.npy", (np.arange(nz_i)[:, None] >= 6) * np.ones((1, nx_i)))
+    cfg["geom"]["seabed"] = str(tmp_path / "seabed.npy")
     # no records yet: model them from the true models
     p = fwi.problem_from_config({**cfg, "geom": {**cfg["geom"], "obsPath": None}})
     obs = p.records(p.models_true).cpu().numpy()
@@ -227,6 +231,10 @@ def test_run_from_config_elastic(tmp_path):
     for i, shot in enumerate(sio.split_records(obs, [len(r[0]) for r in receivers])):
         store[i] = shot
     store.write()
-    p, models, hist = fwi.run_from_config(cfg, log=print)
+    p, models, hist = fwi.run_from_config(cfg, grad_smooth=True, grad_cut=True,
+                                          source_illumination=True, log=print)
     assert hist.misfit[-1] < hist.misfit[0]
     assert torch.equal(models["rho"], p.models_init["rho"])
+    # above the seabed (6 rows) vp and vs keep their starting values
+    for k in ("vp", "vs"):
+        assert torch.equal(p.interior(models[k])[:6], p.interior(p.models_init[k])[:6])
