@@ -157,3 +157,17 @@ def test_full_size_properties():
     print(f"900x2500 constant after 5+5 passes: max deviation {dev:.2e}")
     assert dev <= 10 * 21 * 2.0 ** -24 * 2.5
     assert bool((cut[:100] == 0).all())
+
+
+@pytest.mark.parametrize("shape,rz,rx", [((61, 97), 2, 4), ((900, 2500), 10, 10), ((40, 300), 16, 0)])
+def test_one_count_equals_two_single_axis_passes(shape, rz, rx):
+    """One smoothing count of seis_smooth_gradient equals the z pass then the
+    x pass of the single-axis entry point bit for bit (same operands, same
+    order, same FP32 intermediate), at small and full sizes."""
+    from seistorch_amd import process as pp
+    g = torch.randn(shape, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+    sm = {"counts": 1, "sigma": {"z": 2.0, "x": 3.0}, "radius": {"z": rz, "x": rx}}
+    fused = pp.smooth_gradient(g, sm)
+    two = pp.gaussian_filter(pp.gaussian_filter(g, 2.0, rz, 0), 3.0, rx, 1)
+    print(f"one count {shape} r=({rz},{rx}): equal to two passes {torch.equal(fused, two)}")
+    assert torch.equal(fused, two)
