@@ -179,7 +179,7 @@ int seis_acoustic_backward_batch(int64_t n, const int64_t* shape, const float* d
                                  int32_t vec_x, int32_t use_graph, void* stream);
 
 /* ---------------- acoustic, constant density, HABC ----------------------- */
-/* b is the (nz, nx) HABC weight field (seistorch/habc.py:252-291); the strips
+/* b is the (nz, nx) HABC weight field (seistorch/habc.py:42-81); the strips
  * are 50 cells wide; multiple != 0 selects the free-surface variant (no top
  * strip).  Grids need at least 102 cells per axis. */
 

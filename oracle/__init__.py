@@ -14,7 +14,7 @@ Modules
                (seistorch/probe.py:42-44), the L2 misfit
                (seistorch/loss.py:409-421) and the hand-derived exact discrete
                adjoint gradient of that loop.
-``coeffs``     FP64 restatements of the PML profile (seistorch/pml.py:155-185),
+``coeffs``     FP64 restatements of the PML profile (seistorch/pml.py:29-59),
                the HABC weights/masks (seistorch/habc.py:4-81) and the Ricker
                wavelet (seistorch/utils.py:235-249).
 ``elastic``    FP64 numpy restatement of the velocity-stress step

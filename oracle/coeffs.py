@@ -16,7 +16,7 @@ import numpy as np
 def pml_profile(N=50, cp=1500.0, R=1e-4, order=2):
     """1-D PML damping profile, outermost cell first.
 
-    seistorch/pml.py:161-164: d0 = (1.5*cp/N)*log10(1/R);
+    seistorch/pml.py:35-38: d0 = (1.5*cp/N)*log10(1/R);
     d_vals = d0*linspace(0,1,N+1)**order, flipped so index 0 is the edge.
     """
     d0 = (1.5 * cp / N) * np.log10(1.0 / R)
@@ -38,8 +38,8 @@ def pml_axis(n, N=50, skip_low=False):
 def pml_coefficients_2d(nz, nx, N=50, multiple=False):
     """d(z, x) = sqrt(pz(z)^2 + px(x)^2); top strip omitted when multiple.
 
-    seistorch/pml.py:155-185 (domain_shape=(nz, nx) is unpacked as Nx, Ny and
-    transposed back at :182, so x and z both get the same profile).
+    seistorch/pml.py:29-59 (domain_shape=(nz, nx) is unpacked as Nx, Ny and
+    transposed back at :56, so x and z both get the same profile).
     """
     pz = pml_axis(nz, N, skip_low=multiple)
     px = pml_axis(nx, N)
@@ -66,13 +66,13 @@ def habc_masks(nz, nx, w=50):
 def habc_weights(w=50):
     """Blend weight per strip row, row 0 at the outer edge.
 
-    seistorch/habc.py:261-262: linspace(0, N, N)/N flipped, i.e. (N-1-i)/(N-1).
+    seistorch/habc.py:51-52: linspace(0, N, N)/N flipped, i.e. (N-1-i)/(N-1).
     """
     return np.linspace(0.0, w, w)[::-1] / w
 
 
 def habc_coefficients_2d(nz, nx, w=50, multiple=False):
-    """Weight field of seistorch/habc.py:252-291 (trapezoid-masked strips)."""
+    """Weight field of seistorch/habc.py:42-81 (trapezoid-masked strips)."""
     d = np.zeros((nz, nx))
     b = habc_weights(w)
     top, bottom, left, right = habc_masks(nz, nx, w)

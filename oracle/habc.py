@@ -87,6 +87,8 @@ def step(c, u, p, b, dt, h, multiple=False):
     from .acoustic import laplacian
     c = np.asarray(c, np.float64)
     b = np.asarray(b, np.float64)
+    u = np.asarray(u, np.float64)
+    p = np.asarray(p, np.float64)
     nz, nx = c.shape
     y = (1.0 / dt ** -2) * (2 / dt ** 2 * u - dt ** -2 * p + c ** 2 * laplacian(u, h))
     ys = {"top": one_way(u, p, c, dt, h, -2, True),

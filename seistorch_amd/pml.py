@@ -1,6 +1,6 @@
 """PML damping coefficients (host side, computed once per geometry).
 
-Same profile and FP32 arithmetic as seistorch/pml.py:155-185: the quadratic
+Same profile and FP32 arithmetic as seistorch/pml.py:29-59: the quadratic
 profile d0*linspace(0, 1, N+1)**2 flipped, d0 = (1.5*1500/N)*log10(1e4), laid
 along both ends of x and z (the top z strip omitted with a free surface) and
 combined as sqrt(dx^2 + dz^2).

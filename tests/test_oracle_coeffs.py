@@ -53,3 +53,43 @@ def test_ricker_closed_form():
     np.testing.assert_allclose(w[delay - 50:delay], w[delay + 50:delay:-1], rtol=1e-12)
     # minima at t = +-sqrt(3/2)/(pi fm) with value -2 exp(-3/2)
     assert abs(w.min() + 2 * np.exp(-1.5)) < 2e-3
+
+
+# ------------------------------------------------------- the product's generators
+
+def test_product_pml_matches_oracle():
+    """seistorch_amd.pml (FP32, the reference's op order) against the FP64
+    restatement pinned above, with and without a free surface."""
+    import torch
+    from seistorch_amd.pml import generate_pml_coefficients_2d
+    for mult in (False, True):
+        for nz, nx in ((300, 500), (151, 207)):
+            d = generate_pml_coefficients_2d((nz, nx), 50, multiple=mult)
+            assert d.dtype == torch.float32 and d.shape == (nz, nx)
+            ref = coeffs.pml_coefficients_2d(nz, nx, 50, multiple=mult)
+            np.testing.assert_allclose(d.numpy(), ref, rtol=2e-7, atol=1e-5)
+            assert np.all(d.numpy()[ref == 0.0] == 0.0)
+    d = generate_pml_coefficients_2d((300, 500), 50).numpy()
+    assert abs(d[0, 250] - 180.0) < 2e-5 and abs(d[49, 250] - 0.072) < 1e-7
+
+
+def test_product_habc_matches_oracle():
+    from seistorch_amd.habc import generate_habc_coefficients_2d
+    for mult in (False, True):
+        b = generate_habc_coefficients_2d((130, 170), 50, multiple=mult).numpy()
+        ref = coeffs.habc_coefficients_2d(130, 170, multiple=mult)
+        np.testing.assert_allclose(b, ref, rtol=0, atol=1e-7)
+
+
+def test_product_ricker_matches_oracle():
+    from seistorch_amd.utils import ricker_wave
+    for fm, dt, nt, delay in ((10.0, 1e-3, 1500, 256), (8.0, 8e-4, 5000, 200),
+                              (25.0, 5e-4, 300, 40)):
+        w = ricker_wave(fm, dt, nt, delay=delay).numpy()
+        ref = coeffs.ricker(fm, dt, nt, delay=delay)
+        assert w.dtype == np.float32
+        np.testing.assert_allclose(w, ref, rtol=0, atol=6e-8)
+        assert w[delay] == 1.0
+    wi = ricker_wave(10.0, 1e-3, 300, delay=80, inverse=True).numpy()
+    np.testing.assert_allclose(wi, coeffs.ricker(10.0, 1e-3, 300, delay=80, inverse=True),
+                               rtol=0, atol=6e-8)
