@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 10
+#define SEIS_ABI_VERSION 11
 
 #ifdef __cplusplus
 extern "C" {
@@ -47,9 +47,13 @@ int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
 
 /* Number of cells in the boundary-saving ring of an (nz, nx) grid: rows
  * npml and nz-1-npml, columns npml and nx-1-npml, without the corners
- * (the 5-point neighbours of the undamped interior).  -1 if the grid has no
- * interior.  Replaces the ring shape of seistorch/equations2d/utils.py:50-69. */
-int64_t seis_ring_len(int64_t nz, int64_t nx, int32_t npml);
+ * (the 5-point neighbours of the undamped interior).  multiple != 0 is the
+ * free surface: no top boundary, so the interior reaches row 0, the top row
+ * npml is not part of the ring and the columns run from row 0 to nz-2-npml
+ * (restore_boundaries(..., multiple=True) leaves the top row alone,
+ * seistorch/equations2d/utils.py:71-78).  -1 if the grid has no interior.
+ * Replaces the ring shape of seistorch/equations2d/utils.py:50-69. */
+int64_t seis_ring_len(int64_t nz, int64_t nx, int32_t npml, int32_t multiple);
 
 /* Number of field-pair snapshots the forward stores: floor((nt-2)/snap_every)
  * (0 if snap_every <= 0).  No reference counterpart: the reference keeps the
@@ -100,14 +104,16 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
  * NULL, or ceil(B/shots_per_group) more (nz, nx) planes with which the
  * two-step launches accumulate the illumination themselves (per shot group,
  * summed at the end: the same sum in another order) instead of a pass over
- * every shot's field per time level. */
+ * every shot's field per time level.  multiple != 0: free surface (the
+ * reference's geom.multiple, seistorch/pml.py:49-53 damps no top strip), the
+ * ring is that of seis_ring_len(nz, nx, npml, 1), without a top row. */
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
                           float* illum, float* illum_part, float* u_a, float* u_b, float* u_c,
                           float* u_d, int64_t B, int64_t nz, int64_t nx, float dt, float h,
-                          int32_t npml, int32_t shots_per_group, int32_t rows_per_thread,
-                          int32_t vec_x, void* stream);
+                          int32_t npml, int32_t multiple, int32_t shots_per_group,
+                          int32_t rows_per_thread, int32_t vec_x, void* stream);
 
 /* Adjoint of seis_acoustic_forward (replaces the boundary-saving backward of
  * seistorch/checkpoint_new.py:146-217 and acoustic.py:88-137, with the exact
@@ -116,7 +122,10 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
  * snap_every; u_a/u_b hold the final forward state (consumed); mu_a, mu_b are
  * (B, nz, nx) work buffers;
  * grad_part is ceil(B/shots_per_group) x (nz, nx) work; grad (nz, nx) gets
- * dJ/dvp on the interior [npml+1, n-npml-1) and zero elsewhere.
+ * dJ/dvp on the interior [npml+1, n-npml-1) and zero elsewhere; with
+ * multiple != 0 (free surface, as in the forward) the interior rows are
+ * [0, nz-npml-1), the region that seistorch/equations2d/acoustic.py:114-137
+ * rebuilds when it leaves the top ring row alone.
  * If u_c, u_d, mu_c, mu_d (four more (B, nz, nx) buffers) and, when R > 0,
  * rec_map are given, two reverse steps run per launch where the ring is
  * available: rec_map (nz x nx int32) holds each cell's receiver index within
@@ -131,8 +140,8 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            float* mu_a, float* mu_b, float* u_c, float* u_d, float* mu_c,
                            float* mu_d, const int32_t* rec_map, float* grad_part, float* grad,
                            int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                           int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
-                           void* stream);
+                           int32_t multiple, int32_t shots_per_group, int32_t rows_per_thread,
+                           int32_t vec_x, void* stream);
 
 /* ---------------- acoustic PML, uneven multi-model batch ------------------ */
 /* n independent acoustic PML problems advanced together: every time step is
@@ -151,7 +160,7 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
  *   src_zx               (B, 2) int32
  *   rec_flat             (R,) int64, flat indices into the instance's (B, nz, nx)
  *   rec_out, rec_grad    (nt, R)
- *   ring                 (nt-1, B, seis_ring_len(nz, nx, npml))
+ *   ring                 (nt-1, B, seis_ring_len(nz, nx, npml, 0))
  *   snaps                (seis_snapshot_count(nt, snap_every), 2, B, nz, nx)
  *   u_a, u_b, mu_a, mu_b (B, nz, nx)
  *   grad_part            (ceil(B/shots_per_group), nz, nx)
@@ -306,7 +315,7 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
                          float* ring, float* snaps, int32_t snap_every, float* illum,
                          int32_t illum_field, float* const* fields, float* const* scratch,
                          int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                         int32_t shots_per_group, void* stream);
+                         int32_t multiple, int32_t shots_per_group, void* stream);
 
 /* Adjoint of seis_elastic_forward with boundary-saving reconstruction
  * (replaces seistorch/checkpoint.py:108-230 and elastic.py:39-122 with the
@@ -324,7 +333,7 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
                           int32_t snap_every, float* const* fields, float* const* adj_a,
                           float* const* adj_b, float* const* scratch, float* grad_coef, int64_t B,
                           int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                          int32_t shots_per_group, void* stream);
+                          int32_t multiple, int32_t shots_per_group, void* stream);
 
 /* ---------------- multiscale record filtering ----------------------------- */
 /* Zero-phase IIR filter of traces along time: replaces seistorch/signal.py:91-96

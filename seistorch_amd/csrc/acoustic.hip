@@ -431,7 +431,8 @@ __device__ __forceinline__ void bwd_tile(
     xin[e] = x0 + e > g.npml && x0 + e < g.nx - 1 - g.npml;
     anyx = anyx || xin[e];
   }
-  const int urow0 = g.npml, urow1 = g.nz - 1 - g.npml;  // rows of u^{k-1} that are read
+  // rows of u^{k-1} that are read: the interior's and its ring rows
+  const int urow0 = max(g.ztop, 0), urow1 = g.nz - 1 - g.npml;
   // model: c^2 on the column rows zb-1 .. zb+RZ and at the vector's two x-ends
   float c2col[RZ + 2][VX], c2L[RZ], c2R[RZ], cc[RZ][VX], A[RZ][VX], Bc[RZ][VX];
 #pragma unroll
@@ -497,7 +498,7 @@ __device__ __forceinline__ void bwd_tile(
           mu[e] = acoustic_update(A[r][e], Bc[r][e], k.two_idt2, mcol[r + 1][e], mu[e], 1.0f, Lm);
         }
         vstore<VX>(M2 + i, mu);
-        if (anyx && z > g.npml && z < g.nz - 1 - g.npml) {
+        if (anyx && interior_row(z, g)) {
           const float uL = x0 > 0 ? U[i - 1] : 0.0f;
           const float uR = x0 + VX < g.nx ? U[i + VX] : 0.0f;
           float uv[VX];
@@ -529,7 +530,7 @@ __device__ __forceinline__ void bwd_tile(
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int z = zb + r;
-    if (anyx && z > g.npml && z < g.nz - 1 - g.npml) {
+    if (anyx && interior_row(z, g)) {
       float gv[VX];
       vload<VX>(G + z * g.nx + x0, gv);
 #pragma unroll
@@ -598,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2(
   const bool xin = x0 >= 0 && x0 < g.nx;  // nx is a multiple of VX
   const bool inner = lane > 0 && lane < kLanes - 1;
   const float wA = wn[1], wB = wn[0];  // w[kk-1], w[kk-2]
-  const int ur0 = g.npml, ur1 = g.nz - 1 - g.npml;  // rows of u that are read
+  const int ur0 = max(g.ztop, 0), ur1 = g.nz - 1 - g.npml;  // rows of u that are read
   bool xint[VX];
   bool anyi = false, anyu = false;
 #pragma unroll
@@ -698,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2(
       const int i = z * g.nx + x0;
       float m2v[VX];
       vload<VX>(mb + off + i, m2v);
-      const bool zi = z > g.npml && z < g.nz - 1 - g.npml;
+      const bool zi = interior_row(z, g);
       float ukk[VX];
       if (anyi && zi) vload<VX>(ub + off + i, ukk);
       else vzero<VX>(ukk);
@@ -739,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2(
       const float ur = __shfl_down(UA[q][0], 1, kLanes);
       if (!inner || !xin || z >= g.nz) continue;
       const int i = z * g.nx + x0;
-      const bool zi = z > g.npml && z < g.nz - 1 - g.npml;
+      const bool zi = interior_row(z, g);
       float mo[VX], uo[VX];
 #pragma unroll
       for (int e = 0; e < VX; ++e) {
@@ -770,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2(
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int z = zb + r;
-    if (z > g.npml && z < g.nz - 1 - g.npml) {
+    if (interior_row(z, g)) {
       float gv[VX];
       vload<VX>(G + z * g.nx + x0, gv);
 #pragma unroll
@@ -888,7 +889,7 @@ __device__ __forceinline__ int find_row(const int64_t* __restrict__ tab, int n_a
 }
 
 __device__ __forceinline__ Grid row_grid(const int64_t* r, int npml) {
-  return Grid{(int)r[kCNz], (int)r[kCNx], npml, r[kCNz] * r[kCNx]};
+  return seis::make_grid(r[kCNz], r[kCNx], npml, false);
 }
 
 __device__ __forceinline__ seis::TimeConsts row_consts(const int64_t* r) {
@@ -1236,12 +1237,12 @@ int make_plan(BatchPlan& p, int64_t n, const int64_t* shape, const float* dt_h, 
                    (long long)nt, (long long)R);
     SEIS_CHECK_ARG(nx % VX == 0, "instance %lld: vec_x %d does not divide nx = %lld",
                    (long long)q, VX, (long long)nx);
-    const int64_t L = seis_ring_len(nz, nx, npml);
+    const int64_t L = seis_ring_len(nz, nx, npml, 0);
     SEIS_CHECK_ARG(!need_ring || L > 0, "instance %lld: grid %lld x %lld has no interior "
                    "inside npml=%d", (long long)q, (long long)nz, (long long)nx, npml);
     const int64_t plane = nz * nx;
     const int64_t G = (B + S - 1) / S;
-    const Grid g{(int)nz, (int)nx, npml, plane};
+    const Grid g = seis::make_grid(nz, nx, npml, false);
     const TileMap tm = make_tiles(g, VX, RZ, (int)G);
     const seis::TimeConsts k = seis::make_consts(dt, h);
     int64_t* r = rows.data() + q * kBatchCols;
@@ -1434,11 +1435,11 @@ int run_maybe_graph(bool use_graph, hipStream_t s, F enqueue) {
 
 extern "C" {
 
-int64_t seis_ring_len(int64_t nz, int64_t nx, int32_t npml) {
+int64_t seis_ring_len(int64_t nz, int64_t nx, int32_t npml, int32_t multiple) {
   const int64_t nxi = nx - 2 * (int64_t)npml - 2;
-  const int64_t nzi = nz - 2 * (int64_t)npml - 2;
+  const int64_t nzi = multiple ? nz - (int64_t)npml - 1 : nz - 2 * (int64_t)npml - 2;
   if (npml < 0 || nxi < 1 || nzi < 1) return -1;
-  return 2 * nxi + 2 * nzi;
+  return (multiple ? 1 : 2) * nxi + 2 * nzi;
 }
 
 int64_t seis_snapshot_count(int64_t nt, int32_t snap_every) {
@@ -1451,7 +1452,7 @@ int seis_acoustic_step(const float* vp, const float* d, const float* h1, const f
                        void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
   SEIS_CHECK_ARG(vp && d && h1 && h2 && y, "null pointer argument");
-  const Grid g{(int)nz, (int)nx, 0, nz * nx};
+  const Grid g = seis::make_grid(nz, nx, 0, false);
   hipStream_t s = (hipStream_t)stream;
   k_step<<<tile_grid(g, (int)B), dim3(kTX, kTZ), 0, s>>>(vp, d, h1, h2, y, g,
                                                          seis::make_consts(dt, h));
@@ -1464,7 +1465,7 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
                            int64_t nz, int64_t nx, float dt, float h, void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
   SEIS_CHECK_ARG(vp && d && h1 && gy && gh1 && gh2, "null pointer argument");
-  const Grid g{(int)nz, (int)nx, 0, nz * nx};
+  const Grid g = seis::make_grid(nz, nx, 0, false);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   k_step_vjp_fields<<<tile_grid(g, (int)B), dim3(kTX, kTZ), 0, s>>>(vp, d, gy, gh1_pass, gh1,
@@ -1483,8 +1484,8 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
                           float* rec_out, float* ring, float* snaps, int32_t snap_every,
                           float* illum, float* illum_part, float* u_a, float* u_b, float* u_c,
                           float* u_d, int64_t B, int64_t nz, int64_t nx, float dt, float h,
-                          int32_t npml, int32_t shots_per_group, int32_t rows_per_thread,
-                          int32_t vec_x, void* stream) {
+                          int32_t npml, int32_t multiple, int32_t shots_per_group,
+                          int32_t rows_per_thread, int32_t vec_x, void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
   SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, vec_x),
                  "launch shape (shots_per_group %d, rows_per_thread %d, vec_x %d) not in "
@@ -1499,11 +1500,11 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
   SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1");
-  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml) : 0;
+  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml, multiple) : 0;
   SEIS_CHECK_ARG(!ring || ring_len > 0, "grid %lld x %lld has no interior inside npml=%d",
                  (long long)nz, (long long)nx, npml);
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
-  const Grid g{(int)nz, (int)nx, npml, nz * nx};
+  const Grid g = seis::make_grid(nz, nx, npml, multiple != 0);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int S = shots_per_group;
@@ -1565,8 +1566,8 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                            float* mu_a, float* mu_b, float* u_c, float* u_d, float* mu_c,
                            float* mu_d, const int32_t* rec_map, float* grad_part, float* grad,
                            int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                           int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
-                           void* stream) {
+                           int32_t multiple, int32_t shots_per_group, int32_t rows_per_thread,
+                           int32_t vec_x, void* stream) {
   if (int e = check_grid(B, nz, nx, dt, h)) return e;
   SEIS_CHECK_ARG(valid_shape(shots_per_group, rows_per_thread, vec_x),
                  "launch shape (shots_per_group %d, rows_per_thread %d, vec_x %d) not in "
@@ -1583,12 +1584,12 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_grad)), "bad receiver arguments");
   SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1");
-  const int64_t ring_len = seis_ring_len(nz, nx, npml);
+  const int64_t ring_len = seis_ring_len(nz, nx, npml, multiple);
   SEIS_CHECK_ARG(ring_len > 0, "grid %lld x %lld has no interior inside npml=%d", (long long)nz,
                  (long long)nx, npml);
   SEIS_CHECK_ARG(ring || nt <= 2, "ring storage is required for nt > 2");
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
-  const Grid g{(int)nz, (int)nx, npml, nz * nx};
+  const Grid g = seis::make_grid(nz, nx, npml, multiple != 0);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int S = shots_per_group;

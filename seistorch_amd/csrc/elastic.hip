@@ -1113,7 +1113,7 @@ int seis_elastic_stage(int32_t stage, const float* coef, float* const* fields, f
   SEIS_CHECK_ARG(coef && fields, "null pointer argument");
   for (int f = 0; f < 5; ++f) SEIS_CHECK_ARG(fields[f], "null field pointer");
   SEIS_CHECK_ARG(stage == 0 || stage == 1, "stage must be 0 (stresses) or 1 (velocities)");
-  const Grid g{(int)nz, (int)nx, 0, nz * nx};
+  const Grid g = seis::make_grid(nz, nx, 0, false);
   const int S = shots_per_group;
   const int VX = stage_vx(nx, coef, {fields});
   const TileMap tm = make_tiles(g, VX, 1, (int)((B + S - 1) / S));
@@ -1139,7 +1139,7 @@ int seis_elastic_stage_vjp(int32_t stage, const float* coef, float* const* cot_i
   for (int f = 0; f < 5; ++f)
     SEIS_CHECK_ARG(cot_in[f] && cot_out[f] && fwd[f], "null field pointer");
   SEIS_CHECK_ARG(stage == 0 || stage == 1, "stage must be 0 (stresses) or 1 (velocities)");
-  const Grid g{(int)nz, (int)nx, 0, nz * nx};
+  const Grid g = seis::make_grid(nz, nx, 0, false);
   const int S = shots_per_group;
   const int VX = stage_vx(nx, coef, {cot_in, cot_out, fwd}, grad_coef);
   const TileMap tm = make_tiles(g, VX, 1, (int)((B + S - 1) / S));
@@ -1176,7 +1176,7 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
                          float* ring, float* snaps, int32_t snap_every, float* illum,
                          int32_t illum_field, float* const* fields, float* const* scratch,
                          int64_t B, int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                         int32_t shots_per_group, void* stream) {
+                         int32_t multiple, int32_t shots_per_group, void* stream) {
   if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
   SEIS_CHECK_ARG(coef && wavelet && src_zx && fields, "null pointer argument");
   SEIS_CHECK_ARG(!illum || (illum_field >= 0 && illum_field < 5),
@@ -1185,10 +1185,10 @@ int seis_elastic_forward(const float* coef, const float* wavelet, int64_t nt,
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && nch >= 1 && (R == 0 || (rec_flat && rec_field && rec_out)),
                  "bad receiver arguments");
-  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml) : 0;
+  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml, multiple) : 0;
   SEIS_CHECK_ARG(!ring || ring_len > 0, "grid has no interior inside npml=%d", npml);
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
-  const Grid g{(int)nz, (int)nx, npml, nz * nx};
+  const Grid g = seis::make_grid(nz, nx, npml, multiple != 0);
   const int S = shots_per_group;
   if (scratch)
     for (int f = 0; f < 5; ++f) SEIS_CHECK_ARG(scratch[f], "null scratch field pointer");
@@ -1255,7 +1255,7 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
                           int32_t snap_every, float* const* fields, float* const* adj_a,
                           float* const* adj_b, float* const* scratch, float* grad_coef, int64_t B,
                           int64_t nz, int64_t nx, float dt, float h, int32_t npml,
-                          int32_t shots_per_group, void* stream) {
+                          int32_t multiple, int32_t shots_per_group, void* stream) {
   if (int e = check(B, nz, nx, dt, h, shots_per_group)) return e;
   SEIS_CHECK_ARG(coef && wavelet && src_zx && fields && adj_a && adj_b && grad_coef,
                  "null pointer argument");
@@ -1266,11 +1266,11 @@ int seis_elastic_backward(const float* coef, const float* wavelet, int64_t nt,
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && nch >= 1 && (R == 0 || (rec_flat && rec_field && rec_grad)),
                  "bad receiver arguments");
-  const int64_t ring_len = seis_ring_len(nz, nx, npml);
+  const int64_t ring_len = seis_ring_len(nz, nx, npml, multiple);
   SEIS_CHECK_ARG(ring_len > 0, "grid has no interior inside npml=%d", npml);
   SEIS_CHECK_ARG(ring || nt <= 1, "ring storage is required for nt > 1");
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
-  const Grid g{(int)nz, (int)nx, npml, nz * nx};
+  const Grid g = seis::make_grid(nz, nx, npml, multiple != 0);
   const int S = shots_per_group;
   const int G = (int)((B + S - 1) / S);
   const int VX = scratch ? stage_vx(nx, coef, {fields, adj_a, adj_b, scratch}, grad_coef)

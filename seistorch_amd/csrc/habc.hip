@@ -849,7 +849,7 @@ int seis_habc_step(const float* vp, const float* b, const float* h1, const float
                    void* stream) {
   if (int e = habc_check(B, nz, nx, dt, h, 1)) return e;
   SEIS_CHECK_ARG(vp && b && h1 && h2 && y, "null pointer argument");
-  const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, dt, h};
+  const HabcCtx hc{seis::make_grid(nz, nx, 0, false), multiple ? 1 : 0, dt, h};
   const int S = B >= 4 ? 4 : 1;
   const TileMap tm = make_tiles(hc.g, 1, 1, (int)((B + S - 1) / S));
   hipStream_t s = (hipStream_t)stream;
@@ -863,7 +863,7 @@ int seis_habc_step_vjp(const float* vp, const float* b, const float* h1, const f
                        int64_t nx, float dt, float h, int32_t multiple, void* stream) {
   if (int e = habc_check(B, nz, nx, dt, h, 1)) return e;
   SEIS_CHECK_ARG(vp && b && h1 && h2 && gy && gh1 && gh2, "null pointer argument");
-  const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, dt, h};
+  const HabcCtx hc{seis::make_grid(nz, nx, 0, false), multiple ? 1 : 0, dt, h};
   const int S = B >= 4 ? 4 : 1;
   const TileMap tm = make_tiles(hc.g, 1, 1, (int)((B + S - 1) / S));
   const seis::TimeConsts k = seis::make_consts(dt, h);
@@ -891,10 +891,10 @@ int seis_habc_forward(const float* vp, const float* b, const float* wavelet, int
                  "illum must not alias a field buffer");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
-  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml) : 0;
+  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml, multiple) : 0;
   SEIS_CHECK_ARG(!ring || ring_len > 0, "grid has no interior inside npml=%d", npml);
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
-  const HabcCtx hc{Grid{(int)nz, (int)nx, npml, nz * nx}, multiple ? 1 : 0, dt, h};
+  const HabcCtx hc{seis::make_grid(nz, nx, npml, multiple != 0), multiple ? 1 : 0, dt, h};
   const Grid& g = hc.g;
   const int S = shots_per_group;
   // the deep interior (kW cells from every edge, outside the strips) runs vectorised
@@ -955,7 +955,7 @@ int seis_rho_habc_step(const float* vp, const float* rho, const float* b, const 
                        double h, int32_t multiple, int32_t blend, void* stream) {
   if (int e = habc_check(B, nz, nx, (float)dt, (float)h, 1)) return e;
   SEIS_CHECK_ARG(vp && rho && h1 && h2 && y && (b || !blend), "null pointer argument");
-  const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, (float)dt, (float)h};
+  const HabcCtx hc{seis::make_grid(nz, nx, 0, false), multiple ? 1 : 0, (float)dt, (float)h};
   const int spt = B < kRhoShots ? (int)B : kRhoShots;
   const TileMap tm = make_tiles(hc.g, 1, 1, (int)((B + spt - 1) / spt));
   const float C = (float)(dt * dt * std::pow(h, -2.0));
@@ -981,7 +981,7 @@ int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
                  "illum must not alias a field buffer");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
-  const HabcCtx hc{Grid{(int)nz, (int)nx, 0, nz * nx}, multiple ? 1 : 0, (float)dt, (float)h};
+  const HabcCtx hc{seis::make_grid(nz, nx, 0, false), multiple ? 1 : 0, (float)dt, (float)h};
   const Grid& g = hc.g;
   const int S = shots_per_group;
   const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
@@ -1033,11 +1033,11 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
                  "null pointer argument");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_grad)), "bad receiver arguments");
-  const int64_t ring_len = seis_ring_len(nz, nx, npml);
+  const int64_t ring_len = seis_ring_len(nz, nx, npml, multiple);
   SEIS_CHECK_ARG(ring_len > 0, "grid has no interior inside npml=%d", npml);
   SEIS_CHECK_ARG(ring || nt <= 2, "ring storage is required for nt > 2");
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
-  const HabcCtx hc{Grid{(int)nz, (int)nx, npml, nz * nx}, multiple ? 1 : 0, dt, h};
+  const HabcCtx hc{seis::make_grid(nz, nx, npml, multiple != 0), multiple ? 1 : 0, dt, h};
   const Grid& g = hc.g;
   const int S = shots_per_group;
   const int G = (int)((B + S - 1) / S);

@@ -7,46 +7,69 @@
 
 namespace seis {
 
-// Padded (nz, nx) grid; npml is the ring index of the boundary saving.
+// Padded (nz, nx) grid; npml is the ring index of the boundary saving at the
+// bottom, left and right edges.  ztop is the ring row at the top: npml, or -1
+// for a free surface (seistorch's `multiple` mode: no top boundary, so the
+// undamped region reaches row 0 and the top ring row is neither stored nor
+// restored, seistorch/equations2d/utils.py:71-78, elastic.py:132-134).
 struct Grid {
   int nz, nx;
   int npml;
   int64_t plane;  // nz*nx
+  int ztop;
 };
 
-// Undamped interior [npml+1, n-npml-1) in both axes.
-__device__ __forceinline__ bool interior(int z, int x, const Grid g) {
-  return z > g.npml && z < g.nz - 1 - g.npml && x > g.npml && x < g.nx - 1 - g.npml;
+__host__ __device__ inline Grid make_grid(int64_t nz, int64_t nx, int npml, bool free_surface) {
+  return Grid{(int)nz, (int)nx, npml, nz * nx, free_surface ? -1 : npml};
 }
 
-// j-th cell of the boundary-saving ring: top row, bottom row, left column,
-// right column, each restricted to the 5-point neighbours of the interior.
+// Rows of the undamped interior: (ztop, nz-npml-1).
+__device__ __forceinline__ bool interior_row(int z, const Grid g) {
+  return z > g.ztop && z < g.nz - 1 - g.npml;
+}
+
+// Undamped interior: rows (ztop, nz-npml-1), columns (npml, nx-npml-1).
+__device__ __forceinline__ bool interior(int z, int x, const Grid g) {
+  return interior_row(z, g) && x > g.npml && x < g.nx - 1 - g.npml;
+}
+
+// Ring sizes: nxi cells in each of the top and bottom rows (no top row with a
+// free surface), nzi cells in each of the left and right columns.
+__host__ __device__ inline int ring_nxi(const Grid g) { return g.nx - 2 * g.npml - 2; }
+__host__ __device__ inline int ring_nzi(const Grid g) { return g.nz - g.npml - 2 - g.ztop; }
+__host__ __device__ inline int ring_ntop(const Grid g) { return g.ztop >= 0 ? ring_nxi(g) : 0; }
+
+// j-th cell of the boundary-saving ring: top row (absent with a free
+// surface), bottom row, left column, right column, each restricted to the
+// 5-point neighbours of the interior.
 __device__ __forceinline__ int64_t ring_cell(int64_t j, const Grid g) {
-  const int nxi = g.nx - 2 * g.npml - 2;
-  const int nzi = g.nz - 2 * g.npml - 2;
+  const int nxi = ring_nxi(g);
+  const int nzi = ring_nzi(g);
+  const int nt = ring_ntop(g);
   int z, x;
-  if (j < nxi) {
-    z = g.npml; x = g.npml + 1 + (int)j;
-  } else if (j < 2 * nxi) {
-    z = g.nz - 1 - g.npml; x = g.npml + 1 + (int)(j - nxi);
-  } else if (j < 2 * nxi + nzi) {
-    z = g.npml + 1 + (int)(j - 2 * nxi); x = g.npml;
+  if (j < nt) {
+    z = g.ztop; x = g.npml + 1 + (int)j;
+  } else if (j < nt + nxi) {
+    z = g.nz - 1 - g.npml; x = g.npml + 1 + (int)(j - nt);
+  } else if (j < nt + nxi + nzi) {
+    z = g.ztop + 1 + (int)(j - nt - nxi); x = g.npml;
   } else {
-    z = g.npml + 1 + (int)(j - 2 * nxi - nzi); x = g.nx - 1 - g.npml;
+    z = g.ztop + 1 + (int)(j - nt - nxi - nzi); x = g.nx - 1 - g.npml;
   }
   return (int64_t)z * g.nx + x;
 }
 
 // Ring index j of cell (z, x), the inverse of ring_cell, or -1 off the ring.
 __device__ __forceinline__ int64_t ring_index(int z, int x, const Grid g) {
-  const int nxi = g.nx - 2 * g.npml - 2;
-  const int nzi = g.nz - 2 * g.npml - 2;
+  const int nxi = ring_nxi(g);
+  const int nzi = ring_nzi(g);
+  const int nt = ring_ntop(g);
   const bool xr = x > g.npml && x < g.nx - 1 - g.npml;
-  const bool zr = z > g.npml && z < g.nz - 1 - g.npml;
-  if (z == g.npml && xr) return x - g.npml - 1;
-  if (z == g.nz - 1 - g.npml && xr) return nxi + (x - g.npml - 1);
-  if (x == g.npml && zr) return 2 * nxi + (z - g.npml - 1);
-  if (x == g.nx - 1 - g.npml && zr) return 2 * (int64_t)nxi + nzi + (z - g.npml - 1);
+  const bool zr = interior_row(z, g);
+  if (nt > 0 && z == g.ztop && xr) return x - g.npml - 1;
+  if (z == g.nz - 1 - g.npml && xr) return nt + (x - g.npml - 1);
+  if (x == g.npml && zr) return nt + nxi + (z - g.ztop - 1);
+  if (x == g.nx - 1 - g.npml && zr) return (int64_t)nt + nxi + nzi + (z - g.ztop - 1);
   return -1;
 }
 

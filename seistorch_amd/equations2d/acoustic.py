@@ -99,8 +99,17 @@ def _time_step_backward_multiple(*args, **kwargs):
 
 # ----------------------------------------------------------------- fused loop
 
-def ring_len(nz, nx, npml=NPML):
-    return int(_lib.load().seis_ring_len(nz, nx, npml))
+def ring_len(nz, nx, npml=NPML, multiple=False):
+    """Cells of the boundary-saving ring; multiple: free surface, no top row."""
+    return int(_lib.load().seis_ring_len(nz, nx, npml, int(bool(multiple))))
+
+
+def check_finite(rec):
+    """The reference's loop raises ValueError on NaN records (rnn.py:215 with
+    type.py:41-46); a time step beyond the stability limit shows up here."""
+    if not bool(torch.isfinite(rec).all()):
+        raise ValueError("the records contain non-finite values (is dt within the CFL limit?)")
+    return rec
 
 
 SNAP_MIN_EVERY = 50        # densest snapshot spacing chosen automatically
@@ -147,8 +156,11 @@ def _launch_params(B, nz, nx, shape):
     return tuple(int(v if v is not None else dv) for v, dv in zip(shape, default))
 
 
-def _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad):
-    """Host-side bounds checks: the kernels index with these values unchecked."""
+def _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad, multiple=False):
+    """Host-side bounds checks: the kernels index with these values unchecked.
+    With a gradient the damping must vanish on the undamped interior that the
+    adjoint rebuilds: rows [50, nz-50), or [0, nz-50) with a free surface
+    (multiple), and columns [50, nx-50)."""
     nz, nx = vp.shape
     if vp.dim() != 2 or d.shape != vp.shape:
         raise RuntimeError(f"vp {tuple(vp.shape)} and d {tuple(d.shape)} must be equal (nz, nx)")
@@ -169,8 +181,10 @@ def _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad):
         m = NPML + 1
         if nz <= 2 * m or nx <= 2 * m:
             raise RuntimeError(f"grid {nz}x{nx} has no interior inside the 50-cell boundary")
-        if bool(d[m:nz - m, m:nx - m].abs().max() != 0):
-            raise RuntimeError("boundary-saving adjoint needs d == 0 inside the 50-cell boundary")
+        z0 = 0 if multiple else m
+        if bool(d[z0:nz - m, m:nx - m].abs().max() != 0):
+            raise RuntimeError("boundary-saving adjoint needs d == 0 inside the 50-cell boundary"
+                               + (" (free surface: from row 0)" if multiple else ""))
 
 
 def shared_receiver_map(rec_flat, B, nz, nx):
@@ -216,7 +230,7 @@ class AcousticLoop(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shape, snapshot_every,
-                two_step=True, two_step_adjoint=False, illum=None):
+                two_step=True, two_step_adjoint=False, illum=None, multiple=False):
         _lib.require_device(vp, d, wavelet, src_zx, rec_flat, illum)
         nz, nx = vp.shape
         if illum is not None and (illum.shape != (nz, nx) or illum.dtype != torch.float32
@@ -226,13 +240,13 @@ class AcousticLoop(torch.autograd.Function):
         R = rec_flat.shape[0]
         S, rz, vx = _launch_params(B, nz, nx, shape)
         need_grad = ctx.needs_input_grad[0]
-        _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad)
+        _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad, multiple)
         u_a = torch.empty((B, nz, nx), device=vp.device, dtype=torch.float32)
         u_b = torch.empty_like(u_a)
         rec = torch.empty((nt, R), device=vp.device, dtype=torch.float32)
         ring = None
         if need_grad and nt > 1:
-            L = ring_len(nz, nx)
+            L = ring_len(nz, nx, multiple=multiple)
             if L <= 0:
                 raise RuntimeError(f"grid {nz}x{nx} has no interior inside the 50-cell boundary")
             ring = torch.empty((nt - 1, B, L), device=vp.device, dtype=torch.float32)
@@ -256,13 +270,14 @@ class AcousticLoop(torch.autograd.Function):
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K, _lib.ptr(illum), _lib.ptr(ipart), _lib.ptr(u_a),
                   _lib.ptr(u_b),
-                  _lib.ptr(u_c), _lib.ptr(u_d), B, nz, nx, dt, h, NPML, S, rz, vx,
-                  _lib.stream_of(vp))
+                  _lib.ptr(u_c), _lib.ptr(u_d), B, nz, nx, dt, h, NPML, int(multiple), S, rz,
+                  vx, _lib.stream_of(vp))
         del u_c, u_d  # scratch of the two-step launches
+        check_finite(rec)
         if need_grad:
             ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
             ctx.state = (ring, snaps, u_a, u_b)
-            ctx.meta = (B, dt, h, S, K, rz, vx, bool(two_step_adjoint))
+            ctx.meta = (B, dt, h, S, K, rz, vx, bool(two_step_adjoint), int(multiple))
         return rec
 
     @staticmethod
@@ -271,7 +286,7 @@ class AcousticLoop(torch.autograd.Function):
         vp, d, wavelet, src_zx, rec_flat = ctx.saved_tensors
         ring, snaps, u_a, u_b = ctx.state
         ctx.state = None  # the backward consumes the final forward state
-        B, dt, h, S, K, rz, vx, two_step_adjoint = ctx.meta
+        B, dt, h, S, K, rz, vx, two_step_adjoint, multiple = ctx.meta
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -290,14 +305,14 @@ class AcousticLoop(torch.autograd.Function):
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(grad_rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b),
                   _lib.ptr(mu_a), _lib.ptr(mu_b), *[_lib.ptr(t) for t in extra],
-                  _lib.ptr(gpart), _lib.ptr(grad), B, nz, nx, dt, h, NPML, S, rz, vx,
+                  _lib.ptr(gpart), _lib.ptr(grad), B, nz, nx, dt, h, NPML, multiple, S, rz, vx,
                   _lib.stream_of(vp))
-        return grad, None, None, None, None, None, None, None, None, None, None, None, None
+        return (grad,) + (None,) * 13
 
 
 def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
               snapshot_every=None, rows_per_thread=None, vec_x=None, two_step=True,
-              two_step_adjoint=False, illum=None):
+              two_step_adjoint=False, illum=None, multiple=False):
     """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint.
 
     The launch shape -- shots_per_group (1, 2, 4, 8), rows_per_thread
@@ -311,8 +326,13 @@ def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
     illum: None, or an (nz, nx) float32 tensor filled with the source
     illumination of this forward (rnn.py:127-128, 204-205), the divisor of
     ``process.PostProcess.precondition``.
+    multiple: free surface (the reference's geom.multiple): d damps no top
+    strip (``pml.generate_pml_coefficients_2d(..., multiple=True)``), the ring
+    has no top row and the gradient covers rows [0, nz-50) of the interior
+    columns, as the reference's _time_step_backward_multiple rebuilds them
+    (acoustic.py:114-137).  Non-finite records raise ValueError (rnn.py:215).
     """
     return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
                               rec_flat.contiguous(), int(B), _f(dt), _f(h),
                               (shots_per_group, rows_per_thread, vec_x), snapshot_every,
-                              bool(two_step), bool(two_step_adjoint), illum)
+                              bool(two_step), bool(two_step_adjoint), illum, bool(multiple))

@@ -18,7 +18,7 @@ interior reverses exactly as in the PML case).
 import torch
 
 from .. import _lib
-from .acoustic import (NPML, _check_acquisition, _f, default_shots_per_group,
+from .acoustic import (NPML, _check_acquisition, _f, check_finite, default_shots_per_group,
                        restore_boundaries, step as _undamped_step)
 
 
@@ -101,7 +101,7 @@ class HabcLoop(torch.autograd.Function):
             raise RuntimeError("illum must have the model's shape")
         need_grad = ctx.needs_input_grad[0]
         # the HABC weight is zero on the interior, which the adjoint's reversal needs
-        _check_acquisition(vp, b, wavelet, src_zx, rec_flat, B, need_grad)
+        _check_acquisition(vp, b, wavelet, src_zx, rec_flat, B, need_grad, multiple)
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -112,7 +112,8 @@ class HabcLoop(torch.autograd.Function):
         K = 0
         if need_grad:
             if nt > 1:
-                ring = torch.empty((nt - 1, B, ring_len(nz, nx)), device=vp.device)
+                ring = torch.empty((nt - 1, B, ring_len(nz, nx, multiple=multiple)),
+                                   device=vp.device)
             K = default_snapshot_every(nt, B, nz, nx, vp.device) if snapshot_every is None \
                 else int(snapshot_every)
             n_snap = int(_lib.load().seis_snapshot_count(nt, K))
@@ -122,6 +123,7 @@ class HabcLoop(torch.autograd.Function):
                   _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K, _lib.ptr(illum), _lib.ptr(u[0]), _lib.ptr(u[1]),
                   _lib.ptr(u[2]), B, nz, nx, dt, h, NPML, int(multiple), S, _lib.stream_of(vp))
+        check_finite(rec)
         if need_grad:
             ctx.save_for_backward(vp, b, wavelet, src_zx, rec_flat)
             ctx.state = (ring, snaps, u[0], u[1])
@@ -155,7 +157,9 @@ def propagate(vp, b, wavelet, src_zx, rec_flat, B, dt, h, multiple=False, shots_
               snapshot_every=None, illum=None):
     """Records (nt, R) of the HABC loop; differentiable w.r.t. vp.  illum:
     None, or an (nz, nx) float32 tensor filled with the source illumination
-    (rnn.py:127-128, 204-205)."""
+    (rnn.py:127-128, 204-205).  multiple: free surface (no top strip); the
+    gradient then covers rows [0, nz-50) of the interior columns
+    (acoustic_habc.py:247-267).  Non-finite records raise ValueError."""
     return HabcLoop.apply(vp, b, wavelet.contiguous(), src_zx.contiguous(), rec_flat.contiguous(),
                           int(B), _f(dt), _f(h), bool(multiple), shots_per_group, snapshot_every,
                           illum)

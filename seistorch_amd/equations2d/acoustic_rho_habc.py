@@ -19,7 +19,7 @@ through these functions raises.
 import torch
 
 from .. import _lib
-from .acoustic import _check_acquisition
+from .acoustic import _check_acquisition, check_finite
 from .utils import restore_boundaries
 from .acoustic_habc import _multiple
 
@@ -112,7 +112,7 @@ class RhoHabcLoop(torch.autograd.Function):
                   float(dt), float(h),
                   int(multiple),
                   int(S), _lib.stream_of(vp))
-        return rec
+        return check_finite(rec)
 
     @staticmethod
     def backward(ctx, grad_rec):

@@ -20,6 +20,7 @@ backward = hand-written adjoint with boundary-saving reconstruction.
 import torch
 
 from .. import _lib
+from .acoustic import check_finite
 from .utils import restore_boundaries
 
 NPML = 49
@@ -175,13 +176,13 @@ class ElasticLoop(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, vp, vs, rho, d, wavelet, src_zx, rec_flat, B, dt, h, src_type, rec_type,
-                shots_per_group, snapshot_every, fused=True, illum=None):
+                shots_per_group, snapshot_every, fused=True, illum=None, multiple=False):
         _lib.require_device(vp, vs, rho, d, wavelet, src_zx, rec_flat, illum)
         if illum is not None and illum.shape != vp.shape:
             raise RuntimeError("illum must have the model's shape")
         from .acoustic import _check_acquisition
         need_grad = any(ctx.needs_input_grad[:3])
-        _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad)
+        _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad, multiple)
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -195,7 +196,7 @@ class ElasticLoop(torch.autograd.Function):
         ring = snaps = None
         K = 0
         if need_grad:
-            L = int(_lib.load().seis_ring_len(nz, nx, NPML))
+            L = int(_lib.load().seis_ring_len(nz, nx, NPML, int(multiple)))
             if nt > 1:
                 ring = torch.empty((nt - 1, 5, B, L), device=vp.device)
             K = default_snapshot_every(nt, B, nz, nx, vp.device) if snapshot_every is None \
@@ -208,13 +209,14 @@ class ElasticLoop(torch.autograd.Function):
                   _mask(src_type), _lib.ptr(rec_flat), R, _lib.ptr(rf), nch, _lib.ptr(rec),
                   _lib.ptr(ring), _lib.ptr(snaps), K, _lib.ptr(illum),
                   FIELDS.index(src_type[-1]), _lib.ptr_array(fields),
-                  _lib.ptr_array(scratch) if fused else None, B, nz, nx, dt, h, NPML, S,
-                  _lib.stream_of(vp))
+                  _lib.ptr_array(scratch) if fused else None, B, nz, nx, dt, h, NPML,
+                  int(multiple), S, _lib.stream_of(vp))
         del scratch  # ping-pong buffers of the fused steps
+        check_finite(rec)
         if need_grad:
             ctx.save_for_backward(vp, vs, rho, cf, wavelet, src_zx, rec_flat, rf)
             ctx.state = (ring, snaps, fields)
-            ctx.meta = (B, dt, h, S, K, _mask(src_type), nch, bool(fused))
+            ctx.meta = (B, dt, h, S, K, _mask(src_type), nch, bool(fused), int(multiple))
         return rec
 
     @staticmethod
@@ -223,7 +225,7 @@ class ElasticLoop(torch.autograd.Function):
         vp, vs, rho, cf, wavelet, src_zx, rec_flat, rf = ctx.saved_tensors
         ring, snaps, fields = ctx.state
         ctx.state = None
-        B, dt, h, S, K, smask, nch, fused = ctx.meta
+        B, dt, h, S, K, smask, nch, fused, multiple = ctx.meta
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
@@ -238,24 +240,26 @@ class ElasticLoop(torch.autograd.Function):
                   _lib.ptr(grad_rec), _lib.ptr(ring), _lib.ptr(snaps),
                   K if snaps is not None else 0, _lib.ptr_array(fields), _lib.ptr_array(adj_a),
                   _lib.ptr_array(adj_b), _lib.ptr_array(scratch) if fused else None,
-                  _lib.ptr(gco), B, nz, nx, dt, h, NPML, S,
-                  _lib.stream_of(vp))
+                  _lib.ptr(gco), B, nz, nx, dt, h, NPML, multiple, S, _lib.stream_of(vp))
         gvp, gvs, grho = (torch.empty_like(vp) for _ in range(3))
         _lib.call("seis_elastic_chain", _lib.ptr(gco), G, _lib.ptr(vp), _lib.ptr(vs),
                   _lib.ptr(rho), _lib.ptr(gvp), _lib.ptr(gvs), _lib.ptr(grho), nz, nx, dt, h,
                   _lib.stream_of(vp))
-        return (gvp, gvs, grho) + (None,) * 13
+        return (gvp, gvs, grho) + (None,) * 14
 
 
 def propagate(vp, vs, rho, d, wavelet, src_zx, rec_flat, B, dt, h, src_type=("txx", "tzz"),
               rec_type=("vx", "vz"), shots_per_group=None, snapshot_every=None, fused=True,
-              illum=None):
+              illum=None, multiple=False):
     """Records (nt, R, nch) of B shots; differentiable w.r.t. vp, vs, rho.
     fused: each forward step and each reverse step of the adjoint in one pass
     (five more field buffers).  illum: None, or an (nz, nx) float32 tensor
     filled with the source illumination of the last source type's field
-    (rnn.py:204-205)."""
+    (rnn.py:204-205).  multiple: free surface (d from
+    ``pml.generate_pml_coefficients_2d(..., multiple=True)``); the gradients
+    then cover rows [0, nz-50) of the interior columns, the region of
+    elastic.py:124-207.  Non-finite records raise ValueError (rnn.py:215)."""
     return ElasticLoop.apply(vp, vs, rho, d, wavelet.contiguous(), src_zx.contiguous(),
                              rec_flat.contiguous(), int(B), _f(dt), _f(h), tuple(src_type),
                              tuple(rec_type), shots_per_group, snapshot_every, bool(fused),
-                             illum)
+                             illum, bool(multiple))

@@ -64,12 +64,19 @@ def surface_acquisition(nx, nshots, src_depth=2, rec_depth=0, bwidth=50, first=N
 
 
 def rec_flat_index(recs, nz, nx):
-    """Flat (B, nz, nx) indices of each shot's receivers, shot-major."""
+    """Flat (B, nz, nx) indices of each shot's receivers, shot-major.  recs[b]
+    is an (n, 2) array of grid (z, x); a receiver outside the grid raises
+    IndexError, as the reference's probe indexing does (probe.py:42-44)."""
     out = []
     for b, r in enumerate(recs):
-        r = np.asarray(r, np.int64)
+        r = np.asarray(r, np.int64).reshape(-1, 2)
+        bad = (r[:, 0] < 0) | (r[:, 0] >= nz) | (r[:, 1] < 0) | (r[:, 1] >= nx)
+        if bad.any():
+            z, x = r[np.argmax(bad)]
+            raise IndexError(f"shot {b}: receiver (z={z}, x={x}) lies outside the "
+                             f"{nz}x{nx} grid")
         out.append(b * nz * nx + r[:, 0] * nx + r[:, 1])
-    return np.concatenate(out)
+    return np.concatenate(out) if out else np.zeros(0, np.int64)
 
 
 def elastic_models(nz, nx, vmin=1800.0, vmax=4500.0, water=20, pert=0.03, sigma=10.0,

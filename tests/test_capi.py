@@ -34,14 +34,18 @@ def test_library_exports_every_declared_symbol():
 def test_ring_len_and_error_path():
     from seistorch_amd import _lib
     lib = _lib.load()
-    assert lib.seis_ring_len(900, 2500, 49) == 2 * 2400 + 2 * 800
-    assert lib.seis_ring_len(100, 100, 49) == -1
+    assert lib.seis_ring_len(900, 2500, 49, 0) == 2 * 2400 + 2 * 800
+    assert lib.seis_ring_len(100, 100, 49, 0) == -1
+    # free surface: no top row, the columns run from row 0 to nz-51
+    assert lib.seis_ring_len(850, 2500, 49, 1) == 2400 + 2 * 800
+    assert lib.seis_ring_len(51, 101, 49, 1) == 1 + 2 * 1
+    assert lib.seis_ring_len(50, 101, 49, 1) == -1
     # argument validation happens on the host, before any device call
     with pytest.raises(RuntimeError, match="batch size"):
         _lib.call("seis_acoustic_step", None, None, None, None, None, 0, 10, 10, 1e-3, 10.0, None)
     with pytest.raises(RuntimeError, match="launch shape"):
         _lib.call("seis_acoustic_forward", None, None, None, 10, None, None, 0, None, None, None,
-                  0, None, None, None, None, None, None, 1, 120, 120, 1e-3, 10.0, 49, 3, 2, 4,
+                  0, None, None, None, None, None, None, 1, 120, 120, 1e-3, 10.0, 49, 0, 3, 2, 4,
                   None)
     with pytest.raises(RuntimeError, match="null pointer"):
         _lib.call("seis_acoustic_step", None, None, None, None, None, 1, 10, 10, 1e-3, 10.0, None)
@@ -137,4 +141,4 @@ def test_variable_density_and_illumination_argument_checks():
                   f1, f1, 1, 120, 120, 1e-3, 10.0, 0, 4, None)
     with pytest.raises(RuntimeError, match="illum must not alias"):
         _lib.call("seis_acoustic_forward", f1, f1, f1, 10, f1, None, 0, None, None, None, 0, f1,
-                  None, f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 1, 1, 1, None)
+                  None, f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 0, 1, 1, 1, None)
