@@ -29,7 +29,7 @@ _K = torch.tensor([[0.0, 1.0, 0.0], [1.0, -4.0, 1.0], [0.0, 1.0, 0.0]])
 
 
 def laplacian(y, h):
-    op = (h ** (-2) * _K.to(y.dtype)).view(1, 1, 3, 3)
+    op = (h ** (-2) * _K.to(device=y.device, dtype=y.dtype)).view(1, 1, 3, 3)
     return F.conv2d(y.unsqueeze(1), op, padding=1).squeeze(1)
 
 
@@ -40,28 +40,32 @@ def time_step(c, h1, h2, dt, h, b):
     return y, h1
 
 
-def _setup(c, d, dt, h, wavelet, src, rec, dtype):
-    c = torch.as_tensor(c).to(dtype)
-    d = torch.as_tensor(d).to(dtype)
-    dt_t = torch.tensor(dt, dtype=dtype)
-    h_t = torch.tensor(h, dtype=dtype)
-    w = torch.as_tensor(wavelet).to(dtype)
-    src = torch.as_tensor(src, dtype=torch.int64).reshape(-1, 2)
-    recs = [torch.as_tensor(r, dtype=torch.int64).reshape(-1, 2) for r in rec]
+def _setup(c, d, dt, h, wavelet, src, rec, dtype, device=None):
+    """Tensors of one problem; device=None keeps them on the CPU (a HIP
+    device runs the same torch ops there, e.g. for full-size problems)."""
+    c = torch.as_tensor(c).to(device=device, dtype=dtype)
+    d = torch.as_tensor(d).to(device=device, dtype=dtype)
+    dt_t = torch.tensor(dt, dtype=dtype, device=device)
+    h_t = torch.tensor(h, dtype=dtype, device=device)
+    w = torch.as_tensor(wavelet).to(device=device, dtype=dtype)
+    src = torch.as_tensor(src, dtype=torch.int64).reshape(-1, 2).to(device)
+    recs = [torch.as_tensor(r, dtype=torch.int64).reshape(-1, 2).to(device) for r in rec]
     B = src.shape[0]
     nz, nx = c.shape
-    smask = torch.zeros((B, nz, nx), dtype=dtype)
-    smask[torch.arange(B), src[:, 0], src[:, 1]] = 1.0
-    bidx = torch.cat([torch.full((r.shape[0],), b, dtype=torch.int64) for b, r in enumerate(recs)])
+    smask = torch.zeros((B, nz, nx), dtype=dtype, device=device)
+    smask[torch.arange(B, device=device), src[:, 0], src[:, 1]] = 1.0
+    bidx = torch.cat([torch.full((r.shape[0],), b, dtype=torch.int64, device=device)
+                      for b, r in enumerate(recs)])
     rz = torch.cat([r[:, 0] for r in recs])
     rx = torch.cat([r[:, 1] for r in recs])
     counts = [r.shape[0] for r in recs]
     return c, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts
 
 
-def forward(c, d, dt, h, wavelet, src, rec, dtype=torch.float32, c_param=None):
+def forward(c, d, dt, h, wavelet, src, rec, dtype=torch.float32, c_param=None, device=None):
     """Records as a list of (nt, nrec_b, 1) tensors (rnn.py:207-212)."""
-    c0, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(c, d, dt, h, wavelet, src, rec, dtype)
+    c0, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(c, d, dt, h, wavelet, src, rec,
+                                                                dtype, device)
     c = c0 if c_param is None else c_param
     B = smask.shape[0]
     h1 = torch.zeros_like(smask)
@@ -101,43 +105,66 @@ def gradient_ad(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32, rec_co
     return J.detach(), cp.grad.detach(), [s.detach() for s in syn]
 
 
-def _ring_idx(nz, nx, npml=49):
+def _ring_idx(nz, nx, npml=49, multiple=False):
+    """Ring cells (z, x) of seistorch/equations2d/utils.py:50-90 without the
+    corners; with a free surface the top row is not part of it and the
+    columns reach row 0 (utils.py:71-78 leaves the top alone)."""
     z0, z1, x0, x1 = npml, nz - 1 - npml, npml, nx - 1 - npml
     xs = torch.arange(x0 + 1, x1)
-    zs = torch.arange(z0 + 1, z1)
-    z = torch.cat([torch.full_like(xs, z0), torch.full_like(xs, z1), zs, zs])
-    x = torch.cat([xs, xs, torch.full_like(zs, x0), torch.full_like(zs, x1)])
+    zs = torch.arange(0 if multiple else z0 + 1, z1)
+    top_z = [] if multiple else [torch.full_like(xs, z0)]
+    top_x = [] if multiple else [xs]
+    z = torch.cat(top_z + [torch.full_like(xs, z1), zs, zs])
+    x = torch.cat(top_x + [xs, torch.full_like(zs, x0), torch.full_like(zs, x1)])
     return z, x
 
 
-def gradient_bs(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32, npml=49):
+def gradient_bs(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32, npml=49,
+                multiple=False, rec_cotangent=None, habc=False, device=None):
     """Boundary-saving gradient by per-step recompute (CPU baseline shape).
 
-    Returns (J, dJ/dc on the interior [npml+1, -npml-1), records).
+    Returns (J, dJ/dc on the interior [npml+1, -npml-1) -- rows from 0 with a
+    free surface -- and records).  With ``rec_cotangent`` (nt, sum nrec) the
+    records are backpropagated with it instead of the L2 residual (J = 0).
+    habc=True: ``d`` is the HABC weight field and every step is
+    ``habc_step`` (acoustic_habc.py:206-221); the rebuilt interior is the
+    same undamped region.
     """
-    c, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(c, d, dt, h, wavelet, src, rec, dtype)
+    c, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(c, d, dt, h, wavelet, src, rec,
+                                                               dtype, device)
     nz, nx = c.shape
     nt = w.shape[0]
-    rzi, rxi = _ring_idx(nz, nx, npml)
+    rzi, rxi = (i.to(c.device) for i in _ring_idx(nz, nx, npml, multiple))
+
+    def step(cc, h1, h2):
+        if habc:
+            return habc_step(cc, h1, h2, dt_t, h_t, d, multiple=multiple)
+        return time_step(cc, h1, h2, dt_t, h_t, d)
+
     rings = []
     h1 = torch.zeros_like(smask)
     h2 = torch.zeros_like(smask)
     recs = []
     with torch.no_grad():
         for n in range(nt):
-            h1, h2 = time_step(c, h1, h2, dt_t, h_t, d)
+            h1, h2 = step(c, h1, h2)
             h1 = h1 + smask * w[n]
             recs.append(h1[bidx, rz, rx])
             rings.append(h1[:, rzi, rxi].clone())
     stacked = torch.stack(recs, 0).unsqueeze(2)
     syn = list(torch.split(stacked, counts, dim=1))
-    J = l2(syn, obs)
-    res = 2.0 * (stacked - torch.cat([torch.as_tensor(o).to(dtype) for o in obs], 1))[..., 0]
+    if rec_cotangent is None:
+        J = l2(syn, obs)
+        res = 2.0 * (stacked - torch.cat([torch.as_tensor(o).to(dtype) for o in obs], 1))[..., 0]
+    else:
+        J = torch.zeros(())
+        res = torch.as_tensor(rec_cotangent).to(device=c.device, dtype=dtype)
     grad = torch.zeros_like(c)
     # state: u_cur = u^{n+1}, u_prev = u^n ; adjoint grads wrt them
     u_next, u_cur = h1, h2
     g_next = torch.zeros_like(smask)
     g_cur = torch.zeros_like(smask)
+    lo_z = 0 if multiple else npml + 1
     lo, hi_z, hi_x = npml + 1, nz - npml - 1, nx - npml - 1
     for n in range(nt - 1, -1, -1):
         # adjoint of the record of u^{n+1}
@@ -150,19 +177,19 @@ def gradient_bs(c, d, dt, h, wavelet, src, rec, obs, dtype=torch.float32, npml=4
                 lap = laplacian(u_cur, h_t)
                 prev = 2 * u_cur - u_src + (c.pow(2) * dt_t ** 2) * lap
                 keep = torch.zeros_like(prev)
-                keep[:, lo:hi_z, lo:hi_x] = prev[:, lo:hi_z, lo:hi_x]
+                keep[:, lo_z:hi_z, lo:hi_x] = prev[:, lo_z:hi_z, lo:hi_x]
                 keep[:, rzi, rxi] = rings[n - 2]
                 prev = keep
         cp = c.clone().requires_grad_(True)
         ucur = u_cur.clone().requires_grad_(True)
         uprev = prev.clone().requires_grad_(True)
-        y, _ = time_step(cp, ucur, uprev, dt_t, h_t, d)
+        y, _ = step(cp, ucur, uprev)
         y.backward(g_next)
         grad += cp.grad
         g_next, g_cur = g_cur + ucur.grad, uprev.grad
         u_next, u_cur = u_cur, prev
     mask = torch.zeros_like(grad)
-    mask[lo:hi_z, lo:hi_x] = 1.0
+    mask[lo_z:hi_z, lo:hi_x] = 1.0
     return J.detach(), grad * mask, syn
 
 
@@ -211,12 +238,13 @@ ELASTIC_FIELDS = ("vx", "vz", "txx", "tzz", "txz")
 
 
 def elastic_forward(vp, vs, rho, d, dt, h, wavelet, src, rec, src_type=("txx", "tzz"),
-                    rec_type=("vx", "vz"), dtype=torch.float32, params=None):
+                    rec_type=("vx", "vz"), dtype=torch.float32, params=None, device=None):
     """Records as a list of (nt, nrec_b, nch) tensors (rnn.py:176-212, offset 0)."""
     vp0, d, dt_t, h_t, w, smask, (bidx, rz, rx), counts = _setup(vp, d, dt, h, wavelet, src, rec,
-                                                                 dtype)
+                                                                 dtype, device)
     if params is None:
-        params = (vp0, torch.as_tensor(vs).to(dtype), torch.as_tensor(rho).to(dtype))
+        params = (vp0, torch.as_tensor(vs).to(device=device, dtype=dtype),
+                  torch.as_tensor(rho).to(device=device, dtype=dtype))
     f = {n: torch.zeros_like(smask) for n in ELASTIC_FIELDS}
     recs = {k: [] for k in rec_type}
     for n in range(w.shape[0]):

@@ -10,7 +10,10 @@ the same three functions with the same positional arguments,
 so ``equation: acoustic`` in a YAML config selects it.  The per-step
 functions are differentiable (the reference's ``boundary_saving: false`` path
 backpropagates through them, seistorch/cell.py:74-75); their gradient is the
-hand-written VJP kernel, not autograd of elementwise ops.
+hand-written VJP kernel, not autograd of elementwise ops.  They call the
+``torch.library`` operators of ``seistorch_amd.library``, so the reference's
+forced ``torch.compile`` of both functions (seistorch/model.py:83-85) traces
+them without breaking.
 
 ``AcousticLoop`` is the fused alternative used by ``WaveRNN`` when the
 configuration allows it: the whole time loop (step, source, receivers, ring
@@ -20,7 +23,7 @@ boundary-saving reconstruction, replacing autograd-through-the-loop
 """
 import torch
 
-from .. import _lib
+from .. import _lib, library
 from .utils import restore_boundaries
 
 NPML = 49  # ring index of the boundary saving (seistorch/equations2d/utils.py:50)
@@ -30,40 +33,11 @@ def _f(x):
     return float(x.item()) if torch.is_tensor(x) else float(x)
 
 
-class _StepFn(torch.autograd.Function):
-    """y = A*(2/dt^2*h1 - B*h2 + vp^2*L h1) (acoustic.py:81-84) and its VJP."""
-
-    @staticmethod
-    def forward(ctx, vp, h1, h2, d, dt, h):
-        _lib.require_device(vp, h1, h2, d)
-        if h1.shape != h2.shape or h1.dim() != 3 or vp.shape != h1.shape[1:] or d.shape != vp.shape:
-            raise RuntimeError(f"acoustic step: shape mismatch vp{tuple(vp.shape)} "
-                               f"h1{tuple(h1.shape)} h2{tuple(h2.shape)} d{tuple(d.shape)}")
-        B, nz, nx = h1.shape
-        y = torch.empty_like(h1)
-        _lib.call("seis_acoustic_step", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(h1), _lib.ptr(h2),
-                  _lib.ptr(y), B, nz, nx, dt, h, _lib.stream_of(h1))
-        ctx.save_for_backward(vp, d, h1)
-        ctx.dt, ctx.h = dt, h
-        return y
-
-    @staticmethod
-    @torch.autograd.function.once_differentiable
-    def backward(ctx, gy):
-        vp, d, h1 = ctx.saved_tensors
-        gy = gy.contiguous()
-        B, nz, nx = h1.shape
-        gh1 = torch.empty_like(h1)
-        gh2 = torch.empty_like(h1)
-        gvp = torch.empty_like(vp) if ctx.needs_input_grad[0] else None
-        _lib.call("seis_acoustic_step_vjp", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(h1),
-                  _lib.ptr(gy), None, _lib.ptr(gh1), _lib.ptr(gh2), _lib.ptr(gvp), B, nz, nx,
-                  ctx.dt, ctx.h, _lib.stream_of(h1))
-        return gvp, gh1, gh2, None, None, None
-
-
 def step(vp, h1, h2, d, dt, h):
-    return _StepFn.apply(vp, h1.contiguous(), h2.contiguous(), d, _f(dt), _f(h))
+    """y = A*(2/dt^2*h1 - B*h2 + vp^2*L h1) (acoustic.py:81-84): the registered
+    operator torch.ops.seistorch_amd.acoustic_step, whose gradient is the
+    hand-written VJP kernel."""
+    return library.acoustic_step(vp, h1, h2, d, _f(dt), _f(h))
 
 
 def _time_step(*args, **kwargs):

@@ -10,14 +10,15 @@ Drop-in for ``seistorch.equations2d.acoustic_habc``
 ``b`` is the HABC weight field (seistorch_amd.habc.generate_habc_coefficients_2d);
 ``habcs`` are the strip masks the reference's cell passes (cell.py:20-27); only
 whether the top mask is None (free surface) is read from them, the kernel
-evaluates the trapezoids itself.  The step's gradient is the hand-written
-transpose kernel.  ``HabcLoop`` / ``propagate`` is the fused loop with the
+evaluates the trapezoids itself.  The step is the registered operator
+torch.ops.seistorch_amd.habc_step; its gradient is the hand-written transpose
+kernel.  ``HabcLoop`` / ``propagate`` is the fused loop with the
 boundary-saving adjoint (ring at index 49, whose HABC weight is 0, so the
 interior reverses exactly as in the PML case).
 """
 import torch
 
-from .. import _lib
+from .. import _lib, library
 from .acoustic import (NPML, _check_acquisition, _f, check_finite, default_shots_per_group,
                        restore_boundaries, step as _undamped_step)
 
@@ -26,43 +27,13 @@ def _multiple(habcs):
     return habcs is not None and habcs[0] is None
 
 
-class _HabcStepFn(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, vp, h1, h2, b, dt, h, multiple):
-        _lib.require_device(vp, h1, h2, b)
-        if h1.shape != h2.shape or h1.dim() != 3 or vp.shape != h1.shape[1:] or b.shape != vp.shape:
-            raise RuntimeError("habc step: shape mismatch")
-        B, nz, nx = h1.shape
-        y = torch.empty_like(h1)
-        _lib.call("seis_habc_step", _lib.ptr(vp), _lib.ptr(b), _lib.ptr(h1), _lib.ptr(h2),
-                  _lib.ptr(y), B, nz, nx, dt, h, int(multiple), _lib.stream_of(h1))
-        ctx.save_for_backward(vp, b, h1, h2)
-        ctx.meta = (dt, h, int(multiple))
-        return y
-
-    @staticmethod
-    @torch.autograd.function.once_differentiable
-    def backward(ctx, gy):
-        vp, b, h1, h2 = ctx.saved_tensors
-        dt, h, multiple = ctx.meta
-        B, nz, nx = h1.shape
-        gy = gy.contiguous()
-        gh1 = torch.empty_like(h1)
-        gh2 = torch.empty_like(h1)
-        gvp = torch.empty_like(vp) if ctx.needs_input_grad[0] else None
-        _lib.call("seis_habc_step_vjp", _lib.ptr(vp), _lib.ptr(b), _lib.ptr(h1), _lib.ptr(h2),
-                  _lib.ptr(gy), _lib.ptr(gh1), _lib.ptr(gh2), _lib.ptr(gvp), B, nz, nx, dt, h,
-                  multiple, _lib.stream_of(h1))
-        return gvp, gh1, gh2, None, None, None, None
-
-
 def _time_step(*args, **kwargs):
     """acoustic_habc.py:206-221: undamped step + HABC strips, returns (y, h1)."""
     c = args[0]
     h1, h2 = args[1:3]
     dt, h, b = args[3:6]
     multiple = _multiple(kwargs.get("habcs"))
-    y = _HabcStepFn.apply(c, h1.contiguous(), h2.contiguous(), b, _f(dt), _f(h), multiple)
+    y = library.habc_step(c, h1, h2, b, _f(dt), _f(h), multiple)
     return y, h1
 
 

@@ -9,17 +9,18 @@ Drop-in for ``seistorch.equations2d.elastic`` (seistorch/equations2d/elastic.py)
 The step is the stress stage then the velocity stage (elastic.py:16-35); the
 reference's reverse step (elastic.py:39-207) is the velocity stage then the
 stress stage with negated time step on the interior, with ring restores and
-the source re-added.  Every stage is a ``torch.autograd.Function`` whose
-backward is the hand-written transpose kernel, so both the autograd path
-(cell.py:74-75) and the reference's checkpoint mechanism (checkpoint.py)
-differentiate through HIP kernels.
+the source re-added.  Every stage is the registered operator
+torch.ops.seistorch_amd.elastic_stage, whose autograd formula is the
+hand-written transpose kernel, so the autograd path (cell.py:74-75), the
+reference's checkpoint mechanism (checkpoint.py) and its forced
+torch.compile (model.py:83-85) all differentiate through HIP kernels.
 
 ``ElasticLoop`` / ``propagate`` is the fused path: the whole loop in one call,
 backward = hand-written adjoint with boundary-saving reconstruction.
 """
 import torch
 
-from .. import _lib
+from .. import _lib, library
 from .acoustic import check_finite
 from .utils import restore_boundaries
 
@@ -34,73 +35,18 @@ def _f(x):
 
 def coefficients(vp, vs, rho, d, dt, h):
     """(6, nz, nx): lam+2mu, lam, dt*mu/h, dt/(rho h), 1/(1+c), 1-c."""
-    _lib.require_device(vp, vs, rho, d)
-    nz, nx = vp.shape
-    for t in (vs, rho, d):
-        if t.shape != vp.shape:
-            raise RuntimeError("vp, vs, rho and d must share one (nz, nx) shape")
-    cf = torch.empty((6, nz, nx), device=vp.device, dtype=torch.float32)
-    _lib.call("seis_elastic_coefficients", _lib.ptr(vp), _lib.ptr(vs), _lib.ptr(rho),
-              _lib.ptr(d), _lib.ptr(cf), nz, nx, _f(dt), _f(h), _lib.stream_of(vp))
-    return cf
+    return library.elastic_coefficients(vp, vs, rho, d, _f(dt), _f(h))
 
 
-def _shots(B):
-    for s in (4, 2):
-        if B >= s:
-            return s
-    return 1
-
-
-class _StageFn(torch.autograd.Function):
-    """One stage (0: stresses, 1: velocities) with sign and region, out of place."""
-
-    @staticmethod
-    def forward(ctx, vp, vs, rho, d, vx, vz, txx, tzz, txz, stage, sign, region, dt, h):
-        fin = [t.contiguous() for t in (vx, vz, txx, tzz, txz)]
-        _lib.require_device(*fin)
-        B, nz, nx = fin[0].shape
-        for t in fin:
-            if tuple(t.shape) != (B, nz, nx):
-                raise RuntimeError("all five wavefields must share one (B, nz, nx) shape")
-        if tuple(vp.shape) != (nz, nx):
-            raise RuntimeError(f"model {tuple(vp.shape)} does not match fields {(nz, nx)}")
-        cf = coefficients(vp, vs, rho, d, dt, h)
-        out = [t.clone() for t in fin]
-        S = _shots(B)
-        _lib.call("seis_elastic_stage", stage, _lib.ptr(cf), _lib.ptr_array(out), sign,
-                  *region, B, nz, nx, dt, h, S, _lib.stream_of(vp))
-        ctx.save_for_backward(vp, vs, rho, cf, *fin)
-        ctx.meta = (stage, sign, region, dt, h, S)
-        # fields a stage does not update are returned as new tensors all the same
-        return tuple(out)
-
-    @staticmethod
-    @torch.autograd.function.once_differentiable
-    def backward(ctx, *cots):
-        vp, vs, rho, cf, *fin = ctx.saved_tensors
-        stage, sign, region, dt, h, S = ctx.meta
-        B, nz, nx = fin[0].shape
-        cin = [c.contiguous() if c is not None else torch.zeros_like(fin[0]) for c in cots]
-        cout = [torch.empty_like(fin[0]) for _ in range(5)]
-        need_model = any(ctx.needs_input_grad[:3])
-        G = (B + S - 1) // S
-        gco = torch.zeros((G, 4, nz, nx), device=vp.device) if need_model else None
-        _lib.call("seis_elastic_stage_vjp", stage, _lib.ptr(cf), _lib.ptr_array(cin),
-                  _lib.ptr_array(cout), _lib.ptr_array(fin), _lib.ptr(gco), sign, *region, B,
-                  nz, nx, dt, h, S, _lib.stream_of(vp))
-        gvp = gvs = grho = None
-        if need_model:
-            gvp, gvs, grho = (torch.empty_like(vp) for _ in range(3))
-            _lib.call("seis_elastic_chain", _lib.ptr(gco), G, _lib.ptr(vp), _lib.ptr(vs),
-                      _lib.ptr(rho), _lib.ptr(gvp), _lib.ptr(gvs), _lib.ptr(grho), nz, nx, dt, h,
-                      _lib.stream_of(vp))
-        return (gvp, gvs, grho, None, *cout, None, None, None, None, None)
+_shots = library.elastic_shots_per_group
 
 
 def _stage(stage, sign, region, vp, vs, rho, d, fields, dt, h):
-    return _StageFn.apply(vp, vs, rho, d, *fields, stage, float(sign), tuple(region), _f(dt),
-                          _f(h))
+    """One stage as the registered operator torch.ops.seistorch_amd.elastic_stage
+    (gradient: the hand-written transpose kernel and the chain to vp, vs, rho)."""
+    z0, z1, x0, x1 = (int(v) for v in region)
+    return library.elastic_stage(vp, vs, rho, d, *fields, int(stage), float(sign), z0, z1, x0,
+                                 x1, _f(dt), _f(h))
 
 
 def _time_step(*args, **kwargs):

