@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 11
+#define SEIS_ABI_VERSION 14
 
 #ifdef __cplusplus
 extern "C" {
@@ -41,9 +41,18 @@ int32_t seis_abi_version(void);
 #define SEIS_KERNEL_BATCH_BWD 7  /* k_bwd_batch, direct launches only */
 #define SEIS_KERNEL_ACOUSTIC_FWD2 8  /* k_fwd2: two forward steps per launch */
 #define SEIS_KERNEL_ACOUSTIC_BWD2 9  /* k_bwd2: two reverse steps per launch */
-#define SEIS_PROFILE_KERNELS 10
+#define SEIS_KERNEL_STREAM_COPY 10   /* k_stream_copy (seis_stream_copy) */
+#define SEIS_PROFILE_KERNELS 11
 int seis_profile_enable(int32_t on);
 int seis_profile_collect(int32_t kernel, double* total_ms, int64_t* launches);
+
+/* HBM calibration copy (no reference counterpart; used by the benchmark):
+ * dst[i] = src[i] for n floats (n a multiple of 4, 16-byte aligned, no
+ * overlap) with float4 accesses over a grid-stride loop of `blocks` blocks of
+ * 256 threads.  Its achieved rate is the practical HBM ceiling, and its
+ * FETCH_SIZE / WRITE_SIZE counters against the 4n bytes it moves each way
+ * calibrate the counters of the other kernels. */
+int seis_stream_copy(const float* src, float* dst, int64_t n, int32_t blocks, void* stream);
 
 /* Number of cells in the boundary-saving ring of an (nz, nx) grid: rows
  * npml and nz-1-npml, columns npml and nx-1-npml, without the corners
@@ -106,10 +115,18 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
  * summed at the end: the same sum in another order) instead of a pass over
  * every shot's field per time level.  multiple != 0: free surface (the
  * reference's geom.multiple, seistorch/pml.py:49-53 damps no top strip), the
- * ring is that of seis_ring_len(nz, nx, npml, 1), without a top row. */
+ * ring is that of seis_ring_len(nz, nx, npml, 1), without a top row.
+ * extra_src / extra_wav / n_extra: n_extra more point sources (source
+ * encoding, codingfwi.py:184-334 with source.py:54-55: many sources with
+ * their own encoded wavelets in one field), rows (shot, z, x) of extra_src
+ * (n_extra x 3 int32) with wavelets extra_wav (n_extra x nt); after step n,
+ * extra_wav[t*nt + n] is added at each row's cell (cells shared by two
+ * sources get both).  They need one step per launch (u_c = u_d = NULL);
+ * n_extra = 0 (NULL tables) for none. */
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
-                          const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
-                          float* rec_out, float* ring, float* snaps, int32_t snap_every,
+                          const int32_t* src_zx, const int32_t* extra_src,
+                          const float* extra_wav, int64_t n_extra, const int64_t* rec_flat,
+                          int64_t R, float* rec_out, float* ring, float* snaps, int32_t snap_every,
                           float* illum, float* illum_part, float* u_a, float* u_b, float* u_c,
                           float* u_d, int64_t B, int64_t nz, int64_t nx, float dt, float h,
                           int32_t npml, int32_t multiple, int32_t shots_per_group,
@@ -125,7 +142,10 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
  * dJ/dvp on the interior [npml+1, n-npml-1) and zero elsewhere; with
  * multiple != 0 (free surface, as in the forward) the interior rows are
  * [0, nz-npml-1), the region that seistorch/equations2d/acoustic.py:114-137
- * rebuilds when it leaves the top ring row alone.
+ * rebuilds when it leaves the top ring row alone.  extra_src, extra_wav,
+ * n_extra: the forward's extra sources (removed again by the reverse-time
+ * rebuild; they must lie in the rebuilt interior); they turn off the
+ * two-step adjoint.
  * If u_c, u_d, mu_c, mu_d (four more (B, nz, nx) buffers) and, when R > 0,
  * rec_map are given, two reverse steps run per launch where the ring is
  * available: rec_map (nz x nx int32) holds each cell's receiver index within
@@ -134,7 +154,9 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
  * twice.  Results equal the one-step path except for the order of the
  * gradient additions at receiver cells.  NULL keeps one step per launch. */
 int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet, int64_t nt,
-                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
+                           const int32_t* src_zx, const int32_t* extra_src,
+                           const float* extra_wav, int64_t n_extra, const int64_t* rec_flat,
+                           int64_t R,
                            const float* rec_grad, const float* ring, const float* snaps,
                            int32_t snap_every, float* u_a, float* u_b,
                            float* mu_a, float* mu_b, float* u_c, float* u_d, float* mu_c,
@@ -243,18 +265,38 @@ int seis_rho_habc_step(const float* vp, const float* rho, const float* b, const 
 /* Whole forward loop of the variable-density equation (rnn.py:178-212 with
  * acoustic_rho_habc.py:31-57): nt steps from zero fields, wavelet[n] added at
  * src_zx[b] after step n, receivers rec_flat[r] recorded into
- * rec_out[n*R + r]; buoy: (nz, nx) work buffer (receives 1/rho, formed once
- * per call); illum: NULL or the source illumination as in
- * seis_acoustic_forward; u_a, u_b, u_c: (B, nz, nx) rotating buffers (u^{nt} ends
- * in u_a, u^{nt-1} in u_b); shots_per_group >= 1 shots share one thread
- * (the per-cell set-up runs once for them).  Forward only: the gradient with
- * respect to vp and rho has no HIP adjoint yet. */
+ * rec_out[n*R + r]; ring and snaps (may be NULL) as in seis_habc_forward
+ * (ring: (nt-1) x B x seis_ring_len(nz, nx, npml, multiple), snaps:
+ * count x 2 x B x nz x nx); buoy: (nz, nx) work buffer (receives 1/rho,
+ * formed once per call); illum: NULL or the source illumination as in
+ * seis_acoustic_forward; u_a, u_b, u_c: (B, nz, nx) rotating buffers
+ * (u^{nt} ends in u_a, u^{nt-1} in u_b); shots_per_group >= 1 shots share
+ * one thread (the per-cell set-up runs once for them). */
 int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
                           const float* wavelet, int64_t nt, const int32_t* src_zx,
-                          const int64_t* rec_flat, int64_t R, float* rec_out, float* buoy,
-                          float* illum, float* u_a, float* u_b, float* u_c, int64_t B,
-                          int64_t nz, int64_t nx, double dt, double h, int32_t multiple,
-                          int32_t shots_per_group, void* stream);
+                          const int64_t* rec_flat, int64_t R, float* rec_out, float* ring,
+                          float* snaps, int32_t snap_every, float* buoy, float* illum, float* u_a,
+                          float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx, double dt,
+                          double h, int32_t npml, int32_t multiple, int32_t shots_per_group,
+                          void* stream);
+
+/* Adjoint of seis_rho_habc_forward with boundary-saving reconstruction
+ * (replaces the reference's checkpointed backward of
+ * acoustic_rho_habc.py:31-121 through seistorch/checkpoint_new.py:146-217,
+ * with the exact time pairing): rec_grad[n*R + r] = dJ/d rec_out[n*R + r];
+ * ring and snaps as written by the forward with the same snap_every; u_a,
+ * u_b: the forward's final state (consumed); l_a, l_b, l_c: (B, nz, nx)
+ * cotangent buffers; buoy: (nz, nx) work; grad_part: 2 x ceil(B/S) x (nz, nx)
+ * work; grad_vp and grad_rho (nz, nx) get dJ/dvp and dJ/drho on the undamped
+ * interior (rows from 0 with multiple != 0) and zero elsewhere. */
+int seis_rho_habc_backward(const float* vp, const float* rho, const float* b,
+                           const float* wavelet, int64_t nt, const int32_t* src_zx,
+                           const int64_t* rec_flat, int64_t R, const float* rec_grad,
+                           const float* ring, const float* snaps, int32_t snap_every, float* buoy,
+                           float* u_a, float* u_b, float* l_a, float* l_b, float* l_c,
+                           float* grad_part, float* grad_vp, float* grad_rho, int64_t B,
+                           int64_t nz, int64_t nx, double dt, double h, int32_t npml,
+                           int32_t multiple, int32_t shots_per_group, void* stream);
 
 /* ---------------- elastic, isotropic velocity-stress, PML ---------------- */
 /* Fields are passed as an array of five (B, nz, nx) pointers in the order
@@ -379,6 +421,7 @@ int seis_smooth_gradient(const float* grad, float* out, float* work, int64_t nz,
                          const float* wz, int32_t kz, const float* wx, int32_t kx, int32_t counts,
                          const float* pre_mask, const float* post_mask, const float* illum,
                          void* stream);
+
 
 /* ---------------- learned gradient preconditioner (U-Net) ----------------- */
 /* No reference counterpart: a small U-Net maps (gradient, model) to an

@@ -389,11 +389,14 @@ def rho_habc_step(c, rho, h1, h2, dt, h, b, w=50, multiple=False):
     return habc_blend(y, h1, h2, c, dt_t, h_t, b, w, multiple), h1
 
 
-def rho_habc_forward(c, rho, b, dt, h, wavelet, src, rec, dtype=torch.float32, multiple=False):
+def rho_habc_forward(c, rho, b, dt, h, wavelet, src, rec, dtype=torch.float32, multiple=False,
+                     params=None):
     """Records of the variable-density HABC loop (rnn.py:178-212), list of
-    (nt, nrec_b, 1)."""
+    (nt, nrec_b, 1).  params: optional (c, rho) tensors to differentiate."""
     c, b, _, _, w, smask, (bidx, rz, rx), counts = _setup(c, b, dt, h, wavelet, src, rec, dtype)
     rho = torch.as_tensor(rho).to(dtype)
+    if params is not None:
+        c, rho = params
     h1 = torch.zeros_like(smask)
     h2 = torch.zeros_like(smask)
     recs = []
@@ -405,6 +408,16 @@ def rho_habc_forward(c, rho, b, dt, h, wavelet, src, rec, dtype=torch.float32, m
         recs.append(h1[bidx, rz, rx])
     stacked = torch.stack(recs, 0).unsqueeze(2)
     return list(torch.split(stacked, counts, dim=1))
+
+
+def rho_habc_gradient_ad(c, rho, b, dt, h, wavelet, src, rec, rec_cotangent,
+                         dtype=torch.float32, multiple=False):
+    """Autograd through the whole variable-density loop for the records'
+    cotangent: ((dJ/dc, dJ/drho), records)."""
+    ps = [torch.as_tensor(x).to(dtype).clone().requires_grad_(True) for x in (c, rho)]
+    syn = rho_habc_forward(c, rho, b, dt, h, wavelet, src, rec, dtype, multiple, params=ps)
+    torch.cat(syn, 1)[..., 0].backward(torch.as_tensor(rec_cotangent).to(dtype))
+    return tuple(p.grad.detach() for p in ps), [s.detach() for s in syn]
 
 
 def habc_forward(c, b, dt, h, wavelet, src, rec, dtype=torch.float32, c_param=None,

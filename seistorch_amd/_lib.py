@@ -25,18 +25,19 @@ SIGNATURES = {
     "seis_profile_enable": (ctypes.c_int, [_i32]),
     "seis_profile_collect": (ctypes.c_int, [_i32, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(_i64)]),
+    "seis_stream_copy": (ctypes.c_int, [_p, _p, _i64, _i32, _p]),
     "seis_ring_len": (_i64, [_i64, _i64, _i32, _i32]),
     "seis_snapshot_count": (_i64, [_i64, _i32]),
     "seis_acoustic_step": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _p]),
     "seis_acoustic_step_vjp": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64,
                                               _f, _f, _p]),
-    "seis_acoustic_forward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
-                                             _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f,
-                                             _i32, _i32, _i32, _i32, _i32, _p]),
-    "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32,
-                                              _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64,
-                                              _i64, _i64, _f, _f, _i32, _i32, _i32, _i32, _i32,
-                                              _p]),
+    "seis_acoustic_forward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _i64, _p,
+                                             _p, _p, _i32, _p, _p, _p, _p, _p, _p, _i64, _i64,
+                                             _i64, _f, _f, _i32, _i32, _i32, _i32, _i32, _p]),
+    "seis_acoustic_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _i64, _p,
+                                              _p, _p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                              _p, _p, _i64, _i64, _i64, _f, _f, _i32, _i32, _i32,
+                                              _i32, _i32, _p]),
     "seis_acoustic_forward_batch": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                                    _p, _i32, _p, _p, _p, _i32, _i32, _i32, _i32,
                                                    _i32, _p]),
@@ -51,9 +52,14 @@ SIGNATURES = {
                                          _p]),
     "seis_rho_habc_step": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i64, _i64,
                                           ctypes.c_double, ctypes.c_double, _i32, _i32, _p]),
-    "seis_rho_habc_forward": (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _p,
-                                             _p, _p, _i64, _i64, _i64, ctypes.c_double,
-                                             ctypes.c_double, _i32, _i32, _p]),
+    "seis_rho_habc_forward": (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p,
+                                             _i32, _p, _p, _p, _p, _p, _i64, _i64, _i64,
+                                             ctypes.c_double, ctypes.c_double, _i32, _i32, _i32,
+                                             _p]),
+    "seis_rho_habc_backward": (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p,
+                                              _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64,
+                                              _i64, _i64, ctypes.c_double, ctypes.c_double, _i32,
+                                              _i32, _i32, _p]),
     "seis_habc_backward": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _i32, _p,
                                           _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _f, _f, _i32,
                                           _i32, _i32, _p]),
@@ -94,7 +100,7 @@ SIGNATURES = {
     "seis_unet_fill": (ctypes.c_int, [_p, _i64, _f, _p]),
 }
 
-ABI_VERSION = 11
+ABI_VERSION = 14
 BATCH_COLS = 22  # SEIS_BATCH_COLS: int64 columns per instance of the batch table
 
 _lock = threading.Lock()

@@ -382,6 +382,32 @@ __global__ __launch_bounds__(kBlock) void k_fwd2_illum(
 }
 
 // Receivers of the new field into one record row, and its ring into one slot.
+// Additional point sources of one step (source encoding, codingfwi.py:184-334:
+// many sources with their own encoded wavelets in one field; each adds its
+// sample at its cell, source.py:54-55 with the time loop's dt = 1):
+// u[b, z, x] += sign * wav[t*nt + n] for each entry t = (b, z, x) of esrc.
+// Sources that share a cell add.
+__global__ __launch_bounds__(256) void k_extra_src(float* __restrict__ u,
+                                                   const int32_t* __restrict__ esrc,
+                                                   const float* __restrict__ ewav, int64_t nt,
+                                                   int64_t n, int64_t n_extra, float sign,
+                                                   Grid g) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_extra) return;
+  const int64_t cell = (int64_t)esrc[3 * t] * g.plane + (int64_t)esrc[3 * t + 1] * g.nx +
+                       esrc[3 * t + 2];
+  atomicAdd(u + cell, sign * ewav[t * nt + n]);
+}
+
+int extra_sources(float* u, const int32_t* esrc, const float* ewav, int64_t nt, int64_t n,
+                  int64_t n_extra, float sign, const Grid& g, hipStream_t s) {
+  if (n_extra <= 0) return 0;
+  k_extra_src<<<(unsigned)((n_extra + 255) / 256), 256, 0, s>>>(u, esrc, ewav, nt, n, n_extra,
+                                                                  sign, g);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
 __global__ __launch_bounds__(256) void k_record(const float* __restrict__ u,
                                                 const int64_t* __restrict__ rec, int64_t R,
                                                 float* __restrict__ rec_row,
@@ -1480,8 +1506,9 @@ int seis_acoustic_step_vjp(const float* vp, const float* d, const float* h1, con
 }
 
 int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet, int64_t nt,
-                          const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
-                          float* rec_out, float* ring, float* snaps, int32_t snap_every,
+                          const int32_t* src_zx, const int32_t* extra_src,
+                          const float* extra_wav, int64_t n_extra, const int64_t* rec_flat,
+                          int64_t R, float* rec_out, float* ring, float* snaps, int32_t snap_every,
                           float* illum, float* illum_part, float* u_a, float* u_b, float* u_c,
                           float* u_d, int64_t B, int64_t nz, int64_t nx, float dt, float h,
                           int32_t npml, int32_t multiple, int32_t shots_per_group,
@@ -1504,6 +1531,10 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
   SEIS_CHECK_ARG(!ring || ring_len > 0, "grid %lld x %lld has no interior inside npml=%d",
                  (long long)nz, (long long)nx, npml);
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  SEIS_CHECK_ARG(n_extra >= 0 && (n_extra == 0 || (extra_src && extra_wav)),
+                 "bad extra-source arguments");
+  SEIS_CHECK_ARG(n_extra == 0 || !(u_c && u_d),
+                 "extra sources need one time step per launch (u_c = u_d = NULL)");
   const Grid g = seis::make_grid(nz, nx, npml, multiple != 0);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
@@ -1541,6 +1572,7 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
                            prev, src_zx, wavelet + n, (int)B, g, k, tm))
       return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_FWD, s);
+    if (int e = extra_sources(prev, extra_src, extra_wav, nt, n, n_extra, 1.0f, g, s)) return e;
     if (int e = illuminate(prev, (int)B, g.plane, illum, s)) return e;
     float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
     const int64_t work = R + (slot ? B * ring_len : 0);
@@ -1560,7 +1592,9 @@ int seis_acoustic_forward(const float* vp, const float* d, const float* wavelet,
 }
 
 int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet, int64_t nt,
-                           const int32_t* src_zx, const int64_t* rec_flat, int64_t R,
+                           const int32_t* src_zx, const int32_t* extra_src,
+                           const float* extra_wav, int64_t n_extra, const int64_t* rec_flat,
+                           int64_t R,
                            const float* rec_grad, const float* ring, const float* snaps,
                            int32_t snap_every, float* u_a, float* u_b,
                            float* mu_a, float* mu_b, float* u_c, float* u_d, float* mu_c,
@@ -1589,6 +1623,8 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
                  (long long)nx, npml);
   SEIS_CHECK_ARG(ring || nt <= 2, "ring storage is required for nt > 2");
   SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  SEIS_CHECK_ARG(n_extra >= 0 && (n_extra == 0 || (extra_src && extra_wav)),
+                 "bad extra-source arguments");
   const Grid g = seis::make_grid(nz, nx, npml, multiple != 0);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
@@ -1603,7 +1639,7 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
   float* M1 = mu_a; // mu^{k+1}
   float* M2 = mu_b; // mu^{k+2} -> mu^k
   // two reverse iterations per launch (k_bwd2) where possible
-  const bool two = u_c && u_d && mu_c && mu_d && ring && (R == 0 || rec_map);
+  const bool two = u_c && u_d && mu_c && mu_d && ring && (R == 0 || rec_map) && n_extra == 0;
   if (two) {
     SEIS_CHECK_ARG(aligned(u_c, 4 * vec_x) && aligned(u_d, 4 * vec_x) &&
                        aligned(mu_c, 4 * vec_x) && aligned(mu_d, 4 * vec_x),
@@ -1663,6 +1699,11 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
       continue;
     }
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
+    // the rebuild of u^{kk-2} takes u^kk without its sources of step kk-1;
+    // Y's value of u^kk is not read again
+    if (recon == 1)
+      if (int e = extra_sources(Y, extra_src, extra_wav, nt, kk - 1, n_extra, -1.0f, g, s))
+        return e;
     SEIS_PROF(SEIS_KERNEL_ACOUSTIC_BWD, s);
     if (int e = launch_bwd(S, rows_per_thread, vec_x, (int64_t)tm.per * kXcds, s, vp, d, X, Y, M1,
                            M2, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, g, k, tm))

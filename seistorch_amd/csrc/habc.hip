@@ -405,6 +405,41 @@ __global__ __launch_bounds__(256) void k_habc_fwd_deep(const float* __restrict__
   }
 }
 
+// One-way terms of the transposed strip blend that reach cell i: the strip
+// cells n whose one-way stencil reads i, with the coefficients on u^n (c1)
+// and on u^{n-1} (c2) times the blend weight beta of n.  At most 16.
+__device__ __forceinline__ int oneway_terms(int z, int x, int i, const float* __restrict__ vp,
+                                            const float* __restrict__ bw, const HabcCtx& hc,
+                                            int* on, float* oc1, float* oc2) {
+  const Grid& g = hc.g;
+  int no = 0;
+  for (int s = 0; s < 4; ++s) {
+    const int dj = sdist(s, z, x, g);
+    const int e = sstep(s, g);
+    for (int m = 0; m < 3; ++m) {
+      const int dn = dj - m;
+      if (dn < 0 || dn > kW - 1 || (m == 2 && dn == kW - 1)) continue;
+      const int n = i - m * e;
+      const int nz_ = n / g.nx, nx_ = n - nz_ * g.nx;
+      const float be = habc_weights(nz_, nx_, hc, bw[n]).beta[s];
+      if (be == 0.0f) continue;
+      const OwCoef kc = ow_coef(vp[n], hc);
+      const float c1 = m == 0 ? kc.k0 : m == 1 ? kc.k1 : kc.k2;
+      const float c2 = m == 0 ? kc.k3 : m == 1 ? kc.k4 : 0.0f;
+      on[no] = n; oc1[no] = be * c1; oc2[no] = be * c2; ++no;
+    }
+    if (dj == 0) {  // the edge cell is the "+2" neighbour of the row-49 cell (roll wrap)
+      const int n = i + (kW - 1) * e;
+      const int nz_ = n / g.nx, nx_ = n - nz_ * g.nx;
+      const float be = habc_weights(nz_, nx_, hc, bw[n]).beta[s];
+      if (be != 0.0f) {
+        on[no] = n; oc1[no] = be * ow_coef(vp[n], hc).k2; oc2[no] = 0.0f; ++no;
+      }
+    }
+  }
+  return no;
+}
+
 // Transpose of the step: out = H_u^T y1 + H_p^T y2 (either may be null).
 // With recon/grad (fused adjoint, y1 = lambda^{k+1}, y2 = lambda^{k+2}):
 // gradient 2c*A0*L(u^{k-1})*lambda^k on the interior (lambda^k without the
@@ -439,33 +474,7 @@ __device__ __forceinline__ void habc_adj_cell(
   // one-way contributions: (flat offset, coefficient on y1, coefficient on y2)
   int on[16];
   float oc1[16], oc2[16];
-  int no = 0;
-  if (!deep) {
-    for (int s = 0; s < 4; ++s) {
-      const int dj = sdist(s, z, x, g);
-      const int e = sstep(s, g);
-      for (int m = 0; m < 3; ++m) {
-        const int dn = dj - m;
-        if (dn < 0 || dn > kW - 1 || (m == 2 && dn == kW - 1)) continue;
-        const int n = i - m * e;
-        const int nz_ = n / g.nx, nx_ = n - nz_ * g.nx;
-        const float be = habc_weights(nz_, nx_, hc, bw[n]).beta[s];
-        if (be == 0.0f) continue;
-        const OwCoef kc = ow_coef(vp[n], hc);
-        const float c1 = m == 0 ? kc.k0 : m == 1 ? kc.k1 : kc.k2;
-        const float c2 = m == 0 ? kc.k3 : m == 1 ? kc.k4 : 0.0f;
-        on[no] = n; oc1[no] = be * c1; oc2[no] = be * c2; ++no;
-      }
-      if (dj == 0) {  // the edge cell is the "+2" neighbour of the row-49 cell (roll wrap)
-        const int n = i + (kW - 1) * e;
-        const int nz_ = n / g.nx, nx_ = n - nz_ * g.nx;
-        const float be = habc_weights(nz_, nx_, hc, bw[n]).beta[s];
-        if (be != 0.0f) {
-          on[no] = n; oc1[no] = be * ow_coef(vp[n], hc).k2; oc2[no] = 0.0f; ++no;
-        }
-      }
-    }
-  }
+  const int no = deep ? 0 : oneway_terms(z, x, i, vp, bw, hc, on, oc1, oc2);
   const bool in = interior(z, x, g);
   const float w = wn ? *wn : 0.0f;
   // groups grp .. grp+ng-1 of S shots, each with its own gradient partial
@@ -520,6 +529,161 @@ __global__ __launch_bounds__(256) void k_habc_adj(
   const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
   if (x >= g.nx || z >= g.nz) return;
   habc_adj_cell<S>(z, x, grp, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B, hc, k);
+}
+
+// Variable-density adjoint (acoustic_rho_habc.py:31-57), exact transpose of
+// the step y = (2u - p) + K*M(u) with K = (C*rho)*vp^2 and M the face-averaged
+// operator of k_rho_habc_fwd, followed by the strip blend.  M is symmetric
+// (the face (i, j) carries the same average in rows i and j; zero outside the
+// grid), so with alpha the weight of y in the blended value:
+//   out = H_u^T y1 + H_p^T y2,  H_u^T y1 = 2 alpha y1 + M(K alpha y1) + one-way
+//   terms,  H_p^T y2 = -alpha y2 + one-way terms.
+// One cell per thread for spt shots; the cell's set-up (face averages, K*alpha
+// of its neighbours, one-way terms) runs once for them.
+__global__ __launch_bounds__(256) void k_rho_adj(
+    const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ buoy,
+    const float* __restrict__ bw, const float* __restrict__ y1, const float* __restrict__ y2,
+    float* __restrict__ out, int B, int spt, HabcCtx hc, float C, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x >= g.nx || z >= g.nz) return;
+  const int i = z * g.nx + x;
+  const float bc = buoyancy(rho, buoy, z, x, g);
+  const float bxp = (bc + buoyancy(rho, buoy, z, x + 1, g)) / 2.0f;
+  const float bxn = (buoyancy(rho, buoy, z, x - 1, g) + bc) / 2.0f;
+  const float bzp = (bc + buoyancy(rho, buoy, z + 1, x, g)) / 2.0f;
+  const float bzn = (buoyancy(rho, buoy, z - 1, x, g) + bc) / 2.0f;
+  const bool deep = !near_edge(z, x, g, kW + 3);
+  // K*alpha at the cell and its neighbours (0 outside the grid)
+  auto ka = [&](int zz, int xx) {
+    if (zz < 0 || zz >= g.nz || xx < 0 || xx >= g.nx) return 0.0f;
+    const int j = zz * g.nx + xx;
+    const float cn = vp[j];
+    const float al = deep ? 1.0f : habc_weights(zz, xx, hc, bw[j]).alpha;
+    return ((C * rho[j]) * (cn * cn)) * al;
+  };
+  const float k0 = ka(z, x), kxp = ka(z, x + 1), kxn = ka(z, x - 1), kzp = ka(z + 1, x),
+              kzn = ka(z - 1, x);
+  const float al0 = deep ? 1.0f : habc_weights(z, x, hc, bw[i]).alpha;
+  int on[16];
+  float oc1[16], oc2[16];
+  const int no = deep ? 0 : oneway_terms(z, x, i, vp, bw, hc, on, oc1, oc2);
+  const int b0 = grp * spt, nb = min(spt, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int64_t off = (int64_t)(b0 + s) * g.plane;
+    float acc = 0.0f;
+    if (y1) {
+      const float* Y = y1 + off;
+      const float m0 = k0 * Y[i];
+      const float mxp = x + 1 < g.nx ? kxp * Y[i + 1] : 0.0f;
+      const float mxn = x > 0 ? kxn * Y[i - 1] : 0.0f;
+      const float mzp = z + 1 < g.nz ? kzp * Y[i + g.nx] : 0.0f;
+      const float mzn = z > 0 ? kzn * Y[i - g.nx] : 0.0f;
+      const float Mm = ((bxp * (mxp - m0) + bxn * (mxn - m0)) + bzp * (mzp - m0)) +
+                       bzn * (mzn - m0);
+      acc = (2.0f * al0) * Y[i] + Mm;
+      for (int o = 0; o < no; ++o) acc += oc1[o] * Y[on[o]];
+    }
+    if (y2) {
+      const float* Y = y2 + off;
+      acc += -al0 * Y[i];
+      for (int o = 0; o < no; ++o) acc += oc2[o] * Y[on[o]];
+    }
+    out[off + i] = acc;
+  }
+}
+
+// Adjoint source of the fused variable-density adjoint: lam[rec[t]] += s[t]
+// (repeated receivers add).
+__global__ __launch_bounds__(256) void k_rho_inject(float* __restrict__ lam,
+                                                   const int64_t* __restrict__ rec,
+                                                   const float* __restrict__ s, int64_t R) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R) atomicAdd(lam + rec[t], s[t]);
+}
+
+// After k_rho_adj and k_rho_inject (lam = lambda^k, the cotangent of u^k):
+// on the undamped interior, where alpha = 1 at the cell and its neighbours,
+// the model gradient of the step u^k = F(u^{k-1}, u^{k-2}) and the reverse-time
+// rebuild of u^{k-2} over uc (u^k on entry); on the ring, its restore.
+//   dJ/dvp  += 2 C rho vp D lam,  D = M(u^{k-1})
+//   dJ/drho += C vp^2 D lam - (1/(2 rho^2)) sum_j (u_j - u) (mu - mu_j),
+//   mu = K lam  (the face average of 1/rho enters the rows of both its cells)
+//   u^{k-2} = 2 u^{k-1} - (u^k - w - K D)   (w at the shot's source cell)
+// gpart: [2][G][plane] (vp, then rho partials per group of spt shots).
+// recon: 0 none, 1 rebuild + ring from ring_slot, 2 zeros (u^0).
+__global__ __launch_bounds__(256) void k_rho_grad(
+    const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ buoy,
+    const float* __restrict__ lam, const float* __restrict__ um, float* __restrict__ uc,
+    float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
+    const float* __restrict__ ring_slot, int64_t ring_len, int recon, int B, int spt, int G,
+    HabcCtx hc, float C, TileMap tm) {
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;
+  const Grid& g = hc.g;
+  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
+  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x >= g.nx || z >= g.nz) return;
+  const int i = z * g.nx + x;
+  const int b0 = grp * spt, nb = min(spt, B - b0);
+  if (!seis::interior(z, x, g)) {
+    const int64_t j = recon ? seis::ring_index(z, x, g) : -1;
+    if (j >= 0)
+      for (int s = 0; s < nb; ++s) {
+        const int bb = b0 + s;
+        uc[(int64_t)bb * g.plane + i] = recon == 1 ? ring_slot[bb * ring_len + j] : 0.0f;
+      }
+    return;
+  }
+  const float c = vp[i];
+  const float r = rho[i];
+  const float bc = buoyancy(rho, buoy, z, x, g);
+  const float bxp = (bc + buoyancy(rho, buoy, z, x + 1, g)) / 2.0f;
+  const float bxn = (buoyancy(rho, buoy, z, x - 1, g) + bc) / 2.0f;
+  const float bzp = (bc + buoyancy(rho, buoy, z + 1, x, g)) / 2.0f;
+  const float bzn = (buoyancy(rho, buoy, z - 1, x, g) + bc) / 2.0f;
+  auto kk = [&](int zz, int xx) {
+    if (zz < 0 || zz >= g.nz || xx < 0 || xx >= g.nx) return 0.0f;
+    const int j = zz * g.nx + xx;
+    const float cn = vp[j];
+    return (C * rho[j]) * (cn * cn);
+  };
+  const float k0 = (C * r) * (c * c);
+  const float kxp = kk(z, x + 1), kxn = kk(z, x - 1), kzp = kk(z + 1, x), kzn = kk(z - 1, x);
+  const float w = wn ? *wn : 0.0f;
+  float gv = 0.0f, gb = 0.0f;
+  for (int s = 0; s < nb; ++s) {
+    const int bb = b0 + s;
+    const int64_t off = (int64_t)bb * g.plane;
+    const float* U = um + off;
+    const float* Lm = lam + off;
+    const float u0 = U[i];
+    const float pxp = (x + 1 < g.nx ? U[i + 1] : 0.0f) - u0;
+    const float pxn = u0 - (x > 0 ? U[i - 1] : 0.0f);
+    const float pzp = (z + 1 < g.nz ? U[i + g.nx] : 0.0f) - u0;
+    const float pzn = u0 - (z > 0 ? U[i - g.nx] : 0.0f);
+    const float D = ((bxp * pxp - bxn * pxn) + bzp * pzp) - bzn * pzn;
+    const float l0 = Lm[i];
+    gv += D * l0;
+    const float m0 = k0 * l0;
+    const float mxp = x + 1 < g.nx ? kxp * Lm[i + 1] : 0.0f;
+    const float mxn = x > 0 ? kxn * Lm[i - 1] : 0.0f;
+    const float mzp = z + 1 < g.nz ? kzp * Lm[i + g.nx] : 0.0f;
+    const float mzn = z > 0 ? kzn * Lm[i - g.nx] : 0.0f;
+    gb += ((pxp * (m0 - mxp) - pxn * (m0 - mxn)) + pzp * (m0 - mzp)) - pzn * (m0 - mzn);
+    if (recon == 1) {
+      float un = uc[off + i];
+      if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
+      uc[off + i] = 2.0f * u0 - (un - k0 * D);
+    } else if (recon == 2) {
+      uc[off + i] = 0.0f;
+    }
+  }
+  gpart[(int64_t)grp * g.plane + i] += ((2.0f * C) * r * c) * gv;
+  gpart[((int64_t)G + grp) * g.plane + i] += (C * (c * c)) * gv - (0.5f * bc * bc) * gb;
 }
 
 // The fused adjoint step in one launch of two block ranges.  One covers the
@@ -969,10 +1133,11 @@ int seis_rho_habc_step(const float* vp, const float* rho, const float* b, const 
 
 int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
                           const float* wavelet, int64_t nt, const int32_t* src_zx,
-                          const int64_t* rec_flat, int64_t R, float* rec_out, float* buoy,
-                          float* illum, float* u_a, float* u_b, float* u_c, int64_t B,
-                          int64_t nz, int64_t nx, double dt, double h, int32_t multiple,
-                          int32_t shots_per_group, void* stream) {
+                          const int64_t* rec_flat, int64_t R, float* rec_out, float* ring,
+                          float* snaps, int32_t snap_every, float* buoy, float* illum, float* u_a,
+                          float* u_b, float* u_c, int64_t B, int64_t nz, int64_t nx, double dt,
+                          double h, int32_t npml, int32_t multiple, int32_t shots_per_group,
+                          void* stream) {
   if (int e = habc_check(B, nz, nx, (float)dt, (float)h, 1)) return e;
   SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1, got %d", shots_per_group);
   SEIS_CHECK_ARG(vp && rho && b && buoy && wavelet && src_zx && u_a && u_b && u_c,
@@ -981,7 +1146,11 @@ int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
                  "illum must not alias a field buffer");
   SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
   SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_out)), "bad receiver arguments");
-  const HabcCtx hc{seis::make_grid(nz, nx, 0, false), multiple ? 1 : 0, (float)dt, (float)h};
+  const int64_t ring_len = ring ? seis_ring_len(nz, nx, npml, multiple) : 0;
+  SEIS_CHECK_ARG(!ring || ring_len > 0, "grid has no interior inside npml=%d", npml);
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  const HabcCtx hc{seis::make_grid(nz, nx, npml, multiple != 0), multiple ? 1 : 0, (float)dt,
+                   (float)h};
   const Grid& g = hc.g;
   const int S = shots_per_group;
   const TileMap tm = make_tiles(g, 1, 1, (int)((B + S - 1) / S));
@@ -1011,12 +1180,91 @@ int seis_rho_habc_forward(const float* vp, const float* rho, const float* b,
                                                         wavelet + n, (int)B, S, hc, C, 1, tm);
     SEIS_CHECK_LAUNCH();
     if (int e = illuminate(out, (int)B, g.plane, illum, s)) return e;
-    if (R > 0) {
-      k_habc_record<<<nblk(R), 256, 0, s>>>(out, rec_flat, R, rec_out + n * R, nullptr, 0,
-                                            (int)B, g);
+    float* slot = (ring && n + 1 < nt) ? ring + n * B * ring_len : nullptr;
+    const int64_t work = R + (slot ? B * ring_len : 0);
+    if (work > 0) {
+      k_habc_record<<<nblk(work), 256, 0, s>>>(out, rec_flat, R, R ? rec_out + n * R : nullptr,
+                                               slot, ring_len, (int)B, g);
+      SEIS_CHECK_LAUNCH();
+    }
+    if (snaps && n >= snap_every && n % snap_every == 0 && n <= nt - 2) {
+      float* sp = snaps + (n / snap_every - 1) * 2 * field;
+      k_habc_copy2<<<nblk(field), 256, 0, s>>>(un, sp, out, sp + field, field);
       SEIS_CHECK_LAUNCH();
     }
   }
+  return 0;
+}
+
+int seis_rho_habc_backward(const float* vp, const float* rho, const float* b,
+                           const float* wavelet, int64_t nt, const int32_t* src_zx,
+                           const int64_t* rec_flat, int64_t R, const float* rec_grad,
+                           const float* ring, const float* snaps, int32_t snap_every, float* buoy,
+                           float* u_a, float* u_b, float* l_a, float* l_b, float* l_c,
+                           float* grad_part, float* grad_vp, float* grad_rho, int64_t B,
+                           int64_t nz, int64_t nx, double dt, double h, int32_t npml,
+                           int32_t multiple, int32_t shots_per_group, void* stream) {
+  if (int e = habc_check(B, nz, nx, (float)dt, (float)h, 1)) return e;
+  SEIS_CHECK_ARG(shots_per_group >= 1, "shots_per_group must be >= 1, got %d", shots_per_group);
+  SEIS_CHECK_ARG(vp && rho && b && buoy && wavelet && src_zx && u_a && u_b && l_a && l_b && l_c &&
+                     grad_part && grad_vp && grad_rho,
+                 "null pointer argument");
+  SEIS_CHECK_ARG(nt >= 1, "nt must be >= 1");
+  SEIS_CHECK_ARG(R >= 0 && (R == 0 || (rec_flat && rec_grad)), "bad receiver arguments");
+  const int64_t ring_len = seis_ring_len(nz, nx, npml, multiple);
+  SEIS_CHECK_ARG(ring_len > 0, "grid has no interior inside npml=%d", npml);
+  SEIS_CHECK_ARG(ring || nt <= 2, "ring storage is required for nt > 2");
+  SEIS_CHECK_ARG(!snaps || snap_every > 0, "snapshots need snap_every > 0");
+  const HabcCtx hc{seis::make_grid(nz, nx, npml, multiple != 0), multiple ? 1 : 0, (float)dt,
+                   (float)h};
+  const Grid& g = hc.g;
+  const int S = shots_per_group;
+  const int G = (int)((B + S - 1) / S);
+  const TileMap tm = make_tiles(g, 1, 1, G);
+  const float C = (float)(dt * dt * std::pow(h, -2.0));
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t field = B * g.plane;
+  for (float* q : {l_a, l_b, l_c}) {
+    k_habc_fill<<<nblk(field), 256, 0, s>>>(q, field);
+    SEIS_CHECK_LAUNCH();
+  }
+  k_habc_fill<<<nblk(2 * (int64_t)G * g.plane), 256, 0, s>>>(grad_part, 2 * (int64_t)G * g.plane);
+  SEIS_CHECK_LAUNCH();
+  k_reciprocal<<<nblk(g.plane), 256, 0, s>>>(rho, buoy, g.plane);
+  SEIS_CHECK_LAUNCH();
+  float* X = u_b;   // u^{k-1}
+  float* Y = u_a;   // u^k -> u^{k-2}
+  float* L1 = l_a;  // lambda^{k+1}
+  float* L2 = l_b;  // lambda^{k+2}
+  float* L0 = l_c;  // lambda^k
+  for (int64_t kk = nt; kk >= 1; --kk) {
+    const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
+    k_rho_adj<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, buoy, b, L1, L2, L0, (int)B, S, hc,
+                                                   C, tm);
+    SEIS_CHECK_LAUNCH();
+    if (R > 0) {
+      k_rho_inject<<<nblk(R), 256, 0, s>>>(L0, rec_flat, rec_grad + (kk - 1) * R, R);
+      SEIS_CHECK_LAUNCH();
+    }
+    const float* slot = recon == 1 ? ring + (kk - 3) * B * ring_len : nullptr;
+    k_rho_grad<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, buoy, L0, X, Y, grad_part, src_zx,
+                                                    wavelet + (kk - 1), slot, ring_len, recon,
+                                                    (int)B, S, G, hc, C, tm);
+    SEIS_CHECK_LAUNCH();
+    const int64_t m = kk - 2;
+    if (snaps && m >= snap_every && m % snap_every == 0 && m <= nt - 2) {
+      const float* sp = snaps + (m / snap_every - 1) * 2 * field;
+      k_habc_copy2<<<nblk(field), 256, 0, s>>>(sp, Y, sp + field, X, field);
+      SEIS_CHECK_LAUNCH();
+    }
+    float* t = X; X = Y; Y = t;
+    t = L2; L2 = L1; L1 = L0; L0 = t;
+  }
+  k_habc_sum<<<nblk(g.plane), 256, 0, s>>>(grad_part, G, grad_vp, g.plane);
+  SEIS_CHECK_LAUNCH();
+  k_habc_sum<<<nblk(g.plane), 256, 0, s>>>(grad_part + (int64_t)G * g.plane, G, grad_rho,
+                                           g.plane);
+  SEIS_CHECK_LAUNCH();
   return 0;
 }
 

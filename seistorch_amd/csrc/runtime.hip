@@ -38,9 +38,34 @@ int prof_mark(int kernel, hipStream_t s) {
   return 0;
 }
 
+// Streaming copy for the HBM calibration: float4 loads and stores, a
+// grid-stride loop over all CUs with several blocks each.
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ a,
+                                                     float4* __restrict__ b, int64_t n4) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+    b[i] = a[i];
+}
+
 }  // namespace seis
 
 extern "C" {
+
+int seis_stream_copy(const float* src, float* dst, int64_t n, int32_t blocks, void* stream) {
+  SEIS_CHECK_ARG(src && dst && src != dst, "null or aliased pointer argument");
+  SEIS_CHECK_ARG(n > 0 && n % 4 == 0, "n must be a positive multiple of 4, got %lld",
+                 (long long)n);
+  SEIS_CHECK_ARG((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 16 == 0,
+                 "pointers must be 16-byte aligned");
+  SEIS_CHECK_ARG(blocks >= 1 && blocks <= 1 << 20, "blocks %d outside [1, 2^20]", blocks);
+  hipStream_t s = (hipStream_t)stream;
+  SEIS_PROF(SEIS_KERNEL_STREAM_COPY, s);
+  seis::k_stream_copy<<<blocks, 256, 0, s>>>(reinterpret_cast<const float4*>(src),
+                                              reinterpret_cast<float4*>(dst), n / 4);
+  SEIS_CHECK_LAUNCH();
+  SEIS_PROF(SEIS_KERNEL_STREAM_COPY, s);
+  return 0;
+}
 
 int seis_profile_enable(int32_t on) {
   seis::g_prof.on = on != 0;

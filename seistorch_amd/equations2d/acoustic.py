@@ -153,9 +153,9 @@ def _check_acquisition(vp, d, wavelet, src_zx, rec_flat, B, need_grad, multiple=
         raise RuntimeError(f"a receiver index lies outside the ({B}, {nz}, {nx}) fields")
     if need_grad:
         m = NPML + 1
-        if nz <= 2 * m or nx <= 2 * m:
-            raise RuntimeError(f"grid {nz}x{nx} has no interior inside the 50-cell boundary")
         z0 = 0 if multiple else m
+        if nz - m <= z0 or nx <= 2 * m:
+            raise RuntimeError(f"grid {nz}x{nx} has no interior inside the 50-cell boundary")
         if bool(d[z0:nz - m, m:nx - m].abs().max() != 0):
             raise RuntimeError("boundary-saving adjoint needs d == 0 inside the 50-cell boundary"
                                + (" (free surface: from row 0)" if multiple else ""))
@@ -204,8 +204,12 @@ class AcousticLoop(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, vp, d, wavelet, src_zx, rec_flat, B, dt, h, shape, snapshot_every,
-                two_step=True, two_step_adjoint=False, illum=None, multiple=False):
-        _lib.require_device(vp, d, wavelet, src_zx, rec_flat, illum)
+                two_step=True, two_step_adjoint=False, illum=None, multiple=False,
+                extra_src=None, extra_wav=None):
+        _lib.require_device(vp, d, wavelet, src_zx, rec_flat, illum, extra_src, extra_wav)
+        n_extra = 0 if extra_src is None else int(extra_src.shape[0])
+        if n_extra:
+            two_step = False  # the extra sources are added between single steps
         nz, nx = vp.shape
         if illum is not None and (illum.shape != (nz, nx) or illum.dtype != torch.float32
                                   or not illum.is_contiguous()):
@@ -241,7 +245,8 @@ class AcousticLoop(torch.autograd.Function):
         if illum is not None and u_c is not None:
             ipart = torch.empty(((B + S - 1) // S, nz, nx), device=vp.device, dtype=torch.float32)
         _lib.call("seis_acoustic_forward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
-                  _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
+                  _lib.ptr(src_zx), _lib.ptr(extra_src), _lib.ptr(extra_wav), n_extra,
+                  _lib.ptr(rec_flat), R, _lib.ptr(rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K, _lib.ptr(illum), _lib.ptr(ipart), _lib.ptr(u_a),
                   _lib.ptr(u_b),
                   _lib.ptr(u_c), _lib.ptr(u_d), B, nz, nx, dt, h, NPML, int(multiple), S, rz,
@@ -250,16 +255,18 @@ class AcousticLoop(torch.autograd.Function):
         check_finite(rec)
         if need_grad:
             ctx.save_for_backward(vp, d, wavelet, src_zx, rec_flat)
-            ctx.state = (ring, snaps, u_a, u_b)
-            ctx.meta = (B, dt, h, S, K, rz, vx, bool(two_step_adjoint), int(multiple))
+            ctx.state = (ring, snaps, u_a, u_b, extra_src, extra_wav)
+            ctx.meta = (B, dt, h, S, K, rz, vx, bool(two_step_adjoint) and not n_extra,
+                        int(multiple))
         return rec
 
     @staticmethod
     @torch.autograd.function.once_differentiable
     def backward(ctx, grad_rec):
         vp, d, wavelet, src_zx, rec_flat = ctx.saved_tensors
-        ring, snaps, u_a, u_b = ctx.state
+        ring, snaps, u_a, u_b, extra_src, extra_wav = ctx.state
         ctx.state = None  # the backward consumes the final forward state
+        n_extra = 0 if extra_src is None else int(extra_src.shape[0])
         B, dt, h, S, K, rz, vx, two_step_adjoint, multiple = ctx.meta
         nz, nx = vp.shape
         nt = wavelet.shape[0]
@@ -276,17 +283,41 @@ class AcousticLoop(torch.autograd.Function):
             if ok:
                 extra = [torch.empty_like(u_a) for _ in range(4)] + [rmap]
         _lib.call("seis_acoustic_backward", _lib.ptr(vp), _lib.ptr(d), _lib.ptr(wavelet), nt,
-                  _lib.ptr(src_zx), _lib.ptr(rec_flat), R, _lib.ptr(grad_rec), _lib.ptr(ring),
+                  _lib.ptr(src_zx), _lib.ptr(extra_src), _lib.ptr(extra_wav), n_extra,
+                  _lib.ptr(rec_flat), R, _lib.ptr(grad_rec), _lib.ptr(ring),
                   _lib.ptr(snaps), K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b),
                   _lib.ptr(mu_a), _lib.ptr(mu_b), *[_lib.ptr(t) for t in extra],
                   _lib.ptr(gpart), _lib.ptr(grad), B, nz, nx, dt, h, NPML, multiple, S, rz, vx,
                   _lib.stream_of(vp))
-        return (grad,) + (None,) * 13
+        return (grad,) + (None,) * 15
+
+
+def _check_extra_sources(extra_sources, B, nz, nx, nt, need_grad, multiple):
+    """(extra_src int32 (n, 3) of (shot, z, x), extra_wav float32 (n, nt)) or
+    (None, None); the kernels index with these values unchecked."""
+    if extra_sources is None:
+        return None, None
+    es, ew = extra_sources
+    if es.dtype != torch.int32 or es.dim() != 2 or es.shape[1] != 3:
+        raise RuntimeError("extra source table must be int32 (n, 3): shot, z, x")
+    if ew.dtype != torch.float32 or tuple(ew.shape) != (es.shape[0], nt):
+        raise RuntimeError(f"extra wavelets must be float32 ({es.shape[0]}, {nt})")
+    if es.shape[0] == 0:
+        return None, None
+    lo = 0 if multiple else NPML + 1
+    zlo, zhi, xlo, xhi = (lo, nz - NPML - 1, NPML + 1, nx - NPML - 1) if need_grad \
+        else (0, nz, 0, nx)
+    bad = (es[:, 0] < 0) | (es[:, 0] >= B) | (es[:, 1] < zlo) | (es[:, 1] >= zhi) \
+        | (es[:, 2] < xlo) | (es[:, 2] >= xhi)
+    if bool(bad.any()):
+        raise RuntimeError("an extra source lies outside the grid"
+                           + (" interior that the adjoint rebuilds" if need_grad else ""))
+    return es.contiguous(), ew.contiguous()
 
 
 def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
               snapshot_every=None, rows_per_thread=None, vec_x=None, two_step=True,
-              two_step_adjoint=False, illum=None, multiple=False):
+              two_step_adjoint=False, illum=None, multiple=False, extra_sources=None):
     """Records (nt, R) of B shots; differentiable w.r.t. vp through the adjoint.
 
     The launch shape -- shots_per_group (1, 2, 4, 8), rows_per_thread
@@ -305,8 +336,18 @@ def propagate(vp, d, wavelet, src_zx, rec_flat, B, dt, h, shots_per_group=None,
     has no top row and the gradient covers rows [0, nz-50) of the interior
     columns, as the reference's _time_step_backward_multiple rebuilds them
     (acoustic.py:114-137).  Non-finite records raise ValueError (rnn.py:215).
+    extra_sources: None, or (table, wavelets): more point sources, each with
+    its own wavelet, added after every step like the shot's own source (the
+    source encoding of codingfwi.py:184-334, source.py:54-55: many encoded
+    sources in one field); table int32 (n, 3) rows (shot, z, x), wavelets
+    float32 (n, nt).  With a gradient they must lie in the rebuilt interior.
+    They run one time step per launch.
     """
+    nt = wavelet.shape[0]
+    es, ew = _check_extra_sources(extra_sources, int(B), vp.shape[0], vp.shape[1], nt,
+                                  vp.requires_grad and torch.is_grad_enabled(), multiple)
     return AcousticLoop.apply(vp, d, wavelet.contiguous(), src_zx.contiguous(),
                               rec_flat.contiguous(), int(B), _f(dt), _f(h),
                               (shots_per_group, rows_per_thread, vec_x), snapshot_every,
-                              bool(two_step), bool(two_step_adjoint), illum, bool(multiple))
+                              bool(two_step), bool(two_step_adjoint), illum, bool(multiple), es,
+                              ew)

@@ -23,6 +23,7 @@ at smoothed start models only).
 import time
 from dataclasses import dataclass, field
 
+import numpy as np
 import torch
 import torch.nn.functional as F
 
@@ -95,16 +96,20 @@ class Problem:
         with torch.no_grad():
             return float(torch.sum((self._filter(self.records(vp)) - self.observed()) ** 2))
 
-    def gradient(self, vp, illum=None):
-        """(misfit, interior gradient); illum: optional (nz, nx) buffer that
-        receives the forward's source illumination."""
+    def gradient(self, vp, illum=None, padded=False):
+        """(misfit, gradient): the interior gradient, or with padded=True the
+        gradient over the padded grid (zero in the boundary strips, which the
+        boundary-saving adjoint does not rebuild); the top `mute` interior
+        rows are zeroed.  illum: optional (nz, nx) buffer that receives the
+        forward's source illumination."""
         v = vp.detach().clone().requires_grad_(True)
         rec = self._propagate(v, illum=illum)
         loss = torch.sum((self._filter(rec) - self.observed()) ** 2)
         loss.backward()
-        g = self.interior(v.grad).clone()
-        g[:self.mute] = 0.0
-        return float(loss), g
+        g = v.grad.clone()
+        if self.mute:
+            g[PAD:PAD + self.mute] = 0.0
+        return float(loss), (g if padded else self.interior(g).clone())
 
     def rmse(self, vp):
         return float(torch.sqrt(torch.mean((self.interior(vp) - self.interior(self.vp_true)) ** 2)))
@@ -236,6 +241,28 @@ def _gd_direction(g, grad_clamp=True):
     return g / m if m > 0 else g
 
 
+def postprocess_padded(g_pad, smooth=None, seabed=None, illum=None, multiple=False):
+    """The reference's post-processing of a gradient over the padded grid
+    (seistorch_dist.py:270-280 with process.py:26-117): the seabed mask of
+    the unpadded grid is zero-padded (no top pad with a free surface,
+    process.py:26-40); then g * mask, `counts` Gaussian passes over the padded
+    gradient (smooth: cfg['training']['smooth']), g * mask, g / illum -- each
+    step when its argument is given.  Returns the interior, the part the loop
+    updates: cells near its edges see the strips' values through the
+    smoothing, as in the reference, not a reflection at the interior edge."""
+    from . import process
+    mask = None
+    top = 0 if multiple else PAD
+    if seabed is not None:
+        mask = F.pad(seabed[None, None].float(), (PAD, PAD, top, PAD), mode="constant",
+                     value=0.0)[0, 0].contiguous()
+    if smooth is not None or mask is not None or illum is not None:
+        g_pad = process.smooth_gradient(g_pad.contiguous(), smooth,
+                                        pre_mask=mask if smooth is not None else None,
+                                        post_mask=mask, illum=illum)
+    return g_pad[top:g_pad.shape[0] - PAD, PAD:g_pad.shape[1] - PAD].contiguous()
+
+
 def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85,
                    lr_decay=0.995, grad_clamp=True, smooth=None, seabed=None,
                    source_illumination=False, log=None):
@@ -253,9 +280,8 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
     shape) switch on --grad-smooth and --grad-cut (seistorch_dist.py:270-275),
     source_illumination --source-illumination (:279-280: the gradient divided
     by the forward's illumination), on the HIP post-processing kernels,
-    applied to the interior gradient the loop optimises.  Returns
-    (vp, History)."""
-    from . import process
+    applied to the padded gradient before the interior is updated
+    (postprocess_padded).  Returns (vp, History)."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     vi = p.interior(p.vp_init).clone().contiguous()
@@ -272,12 +298,8 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
             cur = slr
             for ep in range(epochs):
                 illum = torch.empty_like(p.vp_true) if source_illumination else None
-                loss, g = p.gradient(p.pad(x.detach()), illum=illum)
-                if smooth is not None or seabed is not None or illum is not None:
-                    g = process.smooth_gradient(
-                        g.contiguous(), smooth, pre_mask=seabed if smooth is not None else None,
-                        post_mask=seabed,
-                        illum=p.interior(illum).contiguous() if illum is not None else None)
+                loss, g = p.gradient(p.pad(x.detach()), illum=illum, padded=True)
+                g = postprocess_padded(g, smooth, seabed, illum)
                 if opt is not None:
                     opt.zero_grad()
                     x.grad = g.contiguous()
@@ -304,7 +326,7 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
 
 # ------------------------------------------------ configuration-driven runs
 
-def problem_from_config(cfg, device="cuda", shots=None):
+def problem_from_config(cfg, device="cuda", shots=None, mute=0):
     """A Problem from the reference's configuration keys (template.yml;
     seistorch_dist.py:90-160): ``equation`` ("acoustic" with a PML boundary,
     "acoustic_habc", or "elastic": an ElasticProblem with vp, vs and rho,
@@ -315,7 +337,9 @@ def problem_from_config(cfg, device="cuda", shots=None):
     geom.wavelet .npy (setup.py:232-252), geom.h, geom.boundary.width and
     training.filter_ord.  The fused path's ring sits at index 49, so the
     boundary width must be 50; multiples (no top boundary) are not supported
-    here.  shots: the shot numbers to use (all by default)."""
+    here.  shots: the shot numbers to use (all by default).  mute: interior
+    rows below the surface whose gradient is zeroed (0: none, as the
+    reference's loop, which cuts only through the seabed mask)."""
     from . import io as sio
     from .habc import generate_habc_coefficients_2d
     from .pml import generate_pml_coefficients_2d
@@ -377,10 +401,11 @@ def problem_from_config(cfg, device="cuda", shots=None):
                            src_type=tuple(geom.get("source_type") or ("txx", "tzz")),
                            rec_type=tuple(geom.get("receiver_type") or ("vx", "vz")),
                            invlist=tuple(k for k in ("vp", "vs", "rho") if inv.get(k)),
-                           filter_ord=filter_ord)
+                           filter_ord=filter_ord, mute=mute)
         nch = len(p.rec_type)
     else:
-        p = Problem(vp_true, vp_init, *common, filter_ord=filter_ord, equation=equation)
+        p = Problem(vp_true, vp_init, *common, filter_ord=filter_ord, equation=equation,
+                    mute=mute)
         nch = 1
     if geom.get("obsPath"):
         records = sio.SeisRecord(cfg)
@@ -395,7 +420,7 @@ def problem_from_config(cfg, device="cuda", shots=None):
 
 
 def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smooth=False,
-                    grad_cut=False, source_illumination=False, log=None):
+                    grad_cut=False, source_illumination=False, log=None, mute=0):
     """The multiscale FWI loop of seistorch_dist.py:166-303 driven by a
     configuration: geom.multiscale bands, training.N_epochs, lr (a number or
     {"vp": lr}), scale_decay, lr_decay; the command-line switches
@@ -404,7 +429,7 @@ def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smoot
     Returns (Problem, vp, History); for the elastic equation (ElasticProblem,
     {name: padded model}, History)."""
     from . import io as sio
-    p = problem_from_config(cfg, device, shots)
+    p = problem_from_config(cfg, device, shots, mute=mute)
     if p.obs is None:
         raise ValueError("geom.obsPath is required for an inversion")
     tr = cfg.get("training", {})
@@ -429,6 +454,110 @@ def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smoot
                               smooth=tr.get("smooth") if grad_smooth else None, seabed=seabed,
                               source_illumination=source_illumination, log=log)
     return p, vp, hist
+
+
+# ---------------------------------------------------------- source encoding
+
+def encode_shots(rng, wavelet, records, shots, split=0.2):
+    """codingfwi.py:214-234 with roll() of utils.py:201-215: every chosen shot
+    gets a random polarity (-1)**p, p in {1, 2}, and a random time shift tau
+    in [0, split*nt); its wavelet and its records are shifted by tau (the
+    first tau samples zero) and signed.  wavelet: (nt,) or (B, nt);
+    records: (nt, B, Rs).  Returns (wavelets (n, nt), summed records
+    (nt, Rs), [(shot, sign, tau)])."""
+    nt = records.shape[0]
+    n_shift = int(np.ceil(split * nt))
+    wavs = torch.zeros((len(shots), nt), device=records.device, dtype=torch.float32)
+    summed = torch.zeros((nt, records.shape[2]), device=records.device, dtype=torch.float32)
+    codes = []
+    for i, b in enumerate(shots):
+        p = int(rng.integers(1, 3))
+        tau = int(rng.integers(0, n_shift))
+        sign = -1.0 if p % 2 else 1.0
+        w = wavelet if wavelet.dim() == 1 else wavelet[b]
+        wavs[i, tau:] = sign * w[:nt - tau]
+        summed[tau:] += sign * records[:nt - tau, b]
+        codes.append((int(b), sign, tau))
+    return wavs, summed, codes
+
+
+def run_encoded(p, scales, epochs, lr=5.0, optimizer="adam", batch=None, split=0.2,
+                scale_decay=0.85, lr_decay=0.995, grad_clamp=True, smooth=None, seabed=None,
+                seed=20230503, log=None):
+    """The source-encoding FWI loop (codingfwi.py:184-334) on one GPU: per
+    frequency scale the observed records and the wavelet are band-filtered
+    (scipy backend, as codingfwi does, signal.py:49-96); per epoch all shots,
+    or `batch` random ones, are encoded (encode_shots) into one field -- the
+    first chosen shot's source carries its coded wavelet, the others enter as
+    extra sources -- and one forward and one adjoint give the L2-misfit
+    gradient against the summed coded records; post-processing and the update
+    as in run_multiscale.  Needs every shot to record at the same receiver
+    cells (the reference's fixed-receiver case).  Returns (vp, History)."""
+    from . import signal as sg
+    nt = p.wavelet.shape[0]
+    nz, nx = p.vp_init.shape
+    plane = nz * nx
+    Rs = p.rec.numel() // p.B
+    rf = p.rec.view(p.B, Rs) - torch.arange(p.B, device=p.rec.device)[:, None] * plane
+    if not bool((rf == rf[0]).all()):
+        raise ValueError("source encoding needs the same receiver cells for every shot")
+    rec0 = rf[0].contiguous()
+    rng = np.random.default_rng(seed)
+    seis = sg.SeisSignal(dt=p.dt, filter_ord=p.filter_ord)
+    obs_all = p.obs.view(nt, p.B, Rs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vi = p.interior(p.vp_init).clone().contiguous()
+    hist = History()
+    for si, band in enumerate(scales):
+        if band == "all":
+            obs_b, wav_b = obs_all, p.wavelet
+        else:
+            host = obs_all.permute(1, 0, 2).cpu().numpy()
+            filt = seis.filter(list(host), band, backend="scipy")
+            obs_b = torch.from_numpy(np.stack(list(filt), 1)).to(p.obs.device)
+            wav_b = torch.from_numpy(seis.filter([p.wavelet.cpu().numpy()], band,
+                                                 backend="scipy")[0]).to(p.obs.device)
+        slr = lr * scale_decay ** si
+        x = torch.nn.Parameter(vi.clone())
+        opt = torch.optim.Adam([x], lr=slr, eps=1e-22) if optimizer == "adam" else None
+        sched = torch.optim.lr_scheduler.ExponentialLR(opt, lr_decay) if opt else None
+        cur = slr
+        for ep in range(epochs):
+            shots = (rng.choice(p.B, batch, replace=False) if batch else np.arange(p.B)).tolist()
+            wavs, coded_obs, _ = encode_shots(rng, wav_b, obs_b, shots, split)
+            src = p.src[shots[0]:shots[0] + 1].contiguous()
+            es = torch.zeros((len(shots) - 1, 3), dtype=torch.int32, device=src.device)
+            if len(shots) > 1:
+                es[:, 1:] = p.src[shots[1:]]
+            v = p.pad(x.detach()).requires_grad_(True)
+            syn = ac.propagate(v, p.d, wavs[0].contiguous(), src, rec0, 1, p.dt, p.h,
+                               extra_sources=(es, wavs[1:].contiguous()))
+            loss = torch.sum((syn - coded_obs) ** 2)
+            loss.backward()
+            g = v.grad
+            if p.mute:
+                g[PAD:PAD + p.mute] = 0.0
+            g = postprocess_padded(g, smooth, seabed)
+            if opt is not None:
+                opt.zero_grad()
+                x.grad = g.contiguous()
+                opt.step()
+                sched.step()
+            else:
+                with torch.no_grad():
+                    x -= cur * _gd_direction(g, grad_clamp)
+                cur *= lr_decay
+            with torch.no_grad():
+                x.clamp_(p.vmin, p.vmax)
+            torch.cuda.synchronize()
+            hist.add(time.perf_counter() - t0, float(loss), p.rmse(p.pad(x.detach())),
+                     f"scale{si}")
+            if log:
+                log(f"[encoded] band {band} epoch {ep + 1}: shots {len(shots)}, misfit "
+                    f"{float(loss):.4e} rmse {hist.rmse[-1]:.1f}")
+        vi = x.detach().contiguous()
+    return p.pad(vi), hist
 
 
 # ---------------------------------------------------------- elastic FWI
@@ -475,17 +604,20 @@ class ElasticProblem:
         with torch.no_grad():
             return self._propagate(m)
 
-    def gradient(self, m, illum=None):
-        """(misfit, {name: interior gradient} for the inverted parameters);
-        illum: optional (nz, nx) buffer for the forward's source illumination."""
+    def gradient(self, m, illum=None, padded=False):
+        """(misfit, {name: gradient} for the inverted parameters): interior
+        gradients, or over the padded grid with padded=True (see
+        Problem.gradient); illum: optional (nz, nx) buffer for the forward's
+        source illumination."""
         mm = {k: (v.detach().clone().requires_grad_(k in self.invlist)) for k, v in m.items()}
         loss = torch.sum((self._filter(self._propagate(mm, illum)) - self.observed()) ** 2)
         loss.backward()
         g = {}
         for k in self.invlist:
-            gk = self.interior(mm[k].grad).clone()
-            gk[:self.mute] = 0.0
-            g[k] = gk
+            gk = mm[k].grad.clone()
+            if self.mute:
+                gk[PAD:PAD + self.mute] = 0.0
+            g[k] = gk if padded else self.interior(gk).clone()
         return float(loss), g
 
     def rmse(self, m, name="vp"):
@@ -504,7 +636,6 @@ def run_multiscale_elastic(p, scales, epochs, lr, optimizer="adam", scale_decay=
     ELASTIC_BOUNDS after the update; smooth, seabed and source_illumination
     post-process every inverted parameter's gradient as in run_multiscale.
     Returns (models, History)."""
-    from . import process
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     inv = list(p.invlist)
@@ -529,14 +660,9 @@ def run_multiscale_elastic(p, scales, epochs, lr, optimizer="adam", scale_decay=
             step = {k: lrs[k] * scale_decay ** si for k in inv}
             for ep in range(epochs):
                 illum = torch.empty_like(p.models_init["vp"]) if source_illumination else None
-                loss, g = p.gradient(models_of(xs), illum)
-                if smooth is not None or seabed is not None or illum is not None:
-                    il = p.interior(illum).contiguous() if illum is not None else None
-                    for k in inv:
-                        g[k] = process.smooth_gradient(
-                            g[k].contiguous(), smooth,
-                            pre_mask=seabed if smooth is not None else None, post_mask=seabed,
-                            illum=il)
+                loss, g = p.gradient(models_of(xs), illum, padded=True)
+                for k in inv:
+                    g[k] = postprocess_padded(g[k], smooth, seabed, illum)
                 if opt is not None:
                     opt.zero_grad()
                     for k in inv:

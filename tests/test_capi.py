@@ -44,9 +44,9 @@ def test_ring_len_and_error_path():
     with pytest.raises(RuntimeError, match="batch size"):
         _lib.call("seis_acoustic_step", None, None, None, None, None, 0, 10, 10, 1e-3, 10.0, None)
     with pytest.raises(RuntimeError, match="launch shape"):
-        _lib.call("seis_acoustic_forward", None, None, None, 10, None, None, 0, None, None, None,
-                  0, None, None, None, None, None, None, 1, 120, 120, 1e-3, 10.0, 49, 0, 3, 2, 4,
-                  None)
+        _lib.call("seis_acoustic_forward", None, None, None, 10, None, None, None, 0, None, 0,
+                  None, None, None, 0, None, None, None, None, None, None, 1, 120, 120, 1e-3,
+                  10.0, 49, 0, 3, 2, 4, None)
     with pytest.raises(RuntimeError, match="null pointer"):
         _lib.call("seis_acoustic_step", None, None, None, None, None, 1, 10, 10, 1e-3, 10.0, None)
 
@@ -134,11 +134,29 @@ def test_variable_density_and_illumination_argument_checks():
         _lib.call("seis_rho_habc_step", f1, None, f1, f1, f1, f2, 1, 120, 120, 1e-3, 10.0, 0, 1,
                   None)
     with pytest.raises(RuntimeError, match="shots_per_group"):
-        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 10, f1, f1, 0, None, f1, None, f1,
-                  f1, f1, 1, 120, 120, 1e-3, 10.0, 0, 0, None)
+        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 10, f1, f1, 0, None, None, None, 0,
+                  f1, None, f1, f1, f1, 1, 120, 120, 1e-3, 10.0, 49, 0, 0, None)
     with pytest.raises(RuntimeError, match="nt must be"):
-        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 0, f1, f1, 0, None, f1, None, f1,
-                  f1, f1, 1, 120, 120, 1e-3, 10.0, 0, 4, None)
+        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 0, f1, f1, 0, None, None, None, 0,
+                  f1, None, f1, f1, f1, 1, 120, 120, 1e-3, 10.0, 49, 0, 4, None)
+    with pytest.raises(RuntimeError, match="snap_every"):
+        _lib.call("seis_rho_habc_forward", f1, f1, f1, f1, 10, f1, f1, 0, None, f1, f2, 0,
+                  f1, None, f1, f1, f1, 1, 120, 120, 1e-3, 10.0, 49, 0, 4, None)
+    with pytest.raises(RuntimeError, match="null pointer"):
+        _lib.call("seis_rho_habc_backward", f1, f1, f1, f1, 10, f1, f1, 0, None, f1, None, 0,
+                  f1, f1, f1, f1, f1, f1, f1, None, f1, 1, 120, 120, 1e-3, 10.0, 49, 0, 4, None)
+    with pytest.raises(RuntimeError, match="ring storage"):
+        _lib.call("seis_rho_habc_backward", f1, f1, f1, f1, 10, f1, f1, 0, None, None, None, 0,
+                  f1, f1, f1, f1, f1, f1, f1, f1, f1, 1, 120, 120, 1e-3, 10.0, 49, 0, 4, None)
     with pytest.raises(RuntimeError, match="illum must not alias"):
-        _lib.call("seis_acoustic_forward", f1, f1, f1, 10, f1, None, 0, None, None, None, 0, f1,
-                  None, f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 0, 1, 1, 1, None)
+        _lib.call("seis_acoustic_forward", f1, f1, f1, 10, f1, None, None, 0, None, 0, None, None,
+                  None, 0, f1, None, f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 0, 1, 1, 1,
+                  None)
+    with pytest.raises(RuntimeError, match="extra-source"):
+        _lib.call("seis_acoustic_forward", f1, f1, f1, 10, f1, None, None, 2, None, 0, None, None,
+                  None, 0, None, None, f1, f2, None, None, 1, 120, 120, 1e-3, 10.0, 49, 0, 1, 1, 1,
+                  None)
+    with pytest.raises(RuntimeError, match="one time step per launch"):
+        _lib.call("seis_acoustic_forward", f1, f1, f1, 10, f1, f1, f1, 2, None, 0, None, None,
+                  None, 0, None, None, f1, f2, ctypes.c_void_p(768), ctypes.c_void_p(1024), 1, 120,
+                  120, 1e-3, 10.0, 49, 0, 1, 1, 1, None)

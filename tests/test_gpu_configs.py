@@ -203,7 +203,15 @@ def test_fwi_shot_independence_and_linearity(fwi):
     log(f"FWI shots 0, 31, 63 alone vs in the 64-shot launch: max |diff| {e_shot:.3e}; "
         f"records(2w) - 2 records(w): max {e_lin:.3e} of the peak")
     assert e_shot == 0.0
-    assert e_lin <= 1e-6
+    # FP32 records are homogeneous in the wavelet only up to gradual
+    # underflow: the stencil's precursor ahead of the wavefront passes through
+    # subnormal magnitudes, where the relative precision is coarse, and
+    # doubling the wavelet shifts which values are subnormal; the precursor
+    # then grows into the wave with that relative error (the FP32 torch
+    # restatement on the CPU shows ~2e-6 of the peak on a 160x190 case, the
+    # HIP path 3.3e-6 on this grid, both with and without two steps per
+    # launch, and 0 between two runs with the same wavelet)
+    assert e_lin <= 2e-5
 
 
 def test_fwi_one_shot_full_grid_against_oracles(fwi):
@@ -270,12 +278,15 @@ def test_elastic_finite_differences_full_size(elastic):
     """Central differences of J = sum(G * records) at two seeded cells below
     the water layer per parameter (vp, vs: the stiffnesses are quadratic in
     them, so the central difference is exact to second order; rho: 1/rho
-    enters the velocity update, truncation ~(step/rho)^2 ~ 2e-4)."""
+    enters the velocity update, truncation ~(step/rho)^2 ~ 5e-4).  Steps of
+    100, 100 m/s and 50 kg/m^3 keep the difference of the two FP32
+    propagations well above their rounding noise (a 30 m/s vs step gave 3 %
+    at a cell with 1.6 % of the peak gradient)."""
     from seistorch_amd.equations2d import elastic as el
     P, t, G, _, grads = elastic
     rng = np.random.default_rng(20230504)
     worst = 0.0
-    for pi, (name, step) in enumerate((("vp", 50.0), ("vs", 30.0), ("rho", 30.0))):
+    for pi, (name, step) in enumerate((("vp", 100.0), ("vs", 100.0), ("rho", 50.0))):
         g = grads[pi].cpu().numpy()
         gmax = np.abs(g[INTERIOR]).max()
         for _ in range(2):
@@ -316,7 +327,7 @@ def test_elastic_shot_independence_and_linearity(elastic):
     log(f"elastic shots 0, 31 alone vs in the 32-shot launch: max |diff| {e_shot:.3e}; "
         f"records(2w) - 2 records(w): max {e_lin:.3e} of the peak")
     assert e_shot == 0.0
-    assert e_lin <= 1e-6
+    assert e_lin <= 2e-5  # gradual underflow, see the acoustic case
 
 
 def test_elastic_one_shot_against_restatements(elastic):
@@ -396,11 +407,13 @@ def test_uneven_batch_sampled_instances_against_fp64():
         eg = (rel_l2(g[INTERIOR], gbs[INTERIOR]), rel_l2(g[INTERIOR], g64[INTERIOR]),
               rel_l2(gbs[INTERIOR], g64[INTERIOR]))
         eJ = abs(float(loss[q]) - J64) / J64
+        eJ_ref = abs(float(((s32.cpu().numpy() - o32).astype(np.float64) ** 2).sum()) - J64) / J64
         log(f"uneven batch instance {q} ({it['nz']}x{it['nx']} padded, {it['B']} shots, "
             f"nt={it['nt']}): records HIP vs FP32 restatement {er[0]:.2e}, HIP vs FP64 "
-            f"{er[1]:.2e}, restatement vs FP64 {er[2]:.2e}; misfit rel. diff vs FP64 {eJ:.2e}; "
+            f"{er[1]:.2e}, restatement vs FP64 {er[2]:.2e}; misfit vs FP64: HIP {eJ:.2e}, "
+            f"restatement {eJ_ref:.2e}; "
             f"misfit gradient HIP vs FP32 BS restatement {eg[0]:.2e}, HIP vs FP64 {eg[1]:.2e}, "
             f"BS restatement vs FP64 {eg[2]:.2e}")
         assert er[0] < TOL and er[1] <= 2 * er[2] + 1e-6
         assert eg[1] <= 2 * eg[2] + 1e-6
-        assert eJ < 1e-4
+        assert eJ <= 2 * eJ_ref + 1e-6

@@ -171,3 +171,41 @@ def test_one_count_equals_two_single_axis_passes(shape, rz, rx):
     two = pp.gaussian_filter(pp.gaussian_filter(g, 2.0, rz, 0), 3.0, rx, 1)
     print(f"one count {shape} r=({rz},{rx}): equal to two passes {torch.equal(fused, two)}")
     assert torch.equal(fused, two)
+
+
+@pytest.mark.parametrize("multiple", [False, True])
+def test_fwi_postprocess_on_the_padded_gradient(multiple):
+    """fwi.postprocess_padded against the reference's own steps on the padded
+    gradient (seistorch_dist.py:270-280, process.py:26-117): seabed mask of
+    the unpadded grid zero-padded (no top pad with a free surface), g * mask,
+    counts z and x passes (numpy reflect pad -> conv2d -> crop, FP32), g *
+    mask, g / illumination, then the interior crop.  Cells near the interior
+    edges see the strips' values through the smoothing."""
+    from seistorch_amd import fwi
+    P = 50
+    nz_i, nx_i = 40, 70
+    nz, nx = nz_i + (P if multiple else 2 * P), nx_i + 2 * P
+    rng = np.random.default_rng(3)
+    g = np.zeros((nz, nx), np.float32)
+    top = 0 if multiple else P
+    g[top:top + nz_i, P:P + nx_i] = rng.standard_normal((nz_i, nx_i)).astype(np.float32)
+    seabed = np.ones((nz_i, nx_i), np.float32)
+    seabed[:6] = 0.0
+    illum = (rng.random((nz, nx)) + 0.5).astype(np.float32)
+    smooth = {"counts": 2, "sigma": {"z": 2.0, "x": 3.0}, "radius": {"z": 4, "x": 6}}
+    got = fwi.postprocess_padded(torch.from_numpy(g).cuda(), smooth,
+                                 torch.from_numpy(seabed).cuda(), torch.from_numpy(illum).cuda(),
+                                 multiple=multiple).cpu().numpy()
+    mask = np.pad(seabed, ((top, P), (P, P)))
+    ref = g * mask
+    for _ in range(smooth["counts"]):
+        ref = _fp32_reference_steps(ref, 2.0, 4, 0)
+        ref = _fp32_reference_steps(ref, 3.0, 6, 1)
+    ref = ref * mask / illum
+    ref = ref[top:nz - P, P:nx - P]
+    assert got.shape == ref.shape
+    e = np.abs(got - ref).max() / np.abs(ref).max()
+    edge = np.abs(got[:, :8] - ref[:, :8]).max() / np.abs(ref).max()
+    print(f"fwi post-processing on the padded gradient, multiple={multiple}: max {e:.2e} of the "
+          f"peak (first 8 interior columns {edge:.2e})")
+    assert e < 1e-5
