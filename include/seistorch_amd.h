@@ -422,6 +422,20 @@ int seis_smooth_gradient(const float* grad, float* out, float* work, int64_t nz,
                          const float* pre_mask, const float* post_mask, const float* illum,
                          void* stream);
 
+/* seis_smooth_gradient without the work argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * work plane is a static workspace of SEIS_NOALLOC_MAX_CELLS FP32 cells
+ * (16 MiB) inside the library, one per device, held until the process ends;
+ * nothing is allocated and results equal seis_smooth_gradient's bit for bit.
+ * Grids of more than SEIS_NOALLOC_MAX_CELLS cells are rejected (code 1).
+ * Calls that overlap on one device share the workspace and race: serialise
+ * them (one stream, or events). */
+#define SEIS_NOALLOC_MAX_CELLS (1LL << 22)
+int seis_smooth_gradient_noalloc(const float* grad, float* out, int64_t nz, int64_t nx,
+                                 const float* wz, int32_t kz, const float* wx, int32_t kx,
+                                 int32_t counts, const float* pre_mask, const float* post_mask,
+                                 const float* illum, void* stream);
 
 /* ---------------- learned gradient preconditioner (U-Net) ----------------- */
 /* No reference counterpart: a small U-Net maps (gradient, model) to an

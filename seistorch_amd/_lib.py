@@ -80,6 +80,8 @@ SIGNATURES = {
     "seis_gaussian_filter": (ctypes.c_int, [_p, _p, _i64, _i64, _i32, _p, _i32, _p]),
     "seis_smooth_gradient": (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _i32, _p, _i32, _i32,
                                             _p, _p, _p, _p]),
+    "seis_smooth_gradient_noalloc": (ctypes.c_int, [_p, _p, _i64, _i64, _p, _i32, _p, _i32,
+                                                    _i32, _p, _p, _p, _p]),
     "seis_unet_conv3x3": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _p, _p, _i32, _i32, _p, _p,
                                          _p, _p]),
     "seis_unet_flip_weights": (ctypes.c_int, [_p, _i32, _i32, _p, _p]),

@@ -25,6 +25,19 @@
 #include "common.h"
 #include "seistorch_amd.h"
 
+#include <mutex>
+
+// Workspace of seis_smooth_gradient_noalloc: one FP32 plane of at most
+// SEIS_NOALLOC_MAX_CELLS cells (16 MiB), part of the library's code object, so
+// each device that runs the library holds its own copy until the process
+// ends.  Host code never uses the name `scratch` as an address (on the host
+// it names a stand-in, not device memory): scratch_ptr() resolves the
+// current device's copy once with hipGetSymbolAddress and keeps it.  Every
+// pass writes every cell of its destination before the next pass reads it,
+// so nothing here has to start at zero; calls that overlap on one device
+// share the plane and race.
+static __device__ __align__(256) float scratch[SEIS_NOALLOC_MAX_CELLS];
+
 namespace {
 
 constexpr int kMaxTaps = 129;  // radius <= 64
@@ -179,38 +192,14 @@ void launch_gauss(int axis, const float* in, float* out, const float* pre, const
   }
 }
 
-}  // namespace
-
-extern "C" {
-
-int seis_gaussian_filter(const float* x, float* y, int64_t nz, int64_t nx, int32_t axis,
-                         const float* w, int32_t ks, void* stream) {
-  SEIS_CHECK_ARG(x && y && w, "null pointer argument");
-  SEIS_CHECK_ARG(x != y, "y must not alias x");
-  SEIS_CHECK_ARG(axis == 0 || axis == 1, "axis must be 0 (z) or 1 (x), got %d", axis);
-  SEIS_CHECK_ARG(ks >= 1 && ks <= kMaxTaps, "kernel size %d outside [1, %d]", ks, kMaxTaps);
-  SEIS_CHECK_ARG(nz >= 1 && nx >= 1, "empty field (%lld x %lld)", (long long)nz, (long long)nx);
-  const int64_t n = axis == 0 ? nz : nx;
-  SEIS_CHECK_ARG(ks / 2 < n, "half kernel size %d must be smaller than the axis length %lld",
-                 ks / 2, (long long)n);
-  Taps t{};
-  for (int k = 0; k < ks; ++k) t.w[k] = w[k];
-  const int64_t grow = 2 * (ks / 2) - ks + 1;
-  const int64_t oz = axis == 0 ? nz + grow : nz, ox = axis == 1 ? nx + grow : nx;
-  launch_gauss(axis, x, y, nullptr, nullptr, nullptr, nz, nx, oz, ox, t, ks, (hipStream_t)stream);
-  SEIS_CHECK_LAUNCH();
-  return 0;
-}
-
-int seis_smooth_gradient(const float* grad, float* out, float* work, int64_t nz, int64_t nx,
-                         const float* wz, int32_t kz, const float* wx, int32_t kx, int32_t counts,
-                         const float* pre_mask, const float* post_mask, const float* illum,
-                         void* stream) {
+int smooth_gradient(const float* grad, float* out, float* work, int64_t nz, int64_t nx,
+                    const float* wz, int32_t kz, const float* wx, int32_t kx, int32_t counts,
+                    const float* pre_mask, const float* post_mask, const float* illum,
+                    hipStream_t s) {
   SEIS_CHECK_ARG(grad && out, "null pointer argument");
   SEIS_CHECK_ARG(out != grad, "out must not alias grad");
   SEIS_CHECK_ARG(nz >= 1 && nx >= 1, "empty field (%lld x %lld)", (long long)nz, (long long)nx);
   SEIS_CHECK_ARG(counts >= 0, "counts must be >= 0, got %d", counts);
-  const hipStream_t s = (hipStream_t)stream;
   Taps one{};
   one.w[0] = 1.0f;
   if (counts == 0) {
@@ -244,6 +233,73 @@ int seis_smooth_gradient(const float* grad, float* out, float* work, int64_t nz,
   }
   SEIS_CHECK_LAUNCH();
   return 0;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_ptr(float** out) {
+  int dev = -1;
+  SEIS_CHECK_HIP(hipGetDevice(&dev));
+  SEIS_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device ordinal %d outside [0, %d)", dev,
+                 kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    SEIS_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SEIS_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the workspace");
+    g_scratch_dev[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int seis_gaussian_filter(const float* x, float* y, int64_t nz, int64_t nx, int32_t axis,
+                         const float* w, int32_t ks, void* stream) {
+  SEIS_CHECK_ARG(x && y && w, "null pointer argument");
+  SEIS_CHECK_ARG(x != y, "y must not alias x");
+  SEIS_CHECK_ARG(axis == 0 || axis == 1, "axis must be 0 (z) or 1 (x), got %d", axis);
+  SEIS_CHECK_ARG(ks >= 1 && ks <= kMaxTaps, "kernel size %d outside [1, %d]", ks, kMaxTaps);
+  SEIS_CHECK_ARG(nz >= 1 && nx >= 1, "empty field (%lld x %lld)", (long long)nz, (long long)nx);
+  const int64_t n = axis == 0 ? nz : nx;
+  SEIS_CHECK_ARG(ks / 2 < n, "half kernel size %d must be smaller than the axis length %lld",
+                 ks / 2, (long long)n);
+  Taps t{};
+  for (int k = 0; k < ks; ++k) t.w[k] = w[k];
+  const int64_t grow = 2 * (ks / 2) - ks + 1;
+  const int64_t oz = axis == 0 ? nz + grow : nz, ox = axis == 1 ? nx + grow : nx;
+  launch_gauss(axis, x, y, nullptr, nullptr, nullptr, nz, nx, oz, ox, t, ks, (hipStream_t)stream);
+  SEIS_CHECK_LAUNCH();
+  return 0;
+}
+
+int seis_smooth_gradient(const float* grad, float* out, float* work, int64_t nz, int64_t nx,
+                         const float* wz, int32_t kz, const float* wx, int32_t kx, int32_t counts,
+                         const float* pre_mask, const float* post_mask, const float* illum,
+                         void* stream) {
+  return smooth_gradient(grad, out, work, nz, nx, wz, kz, wx, kx, counts, pre_mask, post_mask,
+                         illum, (hipStream_t)stream);
+}
+
+int seis_smooth_gradient_noalloc(const float* grad, float* out, int64_t nz, int64_t nx,
+                                 const float* wz, int32_t kz, const float* wx, int32_t kx,
+                                 int32_t counts, const float* pre_mask, const float* post_mask,
+                                 const float* illum, void* stream) {
+  SEIS_CHECK_ARG(nz >= 1 && nx >= 1, "empty field (%lld x %lld)", (long long)nz, (long long)nx);
+  SEIS_CHECK_ARG(nz <= SEIS_NOALLOC_MAX_CELLS / nx,
+                 "seis_smooth_gradient_noalloc: %lld x %lld = %lld cells exceed the library's "
+                 "static workspace of %lld cells; use seis_smooth_gradient with a work buffer",
+                 (long long)nz, (long long)nx, (long long)(nz * nx),
+                 (long long)SEIS_NOALLOC_MAX_CELLS);
+  float* work = nullptr;
+  if (int rc = scratch_ptr(&work)) return rc;
+  return smooth_gradient(grad, out, work, nz, nx, wz, kz, wx, kx, counts, pre_mask, post_mask,
+                         illum, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -119,6 +119,10 @@ def test_gradient_postprocess_argument_checks():
     with pytest.raises(RuntimeError, match="alias"):
         _lib.call("seis_smooth_gradient", f1, f2, f2, 10, 10, w3, 3, w3, 3, 1, None, None, None,
                   None)
+    # the no-alloc variant refuses grids beyond its static workspace before any device call
+    with pytest.raises(RuntimeError, match="static workspace"):
+        _lib.call("seis_smooth_gradient_noalloc", f1, f2, 2049, 2048, w3, 3, w3, 3, 1, None, None,
+                  None, None)
 
 
 def test_variable_density_and_illumination_argument_checks():
