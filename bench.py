@@ -158,6 +158,33 @@ def gradient_step(P):
     return loss.detach(), vp.grad
 
 
+def _launch_shape_json(workload, shape, tune):
+    """The tuned launch shape and the candidates' times for the JSON line."""
+    if workload == "elastic":
+        out = {"shots_per_group_forward": shape[0], "shots_per_group_adjoint": shape[1]}
+        times = {f"{k[0]}_S{k[1]}": round(t, 2) for k, t in tune.items()}
+    elif len(shape) == 3:
+        out = {"shots_per_group": shape[0], "rows_per_thread": shape[1], "vec_x": shape[2]}
+        times = {f"S{a}_rz{b}_vx{c}": round(t, 2) for (a, b, c), t in tune.items()}
+    else:
+        out = {"shots_per_group": shape[0]}
+        times = {f"S{k[0]}": round(t, 2) for k, t in tune.items()}
+    out["autotune_ms"] = times
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(P, steps):
     """The FP32 torch restatement of the reference path on the host cores:
     1 shot on the full padded grid for `steps` time steps (forward with ring
@@ -174,7 +201,7 @@ def cpu_baseline(P, steps):
     el = time.perf_counter() - t0
     cells = P["nz"] * P["nx"] * steps
     return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port",
+            "cpu": cpu_model(), "kind": "port",
             "sample": f"1 shot, {P['nz']}x{P['nx']} padded grid, {steps} of {P['nt']} steps, "
                       f"fwd+adjoint (oracle/ref_torch.gradient_bs, torch {torch.__version__}), "
                       f"{el:.1f} s"}
@@ -192,21 +219,26 @@ def copy_calibration(dev, reps=5):
     n = 1 << 28  # 1 GiB of float32
     a = torch.ones(n, device=dev)
     b = torch.empty_like(a)
-    blocks = 8 * torch.cuda.get_device_properties(dev).multi_processor_count
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
     lib = _lib.load()
-    best = None
-    for _ in range(reps):
-        torch.cuda.synchronize()
-        lib.seis_profile_enable(1)
-        _lib.call("seis_stream_copy", _lib.ptr(a), _lib.ptr(b), n, blocks, _lib.stream_of(a))
-        lib.seis_profile_enable(0)
-        ms = ctypes.c_double(0.0)
-        cnt = ctypes.c_int64(0)
-        _lib.call("seis_profile_collect", 10, ctypes.byref(ms), ctypes.byref(cnt))
-        best = ms.value if best is None else min(best, ms.value)
+    rates = {}
+    for per_cu in (2, 4, 8, 16, 32):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            lib.seis_profile_enable(1)
+            _lib.call("seis_stream_copy", _lib.ptr(a), _lib.ptr(b), n, per_cu * cus,
+                      _lib.stream_of(a))
+            lib.seis_profile_enable(0)
+            ms = ctypes.c_double(0.0)
+            cnt = ctypes.c_int64(0)
+            _lib.call("seis_profile_collect", 10, ctypes.byref(ms), ctypes.byref(cnt))
+            best = ms.value if best is None else min(best, ms.value)
+        rates[per_cu] = round(2 * 4 * n / (best / 1e3) / 1e12, 3)
     torch.cuda.synchronize()
     del a, b
-    return {"bytes_each_way": 4 * n, "tb_per_s": round(2 * 4 * n / (best / 1e3) / 1e12, 3)}
+    return {"bytes_each_way": 4 * n, "tb_per_s": max(rates.values()),
+            "tb_per_s_by_blocks_per_cu": rates}
 
 
 def run_generator(args, dev):
@@ -373,7 +405,7 @@ def run_filter(args, dev):
         osg.filtfilt(b, a, xs)
         cel = time.perf_counter() - t1
         out["cpu_baseline"] = {"value": round(xs.size / cel / 1e9, 5), "unit": "Gsamples/s",
-                               "cores": 1, "kind": "port",
+                               "cores": 1, "cpu": cpu_model(), "kind": "port",
                                "sample": f"all {shots * nrec} traces x {nt} samples, "
                                          f"oracle/signal.filtfilt (numpy FP64, vectorised over "
                                          f"traces, one thread), {cel:.1f} s"}
@@ -438,7 +470,8 @@ def run_postprocess(args, dev):
         opp.postprocess(gs, sm, pre_mask=ms, post_mask=ms, illum=il)
         cel = time.perf_counter() - t1
         out["cpu_baseline"] = {"value": round(cells * passes / cel / 1e9, 5),
-                               "unit": "Gcell-passes/s", "cores": 1, "kind": "port",
+                               "unit": "Gcell-passes/s", "cores": 1, "cpu": cpu_model(),
+                               "kind": "port",
                                "sample": f"the whole call on the same gradient, "
                                          f"oracle/process.postprocess (numpy FP64, one "
                                          f"thread), {cel:.2f} s"}
@@ -512,7 +545,7 @@ def run_rho(args, dev):
         cel = time.perf_counter() - t1
         out["cpu_baseline"] = {"value": round(nz * nx * steps / cel / 1e9, 5),
                                "unit": "Gcell-updates/s", "cores": torch.get_num_threads(),
-                               "kind": "port",
+                               "cpu": cpu_model(), "kind": "port",
                                "sample": f"1 shot, {nz}x{nx} padded grid, {steps} of {nt} steps, "
                                          f"oracle/ref_torch.rho_habc_forward (FP32 torch on the "
                                          f"host), {cel:.1f} s"}
@@ -626,10 +659,18 @@ def main():
     print(f"[bench] problem ready: {P['B']} shots, {P['nz']}x{P['nx']}, nt={P['nt']}",
           file=sys.stderr, flush=True)
     shape, tune = None, {}
+    from seistorch_amd import autotune
     if args.workload == "acoustic":
-        from seistorch_amd import autotune
         shape, tune = autotune.tune_acoustic(P["vp_init"], P["d"], P["w"], P["src"], P["rec"],
                                              P["B"], P["dt"], P["h"])
+    elif args.workload == "elastic":
+        shape, tune = autotune.tune_elastic(P["vp_init"], P["vs_init"], P["rho_init"], P["d"],
+                                            P["w"], P["src"], P["rec"], P["B"], P["dt"], P["h"],
+                                            src_type=("txx", "tzz"), rec_type=("vx", "vz"))
+    elif args.workload == "habc":
+        shape, tune = autotune.tune_habc(P["vp_init"], P["b"], P["w"], P["src"], P["rec"], P["B"],
+                                         P["dt"], P["h"])
+    if shape is not None:
         print(f"[bench] autotuned launch shape {shape}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         step_fn(P)
@@ -683,9 +724,7 @@ def main():
                      "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch,
                      "bytes_per_cell_update": BYTES_PER_CELL[dom]},
         "kernels_ms_total": {k: round(v[0], 2) for k, v in kern.items()},
-        "launch_shape": None if shape is None else {
-            "shots_per_group": shape[0], "rows_per_thread": shape[1], "vec_x": shape[2],
-            "autotune_ms": {f"S{a}_rz{b}_vx{c}": round(v, 2) for (a, b, c), v in tune.items()}},
+        "launch_shape": None if shape is None else _launch_shape_json(args.workload, shape, tune),
         "loss": float(loss),
     }
     if args.copy_calibration:

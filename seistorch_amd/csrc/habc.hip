@@ -148,6 +148,16 @@ __device__ __forceinline__ bool in_deep(int z, int x, const Deep& dp) {
 // per kFrameGroups groups of S shots.
 constexpr int kFrameGroups = 8;
 
+// Shot groups per thread of the adjoint's top and bottom bands.
+constexpr int kBandGroups = 8;
+
+// Cells per lane on the adjoint's bands, at most the Deep lane width VX (so
+// Deep's column bounds stay aligned).  The bands share the kernel with Deep,
+// whose occupancy follows the larger branch: one cell per lane keeps the
+// kernel at 98 VGPRs for VX = 4 (5 waves per SIMD); two took 120 and four 161.
+constexpr int kBandVX = 1;
+__host__ __device__ constexpr int band_vx(int vx) { return vx < kBandVX ? vx : kBandVX; }
+
 // Rows per thread of the vectorised adjoint on Deep.  One: the deep and
 // frame block ranges share one kernel, whose register count (and so the
 // frame threads' occupancy) follows the larger branch; two rows per thread
@@ -515,6 +525,179 @@ __device__ __forceinline__ void habc_adj_cell(
   }
 }
 
+// The adjoint on the top and bottom bands: the rows outside Deep over Deep's
+// columns (x0 >= kW+3 and x0+VB <= nx-kW-3).  There only the top (bottom)
+// strip's one-way stencil reaches a cell: from the cells 0, 1 and 2 rows
+// outward in its own column and, on the edge row, from strip row kW-1 (the
+// roll wrap), and which of those four terms exist depends on the row alone.
+// So a lane takes VB consecutive cells of one row, every access is a row
+// vector, and each cell sums the same operands in the same order as
+// habc_adj_cell (bit-identical).  TOP: the top band, else the bottom band.
+template <int S, int VB, bool TOP>
+__device__ __forceinline__ void habc_adj_band(
+    int z, int x0, int grp, int ng, const float* __restrict__ vp, const float* __restrict__ bw,
+    const float* __restrict__ y1, const float* __restrict__ y2, float* __restrict__ out,
+    const float* __restrict__ um, float* __restrict__ uc, float* __restrict__ gpart,
+    const int32_t* __restrict__ src, const float* __restrict__ wn, int recon, int B,
+    const HabcCtx& hc, const seis::TimeConsts& k) {
+  const Grid& g = hc.g;
+  constexpr int st = TOP ? 0 : 1;  // the strip
+  constexpr int er = TOP ? 1 : -1;  // its inward step in rows
+  const int dj = TOP ? z : g.nz - 1 - z;
+  const int i0 = z * g.nx + x0;
+  const float A0 = 1.0f / k.idt2;
+  const float B0 = k.idt2;
+  // one-way term t reads row trow[t]: t = 0, 1, 2 the cells t rows outward
+  // (their stencils reach this row through u, u[+e], u[+2e]), t = 3 the wrap
+  bool tv[4];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int dn = dj - m;
+    tv[m] = !(dn < 0 || dn > kW - 1 || (m == 2 && dn == kW - 1));
+  }
+  tv[3] = dj == 0;
+  const int trow[4] = {z, z - er, z - 2 * er, z + (kW - 1) * er};
+  const bool hz = z > 0, hzp = z < g.nz - 1;
+  auto qw = [&](int zz, int xx) {
+    const int j = zz * g.nx + xx;
+    const float cn = vp[j];
+    return (cn * cn) * A0 * habc_weights(zz, xx, hc, bw[j]).alpha;
+  };
+  float q0[VB], qzm[VB], qzp[VB], qxm[VB], qxp[VB], al0[VB], c[VB];
+  float oc1[4][VB], oc2[4][VB];
+  bool on[4][VB];
+  bool in[VB];
+  bool allin = true, anyin = false;
+#pragma unroll
+  for (int e = 0; e < VB; ++e) {
+    const int x = x0 + e;
+    c[e] = vp[i0 + e];
+    q0[e] = qw(z, x);
+    qzm[e] = hz ? qw(z - 1, x) : 0.0f;
+    qzp[e] = hzp ? qw(z + 1, x) : 0.0f;
+    qxm[e] = qw(z, x - 1);
+    qxp[e] = qw(z, x + 1);
+    al0[e] = habc_weights(z, x, hc, bw[i0 + e]).alpha;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      on[t][e] = false;
+      oc1[t][e] = 0.0f;
+      oc2[t][e] = 0.0f;
+      if (!tv[t]) continue;
+      const int n = trow[t] * g.nx + x;
+      const float be = habc_weights(trow[t], x, hc, bw[n]).beta[st];
+      if (be == 0.0f) continue;
+      const OwCoef kc = ow_coef(vp[n], hc);
+      on[t][e] = true;
+      oc1[t][e] = be * (t == 0 ? kc.k0 : t == 1 ? kc.k1 : kc.k2);
+      oc2[t][e] = t == 0 ? be * kc.k3 : t == 1 ? be * kc.k4 : 0.0f;
+    }
+    in[e] = interior(z, x, g);
+    allin = allin && in[e];
+    anyin = anyin || in[e];
+  }
+  const float w = wn ? *wn : 0.0f;
+  for (int gi = grp; gi < grp + ng; ++gi) {
+    float gacc[VB];
+    vzero<VB>(gacc);
+    const int b0 = gi * S, nb = min(S, B - b0);
+    for (int s = 0; s < nb; ++s) {
+      const int bb = b0 + s;
+      const int64_t off = (int64_t)bb * g.plane;
+      const float* Y = y1 + off;
+      const float* Y2 = y2 + off;
+      float yc[VB], yzm[VB], yzp[VB], acc[VB];
+      vload<VB>(Y + i0, yc);
+      if (hz) vload<VB>(Y + i0 - g.nx, yzm); else vzero<VB>(yzm);
+      if (hzp) vload<VB>(Y + i0 + g.nx, yzp); else vzero<VB>(yzp);
+      const float yl = Y[i0 - 1], yr = Y[i0 + VB];
+#pragma unroll
+      for (int e = 0; e < VB; ++e) {
+        const float Lq = lap_taps(hz ? qzm[e] * yzm[e] : 0.0f,
+                                  qxm[e] * (e == 0 ? yl : yc[e - 1]), q0[e] * yc[e],
+                                  qxp[e] * (e == VB - 1 ? yr : yc[e + 1]),
+                                  hzp ? qzp[e] * yzp[e] : 0.0f, k.ih2);
+        acc[e] = (k.two_idt2 * (A0 * al0[e])) * yc[e] + Lq;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (!tv[t]) continue;
+        float yt[VB];
+        if (t == 0) {
+#pragma unroll
+          for (int e = 0; e < VB; ++e) yt[e] = yc[e];
+        } else if (t == 1) {
+#pragma unroll
+          for (int e = 0; e < VB; ++e) yt[e] = TOP ? yzm[e] : yzp[e];
+        } else {
+          vload<VB>(Y + trow[t] * g.nx + x0, yt);
+        }
+#pragma unroll
+        for (int e = 0; e < VB; ++e)
+          if (on[t][e]) acc[e] += oc1[t][e] * yt[e];
+      }
+      float y2c[VB];
+      vload<VB>(Y2 + i0, y2c);
+#pragma unroll
+      for (int e = 0; e < VB; ++e) acc[e] += -(B0 * (A0 * al0[e])) * y2c[e];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (!tv[t]) continue;
+        float yt[VB];
+        if (t == 0) {
+#pragma unroll
+          for (int e = 0; e < VB; ++e) yt[e] = y2c[e];
+        } else {
+          vload<VB>(Y2 + trow[t] * g.nx + x0, yt);
+        }
+#pragma unroll
+        for (int e = 0; e < VB; ++e)
+          if (on[t][e]) acc[e] += oc2[t][e] * yt[e];
+      }
+      vstore<VB>(out + off + i0, acc);
+      if (anyin && recon >= 0) {
+        const float* U = um + off;
+        float u0[VB], uzm[VB], uzp[VB], un[VB];
+        vload<VB>(U + i0, u0);
+        if (hz) vload<VB>(U + i0 - g.nx, uzm); else vzero<VB>(uzm);
+        if (hzp) vload<VB>(U + i0 + g.nx, uzp); else vzero<VB>(uzp);
+        const float ul = U[i0 - 1], ur = U[i0 + VB];
+        if (recon == 1) vload<VB>(uc + off + i0, un);
+        const int sz = src[2 * bb], sx = src[2 * bb + 1];
+#pragma unroll
+        for (int e = 0; e < VB; ++e) {
+          if (!in[e]) continue;
+          const float Lu = lap_taps(hz ? uzm[e] : 0.0f, e == 0 ? ul : u0[e - 1], u0[e],
+                                    e == VB - 1 ? ur : u0[e + 1], hzp ? uzp[e] : 0.0f, k.ih2);
+          gacc[e] += Lu * acc[e];
+          if (recon == 1) {
+            float v = un[e];
+            if (z == sz && x0 + e == sx) v -= w;
+            un[e] = ((k.two_idt2 * u0[e] + (c[e] * c[e]) * Lu) - v / A0) / B0;
+          } else if (recon == 2) {
+            un[e] = 0.0f;
+          }
+        }
+        if (recon != 0) {
+          if (allin) {
+            vstore<VB>(uc + off + i0, un);
+          } else {
+#pragma unroll
+            for (int e = 0; e < VB; ++e)
+              if (in[e]) uc[off + i0 + e] = un[e];
+          }
+        }
+      }
+    }
+    if (anyin) {
+      float* gp = gpart + (int64_t)gi * g.plane + i0;
+#pragma unroll
+      for (int e = 0; e < VB; ++e)
+        if (in[e]) gp[e] += 2.0f * c[e] * A0 * gacc[e];
+    }
+  }
+}
+
 template <int S>
 __global__ __launch_bounds__(256) void k_habc_adj(
     const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
@@ -699,11 +882,15 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
     const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
     float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
     const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
-    TileMap tm, Frame f, int ngf, int G) {
-  // blocks [0, f.per*8) run the frame's cells, one cell per thread for all G
-  // shot groups (the per-cell set-up once per step, not once per group), the
-  // rest Deep's lane vectors; the frame goes first, being the longer work
+    TileMap tm, Frame f, int ngf, int G, TileMap tb, int ngb) {
+  // blocks [0, f.per*8) run the frame's cells (the left and right columns
+  // outside Deep), one cell per thread for ngf shot groups (the per-cell
+  // set-up once per ngf groups); the next tb.per*8 the top and bottom bands
+  // in lane vectors of VX cells for ngb groups; the rest Deep's lane vectors.
+  // The frame goes first, being the longer work per cell.
   const int nf = f.per * kXcds;
+  const int nbnd = tb.per * kXcds;
+  const Grid& g = hc.g;
   if ((int)blockIdx.x < nf) {
     int zc, xc, gc;
     if (frame_cell(f, (int)blockIdx.x, zc, xc, gc))
@@ -711,9 +898,25 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
                        hc, k, min(ngf, G - gc * ngf));
     return;
   }
+  if ((int)blockIdx.x < nf + nbnd) {
+    int tx, tzi, gb;
+    if (!tile_of_w(tb, (int)blockIdx.x - nf, tx, tzi, gb)) return;
+    constexpr int VB = band_vx(VX);
+    const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VB;
+    const int r = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+    const int ztop = dp.z0;  // rows [0, dp.z0) then [dp.z1, nz)
+    if (x0 >= dp.x1 || r >= ztop + (g.nz - dp.z1)) return;
+    const int grp = gb * ngb, ng = min(ngb, G - grp);
+    if (r < ztop)
+      habc_adj_band<S, VB, true>(r, x0, grp, ng, vp, bw, y1, y2, out, um, uc, gpart, src, wn,
+                                 recon, B, hc, k);
+    else
+      habc_adj_band<S, VB, false>(dp.z1 + (r - ztop), x0, grp, ng, vp, bw, y1, y2, out, um, uc,
+                                  gpart, src, wn, recon, B, hc, k);
+    return;
+  }
   int tx, tzi, grp;
-  if (!tile_of_w(tm, (int)blockIdx.x - nf, tx, tzi, grp)) return;
-  const Grid& g = hc.g;
+  if (!tile_of_w(tm, (int)blockIdx.x - nf - nbnd, tx, tzi, grp)) return;
   const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
   const int zb = dp.z0 + (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * kHabcRZ;
   if (x0 >= dp.x1 || zb >= dp.z1) return;
@@ -966,8 +1169,10 @@ int habc_vx(int64_t nx, std::initializer_list<const void*> ptrs) {
   return 1;
 }
 
-// The frame around dp (the whole grid if dp is empty) as rectangles of tiles.
-Frame make_frame(const Grid& g, const Deep& dp, int groups) {
+// The frame around dp (the whole grid if dp is empty) as rectangles of tiles;
+// with sides_only, just the full-height columns left and right of dp (the
+// top and bottom bands over dp's columns go to band_tiles).
+Frame make_frame(const Grid& g, const Deep& dp, int groups, bool sides_only = false) {
   Frame f{};
   auto add = [&](int z0, int z1, int x0, int x1) {
     if (z1 <= z0 || x1 <= x0) return;
@@ -978,6 +1183,9 @@ Frame make_frame(const Grid& g, const Deep& dp, int groups) {
   };
   if (dp.z1 <= dp.z0 || dp.x1 <= dp.x0) {
     add(0, g.nz, 0, g.nx);
+  } else if (sides_only) {
+    add(0, g.nz, 0, dp.x0);
+    add(0, g.nz, dp.x1, g.nx);
   } else {
     add(0, dp.z0, 0, g.nx);
     add(dp.z1, g.nz, 0, g.nx);
@@ -991,6 +1199,16 @@ Frame make_frame(const Grid& g, const Deep& dp, int groups) {
   for (int r = f.n + 1; r < 5; ++r) f.t0[r] = f.t0[f.n];
   f.per = (int)((f.t0[f.n] + kXcds - 1) / kXcds);
   return f;
+}
+
+// Tiles of the top and bottom bands of the adjoint (rows [0, dp.z0) and
+// [dp.z1, nz) over dp's columns, stacked as one virtual grid of lane vectors
+// of vx cells); no tiles if dp is empty.
+TileMap band_tiles(const Grid& g, const Deep& dp, int vx, int groups) {
+  if (dp.z1 <= dp.z0 || dp.x1 <= dp.x0) return TileMap{0, 0, 0, 0};
+  const int rows = dp.z0 + (g.nz - dp.z1);
+  const Grid gb{rows, dp.x1 - dp.x0, 0, (int64_t)rows * (dp.x1 - dp.x0)};
+  return make_tiles(gb, vx, 1, groups);
 }
 
 int habc_check(int64_t B, int64_t nz, int64_t nx, float dt, float h, int S) {
@@ -1296,7 +1514,9 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
   const TileMap tmd = make_tiles(gd, VX, kHabcRZ, G);
   const int ngf = G < kFrameGroups ? G : kFrameGroups;  // groups per frame thread
-  const Frame fr = make_frame(g, dp, (G + ngf - 1) / ngf);
+  const Frame fr = make_frame(g, dp, (G + ngf - 1) / ngf, true);
+  const int ngb = G < kBandGroups ? G : kBandGroups;  // groups per band thread
+  const TileMap tmb = band_tiles(g, dp, band_vx(VX), (G + ngb - 1) / ngb);
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -1314,9 +1534,9 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   for (int64_t kk = nt; kk >= 1; --kk) {
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
-    SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX, dim3((tmd.per + fr.per) * kXcds), s, vp, b, L1,
-                       L2, L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k,
-                       dp, tmd, fr, ngf, G);
+    SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX, dim3((tmd.per + fr.per + tmb.per) * kXcds), s,
+                       vp, b, L1, L2, L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon,
+                       (int)B, hc, k, dp, tmd, fr, ngf, G, tmb, ngb);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
     if (work > 0) {

@@ -39,12 +39,20 @@ int prof_mark(int kernel, hipStream_t s) {
 }
 
 // Streaming copy for the HBM calibration: float4 loads and stores, a
-// grid-stride loop over all CUs with several blocks each.
+// grid-stride loop over all CUs with several blocks each; every thread keeps
+// four loads in flight before its stores.
 __global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ a,
                                                      float4* __restrict__ b, int64_t n4) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
-    b[i] = a[i];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+    b[i] = v0;
+    b[i + stride] = v1;
+    b[i + 2 * stride] = v2;
+    b[i + 3 * stride] = v3;
+  }
+  for (; i < n4; i += stride) b[i] = a[i];
 }
 
 }  // namespace seis

@@ -76,6 +76,10 @@ class HabcLoop(torch.autograd.Function):
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
+        if shots_per_group is None:
+            from .. import autotune
+            tuned = autotune.lookup("habc", B, nz, nx)
+            shots_per_group = tuned[0] if tuned else None
         S = int(shots_per_group or default_shots_per_group(B))
         u = [torch.empty((B, nz, nx), device=vp.device) for _ in range(3)]
         rec = torch.empty((nt, R), device=vp.device)

@@ -132,7 +132,13 @@ class ElasticLoop(torch.autograd.Function):
         nz, nx = vp.shape
         nt = wavelet.shape[0]
         R = rec_flat.shape[0]
-        S = int(shots_per_group or _shots(B))
+        if shots_per_group is None:
+            from .. import autotune
+            shots_per_group = autotune.lookup("elastic", B, vp.shape[0], vp.shape[1])
+        if isinstance(shots_per_group, (tuple, list)):
+            S, S_bwd = (int(v) for v in shots_per_group)
+        else:
+            S = S_bwd = int(shots_per_group or _shots(B))
         cf = coefficients(vp, vs, rho, d, dt, h)
         fields = [torch.empty((B, nz, nx), device=vp.device) for _ in range(5)]
         rf = torch.tensor([FIELDS.index(n) for n in rec_type], dtype=torch.int32,
@@ -162,7 +168,7 @@ class ElasticLoop(torch.autograd.Function):
         if need_grad:
             ctx.save_for_backward(vp, vs, rho, cf, wavelet, src_zx, rec_flat, rf)
             ctx.state = (ring, snaps, fields)
-            ctx.meta = (B, dt, h, S, K, _mask(src_type), nch, bool(fused), int(multiple))
+            ctx.meta = (B, dt, h, S_bwd, K, _mask(src_type), nch, bool(fused), int(multiple))
         return rec
 
     @staticmethod
@@ -198,6 +204,8 @@ def propagate(vp, vs, rho, d, wavelet, src_zx, rec_flat, B, dt, h, src_type=("tx
               rec_type=("vx", "vz"), shots_per_group=None, snapshot_every=None, fused=True,
               illum=None, multiple=False):
     """Records (nt, R, nch) of B shots; differentiable w.r.t. vp, vs, rho.
+    shots_per_group: shots per thread (1, 2, 4, 8), or a pair (forward,
+    adjoint); None: the tuned value for this shape, else a default.
     fused: each forward step and each reverse step of the adjoint in one pass
     (five more field buffers).  illum: None, or an (nz, nx) float32 tensor
     filled with the source illumination of the last source type's field

@@ -146,7 +146,28 @@ def plain_step(p, vp, loss, g, step):
     return vp, loss, step, False
 
 
-def run_plain(p, iters, step0=40.0, time_budget=None, log=None):
+def descent_direction(p, vp, smooth=None, source_illumination=False):
+    """(misfit, raw interior gradient, update direction over the interior).
+    Without smooth and source_illumination the direction is the raw gradient
+    (top `mute` rows zeroed); with them it is the reference's post-processing
+    of the padded gradient before its optimizer step (seistorch_dist.py:270-280
+    through postprocess_padded: `counts` Gaussian passes, then the division by
+    the forward's source illumination, rnn.py:127-128,204-205), muted again."""
+    if smooth is None and not source_illumination:
+        loss, g = p.gradient(vp)
+        return loss, g, g
+    illum = torch.empty_like(p.vp_true) if source_illumination else None
+    loss, gp = p.gradient(vp, illum=illum, padded=True)
+    raw = p.interior(gp).clone()
+    d = postprocess_padded(gp, smooth, None, illum)
+    if p.mute:
+        d[:p.mute] = 0.0
+    return loss, raw, d
+
+
+def run_plain(p, iters, step0=40.0, time_budget=None, log=None, smooth=None,
+              source_illumination=False):
+    """Steepest descent along descent_direction (plain_step's line search)."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     vp = p.vp_init.clone()
@@ -154,7 +175,7 @@ def run_plain(p, iters, step0=40.0, time_budget=None, log=None):
     hist.add(0.0, p.misfit(vp), p.rmse(vp), "start")
     step = step0
     for it in range(iters):
-        loss, g = p.gradient(vp)
+        loss, _, g = descent_direction(p, vp, smooth, source_illumination)
         vp, m, step, ok = plain_step(p, vp, loss, g, step)
         if ok:
             step *= 1.25
@@ -168,10 +189,13 @@ def run_plain(p, iters, step0=40.0, time_budget=None, log=None):
 
 
 def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, net_iters=1, time_budget=None,
-                       log=None):
-    """FWI whose first `net_iters` iterations add the network's predicted
-    perturbation to the plain step (one trial, accepted if the misfit drops,
-    else the plain step alone), then plain iterations.
+                       log=None, smooth=None, source_illumination=False, scales=(1.0, 0.5)):
+    """FWI whose first `net_iters` iterations try the network's predicted
+    perturbation (from the raw gradient and the model, the inputs it was
+    trained on) plus the plain step, at each of `scales` of the prediction in
+    turn, accepting the first trial that lowers the misfit; otherwise (the
+    fallback) the plain step of the same iteration along descent_direction.
+    Later iterations are plain.
 
     The network was trained on gradients at smoothed start models, so it is
     consulted where its inputs look like its training data; afterwards the
@@ -184,20 +208,22 @@ def run_preconditioned(p, net, iters, vscale=500.0, step0=40.0, net_iters=1, tim
     hist.add(0.0, p.misfit(vp), p.rmse(vp), "start")
     step = step0
     for it in range(iters):
-        loss, g = p.gradient(vp)
+        loss, raw, g = descent_direction(p, vp, smooth, source_illumination)
         kind, m = "none", loss
         if it < net_iters:
             vi = p.interior(vp).contiguous()
-            sc = precond.input_scales(g[None], vi[None])
-            d = net.forward(g[None].contiguous(), vi[None], sc)[0] * vscale
+            sc = precond.input_scales(raw[None], vi[None])
+            d = net.forward(raw[None].contiguous(), vi[None], sc)[0] * vscale
             d[:p.mute] = 0.0
             gmax = float(g.abs().max())
             plain_dv = -step * g / gmax if gmax > 0.0 else torch.zeros_like(g)
-            trial = p.pad(_clip(p, vi + d + plain_dv))
-            mt = p.misfit(trial)
-            if mt < loss:
-                vp, m, kind = trial, mt, "net+plain"
-                step *= 1.25
+            for a in scales:
+                trial = p.pad(_clip(p, vi + a * d + plain_dv))
+                mt = p.misfit(trial)
+                if mt < loss:
+                    vp, m, kind = trial, mt, f"net*{a:g}+plain"
+                    step *= 1.25
+                    break
         if kind == "none":  # the plain step of this iteration (the fallback)
             vp, m, step, ok = plain_step(p, vp, loss, g, step)
             kind = "plain" if ok else "none"

@@ -76,6 +76,41 @@ def test_batch_matches_single_instances(S, rz, vx, K):
         assert float(gc.abs().max()) > 0 or SPECS[q][3] <= 2
 
 
+def test_batch_against_the_oracle():
+    """Every instance of one batched launch against the FP32 restatement of
+    the reference loop (oracle/ref_torch: records bit for bit, the gradient of
+    sum(w * records) by autograd through the loop) and the FP64 oracle
+    (oracle/acoustic records, ref_torch autograd in FP64), on the rebuilt
+    interior: no worse than twice the restatement's own FP64 error."""
+    from oracle import acoustic as oa
+    from oracle import ref_torch
+    insts, weights = _instances(seed=5)
+    rb, gb = _batch(insts, weights, shots_per_group=2, rows_per_thread=2, vec_x=4,
+                    snapshot_every=None, use_graph=False)
+    I = (slice(50, -50), slice(50, -50))
+    for q, (nz_i, nx_i, B, nt, dt, h, fm) in enumerate(SPECS):
+        c = acoustic_case(nz_i=nz_i, nx_i=nx_i, nshots=B, nt=nt, dt=dt, h=h, fm=fm, seed=q + 1,
+                          delay=min(60, nt))
+        rr = torch.cat(ref_torch.forward(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec), 1)[..., 0]
+        ro = np.concatenate(oa.forward(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec), 1)[..., 0]
+        r = rb[q].cpu().numpy()
+        e_ref, e_or, e_ro = rel_l2(r, rr.numpy()), rel_l2(r, ro), rel_l2(rr.numpy(), ro)
+        w = weights[q].cpu().numpy()
+        _, g32, _ = ref_torch.gradient_ad(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec, None,
+                                          rec_cotangent=w)
+        _, g64, _ = ref_torch.gradient_ad(c.vp, c.d, c.dt, c.h, c.w, c.src, c.rec, None,
+                                          dtype=torch.float64, rec_cotangent=w)
+        g = gb[q].cpu().numpy()[I]
+        g32, g64 = g32.numpy()[I], g64.numpy()[I]
+        eg, eg64, eg_r = rel_l2(g, g32), rel_l2(g, g64), rel_l2(g32, g64)
+        print(f"instance {q} ({c.nz}x{c.nx}, {B} shots, nt={nt}): records vs restatement "
+              f"{e_ref:.2e}, vs FP64 {e_or:.2e} (restatement {e_ro:.2e}); gradient vs FP32 AD "
+              f"{eg:.2e}, vs FP64 {eg64:.2e} (FP32 AD {eg_r:.2e})", flush=True)
+        assert e_ref == 0.0 and e_or <= 2 * e_ro + 1e-6
+        if nt > 2:  # nt <= 2: no adjoint step reaches the interior gradient
+            assert eg < 1e-5 and eg64 <= 2 * eg_r + 1e-6
+
+
 def test_graph_replays_match_direct_launches():
     insts, weights = _instances(seed=3)
     shape = dict(shots_per_group=4, rows_per_thread=2, vec_x=4, snapshot_every=50)

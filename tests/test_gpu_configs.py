@@ -131,6 +131,8 @@ def test_forward_modelling_gradient_full_size():
     log(f"forward modelling gradient of sum(G*records): 8 shots HIP vs FP64 {e_64:.2e}; "
         f"shot 3 alone: HIP vs FP32 autograd {e1_ref:.2e}, HIP vs FP64 {e1_64:.2e}, FP32 "
         f"autograd vs FP64 {e1_r64:.2e}")
+    # both FP32 paths carry ~5e-5 against FP64 after 1500 steps; they differ
+    # by 1.2e-5 (measured)
     assert e1_ref < 2 * TOL
     assert e1_64 <= 2 * e1_r64 + 1e-6
     assert e_64 <= 2 * e1_r64 + 1e-6
@@ -159,16 +161,21 @@ def test_fwi_gradient_finite_differences_full_size(fwi):
     """Central differences of J = sum(G * records) at 5 seeded random interior
     cells, step 50 m/s (the records depend on vp^2 at the cell, so the
     central difference of a single-cell change is exact to second order);
-    records reduced in FP64."""
+    records reduced in FP64.  The cells are drawn among those whose
+    gradient is at least 1e-3 of the peak: deeper than the two-way
+    traveltime of the 4 s records the sensitivity is the stencil's
+    super-exponentially small precursor, and FD there compares two FP32
+    noise floors."""
     from seistorch_amd.equations2d import acoustic as ac
     P, t, G, _, g = fwi
     g = g.cpu().numpy()
     rng = np.random.default_rng(20230503)
     gmax = np.abs(g[INTERIOR]).max()
+    zz, xx = np.nonzero(np.abs(g[INTERIOR]) >= 1e-3 * gmax)
+    pick = rng.choice(zz.size, 5, replace=False)
     worst = 0.0
-    for _ in range(5):
-        i = int(rng.integers(50, P["nz"] - 50))
-        j = int(rng.integers(50, P["nx"] - 50))
+    for q in pick:
+        i, j = int(zz[q]) + 50, int(xx[q]) + 50
         vals = []
         for sgn in (1.0, -1.0):
             vpp = t["vp_init"].clone()

