@@ -698,6 +698,111 @@ __device__ __forceinline__ void habc_adj_band(
   }
 }
 
+// The adjoint on the left and right columns outside Deep, over Deep's rows
+// (z in [kW+3, nz-kW-3)): only the left (right) strip's one-way stencil
+// reaches a cell there, from the cells 0, 1 and 2 columns outward in its own
+// row and, on the edge column, from strip column kW-1.  One cell per lane,
+// lanes along x, so every term is a row access shifted by up to two cells;
+// which terms exist depends on the column alone.  Same operands in the same
+// order as habc_adj_cell (bit-identical).  LEFT: the left column, else the
+// right one.
+template <int S, bool LEFT>
+__device__ __forceinline__ void habc_adj_side(
+    int z, int x, int grp, int ng, const float* __restrict__ vp, const float* __restrict__ bw,
+    const float* __restrict__ y1, const float* __restrict__ y2, float* __restrict__ out,
+    const float* __restrict__ um, float* __restrict__ uc, float* __restrict__ gpart,
+    const int32_t* __restrict__ src, const float* __restrict__ wn, int recon, int B,
+    const HabcCtx& hc, const seis::TimeConsts& k) {
+  const Grid& g = hc.g;
+  constexpr int st = LEFT ? 2 : 3;  // the strip
+  constexpr int ec = LEFT ? 1 : -1;  // its inward step in columns
+  const int dj = LEFT ? x : g.nx - 1 - x;
+  const int i = z * g.nx + x;
+  const float A0 = 1.0f / k.idt2;
+  const float B0 = k.idt2;
+  bool tv[4];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int dn = dj - m;
+    tv[m] = !(dn < 0 || dn > kW - 1 || (m == 2 && dn == kW - 1));
+  }
+  tv[3] = dj == 0;
+  const int toff[4] = {0, -ec, -2 * ec, (kW - 1) * ec};
+  const bool hxm = x > 0, hxp = x < g.nx - 1;
+  auto qw = [&](int zz, int xx) {
+    const int j = zz * g.nx + xx;
+    const float cn = vp[j];
+    return (cn * cn) * A0 * habc_weights(zz, xx, hc, bw[j]).alpha;
+  };
+  const float c = vp[i];
+  const float q0 = qw(z, x);
+  const float qzm = qw(z - 1, x);
+  const float qzp = qw(z + 1, x);
+  const float qxm = hxm ? qw(z, x - 1) : 0.0f;
+  const float qxp = hxp ? qw(z, x + 1) : 0.0f;
+  const float al0 = habc_weights(z, x, hc, bw[i]).alpha;
+  float oc1[4], oc2[4];
+  bool on[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    on[t] = false;
+    oc1[t] = 0.0f;
+    oc2[t] = 0.0f;
+    if (!tv[t]) continue;
+    const int n = i + toff[t];
+    const float be = habc_weights(z, x + toff[t], hc, bw[n]).beta[st];
+    if (be == 0.0f) continue;
+    const OwCoef kc = ow_coef(vp[n], hc);
+    on[t] = true;
+    oc1[t] = be * (t == 0 ? kc.k0 : t == 1 ? kc.k1 : kc.k2);
+    oc2[t] = t == 0 ? be * kc.k3 : t == 1 ? be * kc.k4 : 0.0f;
+  }
+  const bool in = interior(z, x, g);
+  const float w = wn ? *wn : 0.0f;
+  for (int gi = grp; gi < grp + ng; ++gi) {
+    float gacc = 0.0f;
+    const int b0 = gi * S, nb = min(S, B - b0);
+    for (int s = 0; s < nb; ++s) {
+      const int bb = b0 + s;
+      const int64_t off = (int64_t)bb * g.plane;
+      const float* Y = y1 + off;
+      const float* Y2 = y2 + off;
+      const float yc = Y[i];
+      const float yl = hxm ? Y[i - 1] : 0.0f;
+      const float yr = hxp ? Y[i + 1] : 0.0f;
+      const float Lq = lap_taps(qzm * Y[i - g.nx], hxm ? qxm * yl : 0.0f, q0 * yc,
+                                hxp ? qxp * yr : 0.0f, qzp * Y[i + g.nx], k.ih2);
+      float acc = (k.two_idt2 * (A0 * al0)) * yc + Lq;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (!on[t]) continue;
+        const float yt = t == 0 ? yc : t == 1 ? (LEFT ? yl : yr) : Y[i + toff[t]];
+        acc += oc1[t] * yt;
+      }
+      const float y2c = Y2[i];
+      acc += -(B0 * (A0 * al0)) * y2c;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (!on[t]) continue;
+        acc += oc2[t] * (t == 0 ? y2c : Y2[i + toff[t]]);
+      }
+      out[off + i] = acc;
+      if (in && recon >= 0) {
+        const float Lu = lap5(um + off, z, x, i, g, k.ih2);
+        gacc += Lu * acc;
+        if (recon == 1) {
+          float un = uc[off + i];
+          if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
+          uc[off + i] = ((k.two_idt2 * um[off + i] + (c * c) * Lu) - un / A0) / B0;
+        } else if (recon == 2) {
+          uc[off + i] = 0.0f;
+        }
+      }
+    }
+    if (in) gpart[(int64_t)gi * g.plane + i] += 2.0f * c * A0 * gacc;
+  }
+}
+
 template <int S>
 __global__ __launch_bounds__(256) void k_habc_adj(
     const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
@@ -882,14 +987,16 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
     const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
     float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
     const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
-    TileMap tm, Frame f, int ngf, int G, TileMap tb, int ngb) {
-  // blocks [0, f.per*8) run the frame's cells (the left and right columns
-  // outside Deep), one cell per thread for ngf shot groups (the per-cell
-  // set-up once per ngf groups); the next tb.per*8 the top and bottom bands
-  // in lane vectors of VX cells for ngb groups; the rest Deep's lane vectors.
-  // The frame goes first, being the longer work per cell.
+    TileMap tm, Frame f, int ngf, int G, TileMap tb, int ngb, TileMap ts) {
+  // blocks [0, f.per*8) run the frame's cells (the four corners outside
+  // Deep's rows and columns), one cell per thread for ngf shot groups (the
+  // per-cell set-up once per ngf groups); the next tb.per*8 the top and
+  // bottom bands in lane vectors for ngb groups; the next ts.per*8 the left
+  // and right columns over Deep's rows, one cell per lane for ngb groups; the
+  // rest Deep's lane vectors.  The longer work per cell goes first.
   const int nf = f.per * kXcds;
   const int nbnd = tb.per * kXcds;
+  const int nsd = ts.per * kXcds;
   const Grid& g = hc.g;
   if ((int)blockIdx.x < nf) {
     int zc, xc, gc;
@@ -915,8 +1022,26 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
                                   gpart, src, wn, recon, B, hc, k);
     return;
   }
+  if ((int)blockIdx.x < nf + nbnd + nsd) {
+    // side tiles: tx 0 the left column (x = lane), tx 1 the right one
+    // (x = dp.x1 + lane); both are narrower than 64 cells
+    int tx, tzi, gb;
+    if (!tile_of_w(ts, (int)blockIdx.x - nf - nbnd, tx, tzi, gb)) return;
+    const int lane = (int)(threadIdx.x % kLanes);
+    const int z = dp.z0 + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+    const int x = tx == 0 ? lane : dp.x1 + lane;
+    if (z >= dp.z1 || (tx == 0 ? x >= dp.x0 : x >= g.nx)) return;
+    const int grp = gb * ngb, ng = min(ngb, G - grp);
+    if (tx == 0)
+      habc_adj_side<S, true>(z, x, grp, ng, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon,
+                             B, hc, k);
+    else
+      habc_adj_side<S, false>(z, x, grp, ng, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon,
+                              B, hc, k);
+    return;
+  }
   int tx, tzi, grp;
-  if (!tile_of_w(tm, (int)blockIdx.x - nf - nbnd, tx, tzi, grp)) return;
+  if (!tile_of_w(tm, (int)blockIdx.x - nf - nbnd - nsd, tx, tzi, grp)) return;
   const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
   const int zb = dp.z0 + (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * kHabcRZ;
   if (x0 >= dp.x1 || zb >= dp.z1) return;
@@ -1170,9 +1295,10 @@ int habc_vx(int64_t nx, std::initializer_list<const void*> ptrs) {
 }
 
 // The frame around dp (the whole grid if dp is empty) as rectangles of tiles;
-// with sides_only, just the full-height columns left and right of dp (the
-// top and bottom bands over dp's columns go to band_tiles).
-Frame make_frame(const Grid& g, const Deep& dp, int groups, bool sides_only = false) {
+// with corners_only, just the four corners outside dp's rows and columns (the
+// bands over dp's columns go to band_tiles, the columns over dp's rows to
+// side_tiles).
+Frame make_frame(const Grid& g, const Deep& dp, int groups, bool corners_only = false) {
   Frame f{};
   auto add = [&](int z0, int z1, int x0, int x1) {
     if (z1 <= z0 || x1 <= x0) return;
@@ -1183,9 +1309,11 @@ Frame make_frame(const Grid& g, const Deep& dp, int groups, bool sides_only = fa
   };
   if (dp.z1 <= dp.z0 || dp.x1 <= dp.x0) {
     add(0, g.nz, 0, g.nx);
-  } else if (sides_only) {
-    add(0, g.nz, 0, dp.x0);
-    add(0, g.nz, dp.x1, g.nx);
+  } else if (corners_only) {
+    add(0, dp.z0, 0, dp.x0);
+    add(0, dp.z0, dp.x1, g.nx);
+    add(dp.z1, g.nz, 0, dp.x0);
+    add(dp.z1, g.nz, dp.x1, g.nx);
   } else {
     add(0, dp.z0, 0, g.nx);
     add(dp.z1, g.nz, 0, g.nx);
@@ -1209,6 +1337,19 @@ TileMap band_tiles(const Grid& g, const Deep& dp, int vx, int groups) {
   const int rows = dp.z0 + (g.nz - dp.z1);
   const Grid gb{rows, dp.x1 - dp.x0, 0, (int64_t)rows * (dp.x1 - dp.x0)};
   return make_tiles(gb, vx, 1, groups);
+}
+
+// Tiles of the left and right columns of the adjoint (x in [0, dp.x0) and
+// [dp.x1, nx) over dp's rows): one 64-lane tile column each (both are at
+// most kW+3+3 cells wide); no tiles if dp is empty.
+TileMap side_tiles(const Grid& g, const Deep& dp, int groups) {
+  if (dp.z1 <= dp.z0 || dp.x1 <= dp.x0) return TileMap{0, 0, 0, 0};
+  TileMap m;
+  m.ntx = 2;
+  m.ntz = (dp.z1 - dp.z0 + kRowsOfThreads - 1) / kRowsOfThreads;
+  m.groups = groups;
+  m.per = (int)(((int64_t)m.ntx * m.ntz * groups + kXcds - 1) / kXcds);
+  return m;
 }
 
 int habc_check(int64_t B, int64_t nz, int64_t nx, float dt, float h, int S) {
@@ -1517,6 +1658,9 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   const Frame fr = make_frame(g, dp, (G + ngf - 1) / ngf, true);
   const int ngb = G < kBandGroups ? G : kBandGroups;  // groups per band thread
   const TileMap tmb = band_tiles(g, dp, band_vx(VX), (G + ngb - 1) / ngb);
+  const TileMap tms = side_tiles(g, dp, (G + ngb - 1) / ngb);
+  SEIS_CHECK_ARG(tms.per == 0 || (dp.x0 <= kLanes && (int)nx - dp.x1 <= kLanes),
+                 "side columns wider than a wave");
   const seis::TimeConsts k = seis::make_consts(dt, h);
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -1534,9 +1678,10 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   for (int64_t kk = nt; kk >= 1; --kk) {
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
-    SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX, dim3((tmd.per + fr.per + tmb.per) * kXcds), s,
-                       vp, b, L1, L2, L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon,
-                       (int)B, hc, k, dp, tmd, fr, ngf, G, tmb, ngb);
+    SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX,
+                       dim3((tmd.per + fr.per + tmb.per + tms.per) * kXcds), s, vp, b, L1, L2,
+                       L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k, dp,
+                       tmd, fr, ngf, G, tmb, ngb, tms);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
     if (work > 0) {

@@ -97,3 +97,24 @@ def evaluate(net, pairs):
         zero += float((t ** 2).sum())
         cells += t.numel()
     return tot / cells, zero / cells
+
+
+def heldout_errors(net, pairs):
+    """Squared errors, summed over held-out pairs, against target = true -
+    start: (zero direction, best multiple of the max-normalised gradient,
+    network, best multiple of the network's output).  The second is the
+    plain method's update direction with its step chosen in hindsight, the
+    fairest form of the method without the network."""
+    z = gs = nn = nb = 0.0
+    for g, m, t in pairs:
+        o = net.forward(g[None].contiguous(), m[None].contiguous(),
+                        precond.input_scales(g[None], m[None]))[0]
+        gn = g / g.abs().max().clamp_min(1e-30)
+        a = float((gn * t).sum() / (gn * gn).sum().clamp_min(1e-30))
+        b = float((o * t).sum() / (o * o).sum().clamp_min(1e-30))
+        z += float((t * t).sum())
+        gs += float(((a * gn - t) ** 2).sum())
+        nn += float(((o - t) ** 2).sum())
+        nb += float(((b * o - t) ** 2).sum())
+    return z, gs, nn, nb
+
