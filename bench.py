@@ -64,9 +64,16 @@ def parse():
                         "timing events are then unavailable)")
     p.add_argument("--snapshot-every", type=int, default=500,
                    help="generator: field-pair snapshot spacing of the batched adjoint")
-    p.add_argument("--pair-models", type=int, default=512,
+    p.add_argument("--pair-models", type=int, default=5000,
                    help="precond: generator models whose gradients make the training pairs")
-    p.add_argument("--train-steps", type=int, default=1500, help="precond: AdamW steps")
+    p.add_argument("--train-steps", type=int, default=5000, help="precond: AdamW steps")
+    p.add_argument("--save-net", default=None,
+                   help="precond: save the trained network (torch state dict + .json)")
+    p.add_argument("--load-net", default=None,
+                   help="precond: FWI with a saved network instead of training one")
+    p.add_argument("--precond-model", choices=["generator", "anomaly"], default="generator",
+                   help="precond: true model of the FWI comparison (the generator's family "
+                        "at 800x2400, or the anomaly model of the FWI-gradient workload)")
     p.add_argument("--fwi-iters", type=int, default=30, help="precond: FWI iterations per method")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a plain 1 GiB device copy (HBM-counter calibration)")
@@ -554,44 +561,72 @@ def run_rho(args, dev):
 
 def run_precond(args, dev):
     """Learned preconditioner: pairs from the generator, AdamW training with
-    the hand-written kernels, then plain and preconditioned FWI on the
-    800x2400 64-shot problem; value = wall-clock of the preconditioned run to
-    reach the plain run's final misfit / the plain run's wall-clock."""
+    the hand-written kernels (or a saved network), then plain and
+    preconditioned FWI on the 800x2400 grid with the 64-shot 8 Hz
+    acquisition (workloads.precond_fwi: by default a true model of the
+    generator's family at that size); value = wall-clock of the
+    preconditioned run to reach the plain run's final misfit / the plain
+    run's wall-clock.  Both runs take the same post-processed descent
+    direction (source illumination, two Gaussian passes of sigma 4)."""
     import torch
-    from seistorch_amd import fwi, learned, precond
+    from seistorch_amd import fwi, learned, precond, workloads
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    t0 = time.perf_counter()
-    pairs = learned.make_pairs(args.pair_models, dev, log=log)
-    t_pairs = time.perf_counter() - t0
-    n = len(pairs)
-    n_val = max(1, n // 10)
-    val, trn = pairs[:n_val], pairs[n_val:]
     p = precond.Params(dev, seed=0)
     net = precond.UNet(p)
-    opt = precond.AdamW(p, lr=1e-3)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    hist = learned.train(net, opt, trn, args.train_steps, log=log)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t1
-    val_loss, val_zero = learned.evaluate(net, val)
-    log(f"[bench] pairs {n} ({t_pairs:.1f} s), training {t_train:.1f} s, validation loss "
-        f"{val_loss:.4f} vs zero direction {val_zero:.4f}")
-    if args.fwi_iters == 0:  # training only (profiling runs)
-        print(json.dumps({"training": {"pairs": n, "pairs_s": round(t_pairs, 1),
-                                       "train_s": round(t_train, 1), "loss_history": hist,
-                                       "val_loss": val_loss, "val_zero_loss": val_zero}}),
-              flush=True)
+    training = {}
+    if args.load_net:
+        p.load_state_dict(torch.load(args.load_net, map_location="cpu", weights_only=True))
+        training = {"loaded": os.path.basename(args.load_net)}
+        info = args.load_net + ".json"
+        if os.path.exists(info):
+            training.update(json.load(open(info)))
+    else:
+        t0 = time.perf_counter()
+        pairs = learned.make_pairs(args.pair_models, dev, log=log)
+        t_pairs = time.perf_counter() - t0
+        n = len(pairs)
+        n_val = max(1, n // 20)
+        val, trn = pairs[:n_val], pairs[n_val:]
+        opt = precond.AdamW(p, lr=1e-3)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hist = learned.train(net, opt, trn, args.train_steps, log=log, log_every=500, flip=True)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t1
+        z, gs, nn, nb = learned.heldout_errors(net, val)
+        training = {"pairs": n, "train_pairs": n - n_val, "held_out_pairs": n_val,
+                    "augmentation": "random x-mirror", "pairs_s": round(t_pairs, 1),
+                    "train_steps": args.train_steps, "train_s": round(t_train, 1),
+                    "loss_history": hist,
+                    "held_out_error_over_zero": {"best_multiple_of_gradient": round(gs / z, 4),
+                                                 "network": round(nn / z, 4),
+                                                 "best_multiple_of_network": round(nb / z, 4)}}
+        log(f"[bench] pairs {n} ({t_pairs:.1f} s), training {t_train:.1f} s, held-out "
+            f"{training['held_out_error_over_zero']}")
+        del pairs, val, trn
+        if args.save_net:
+            torch.save(p.state_dict(), args.save_net)
+            json.dump(training, open(args.save_net + ".json", "w"))
+    if args.fwi_iters == 0:  # training only
+        print(json.dumps({"training": training}), flush=True)
         return
-    P = build_problem(dev, nt_override=args.nt)
-    prob = fwi.Problem(P["vp_true"], P["vp_init"], P["d"], P["w"], P["src"], P["rec"], P["B"],
-                       P["dt"], P["h"])
+    W = workloads.precond_fwi(nt=args.nt, model=args.precond_model)
+    from seistorch_amd.pml import generate_pml_coefficients_2d
+    prob = fwi.Problem(torch.from_numpy(W["vp_true"]).to(dev),
+                       torch.from_numpy(W["vp_init"]).to(dev),
+                       generate_pml_coefficients_2d((W["nz"], W["nx"]), 50).to(dev),
+                       torch.from_numpy(W["w"]).to(dev),
+                       torch.from_numpy(W["src"].astype(np.int32)).to(dev),
+                       torch.from_numpy(W["rec_flat"]).to(dev), W["B"], W["dt"], W["h"])
     prob.obs = prob.records(prob.vp_true)
-    _, hp = fwi.run_plain(prob, args.fwi_iters, log=log)
-    _, hn = fwi.run_preconditioned(prob, net, args.fwi_iters, log=log)
+    sm = {"counts": 2, "sigma": {"z": 4.0, "x": 4.0}, "radius": {"z": 8, "x": 8}}
+    kw = dict(source_illumination=True, smooth=sm)
+    _, hp = fwi.run_plain(prob, args.fwi_iters, log=log, **kw)
+    _, hn = fwi.run_preconditioned(prob, net, args.fwi_iters, log=log, net_iters=1,
+                                   scales=(1.0, 0.5, 0.25), **kw)
     target = hp.misfit[-1]
     t_hit = fwi.time_to(hn, target)
     ratio = (t_hit / hp.t[-1]) if t_hit is not None else None
@@ -601,17 +636,18 @@ def run_precond(args, dev):
         "n_gpus": 1, "steps": args.fwi_iters, "warmup": 0,
         "ms_per_step": round(hn.t[-1] / max(len(hn.t) - 1, 1) * 1e3, 1),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
-        "data": "synthetic (generator pairs for training; seeded anomaly model for FWI)",
-        "config": {"workload": "U-Net preconditioned FWI vs plain FWI, 800x2400, 64 shots",
-                   "pair_models": args.pair_models, "pairs": n, "train_steps": args.train_steps,
-                   "fwi_iters": args.fwi_iters, "nt": P["nt"]},
-        "training": {"pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1),
-                     "loss_history": hist, "val_loss": val_loss, "val_zero_loss": val_zero},
+        "data": "synthetic (generator pairs for training; a seeded model for FWI)",
+        "config": {"workload": "U-Net preconditioned FWI vs plain FWI, 800x2400, 64 shots, 8 Hz",
+                   "fwi_model": args.precond_model, "fwi_iters": args.fwi_iters,
+                   "nt": W["nt"], "descent_direction": "gradient / source illumination, "
+                   "2 Gaussian passes sigma 4 cells"},
+        "training": training,
         "plain": {"t": [round(x, 2) for x in hp.t], "misfit": hp.misfit, "rmse": hp.rmse},
         "preconditioned": {"t": [round(x, 2) for x in hn.t], "misfit": hn.misfit,
                            "rmse": hn.rmse, "kind": hn.kind},
         "misfit_at_plain_end": {"plain": target,
                                 "preconditioned": fwi.misfit_at(hn, hp.t[-1])},
+        "rmse_at_plain_end": {"plain": hp.rmse[-1], "preconditioned": hn.rmse[-1]},
     }
     print(json.dumps(out), flush=True)
 

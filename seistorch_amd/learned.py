@@ -68,15 +68,23 @@ def collate(pairs):
     return G, Mo, T, Mk
 
 
-def train(net, opt, pairs, steps, batch=8, seed=0, log=None, log_every=100):
+def _flip(pair):
+    return tuple(torch.flip(a, [1]) for a in pair)
+
+
+def train(net, opt, pairs, steps, batch=8, seed=0, log=None, log_every=100, flip=False):
     """AdamW steps on random mini-batches of whole-model pairs; returns the
-    (step, loss) history."""
+    (step, loss) history.  flip: each drawn pair is mirrored in x with
+    probability 1/2 (the generator's acquisition, evenly spaced surface shots
+    and a receiver at every surface cell, is symmetric under the mirror up to
+    the rounding of the shot positions)."""
     gen = torch.Generator().manual_seed(seed)
     n = len(pairs)
     losses = []
     for s in range(steps):
         idx = torch.randint(0, n, (min(batch, n),), generator=gen).tolist()
-        g, m, t, mk = collate([pairs[i] for i in idx])
+        flips = torch.rand(len(idx), generator=gen).lt(0.5).tolist() if flip else [False] * len(idx)
+        g, m, t, mk = collate([_flip(pairs[i]) if f else pairs[i] for i, f in zip(idx, flips)])
         sc = precond.input_scales(g, m)
         loss = precond.train_step(net, opt, g, m, sc, t, mk)
         if (s + 1) % log_every == 0 or s == steps - 1:

@@ -13,6 +13,12 @@
   smooth perturbation; start model without the perturbations; PML 50,
   10 Hz Ricker on txx and tzz, dt = 0.5 ms, nt = 6000, 32 shots, 1800
   receivers recording vx and vz.
+* ``precond_fwi``       the learned preconditioner's FWI problem: the grid,
+  shots, source, time axis and PML of ``fwi_gradient`` with a true model
+  drawn from the uneven generator's family at that size (layers, faults and
+  Gaussian anomalies, ``generator.random_model``), the start model smoothed
+  with sigma 20 cells as the generator's start models are; ``model=
+  "anomaly"`` takes ``fwi_gradient``'s model instead.
 The uneven multi-model batch lives in ``generator``.
 
 Padding follows seistorch/geom.py:213-240 (edge padding by 50 cells) and
@@ -71,3 +77,24 @@ def elastic(seed=synthetic.DEFAULT_SEED, nt=None):
     P["w"] = ricker_wave(fm, dt, nt, delay=160).numpy()
     _acquisition(P, nx_i, nshots)
     return P
+
+
+def precond_fwi(seed=11, nt=None, model="generator"):
+    if model == "anomaly":
+        return fwi_gradient(nt=nt)
+    if model != "generator":
+        raise ValueError(f"model must be 'generator' or 'anomaly', got {model!r}")
+    from scipy.ndimage import gaussian_filter
+    from . import generator
+    nz_i, nx_i, nshots, dt, h, fm = 800, 2400, 64, 0.8e-3, 10.0, 8.0
+    nt = nt or 5000
+    spec = generator.GenSpec()
+    true = generator.random_model(np.random.default_rng(seed), nz_i, nx_i, spec)
+    init = gaussian_filter(true.astype(np.float64), spec.smooth, mode="nearest")
+    vt, vi = pad_model(true).astype(np.float32), pad_model(init.astype(np.float32))
+    P = dict(nz_i=nz_i, nx_i=nx_i, nz=vi.shape[0], nx=vi.shape[1], B=nshots, nt=nt, dt=dt,
+             h=h, fm=fm, vp_true=vt, vp_init=vi.astype(np.float32))
+    P["w"] = ricker_wave(fm, dt, nt, delay=256).numpy()
+    _acquisition(P, nx_i, nshots)
+    return P
+
