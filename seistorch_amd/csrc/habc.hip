@@ -151,14 +151,12 @@ constexpr int kFrameGroups = 8;
 // Shot groups per thread of the adjoint's top and bottom bands.
 constexpr int kBandGroups = 8;
 
-// Cells per lane on the adjoint's bands, at most the Deep lane width VX (so
-// Deep's column bounds stay aligned).  The bands share the kernel with Deep,
-// whose occupancy follows the larger branch: one cell per lane keeps the
-// kernel at 98 VGPRs for VX = 4 (5 waves per SIMD); two took 120 and four 161.
+// Cells per lane on the adjoint's bands: one, which divides every Deep lane
+// width (Deep's column bounds are multiples of it).  Two and four cells per
+// lane took 120 and 161 VGPRs in the fused kernel against 98.
 constexpr int kBandVX = 1;
-__host__ __device__ constexpr int band_vx(int vx) { return vx < kBandVX ? vx : kBandVX; }
 
-// Rows per thread of the vectorised adjoint on Deep.  One: the deep and
+// Rows per thread of the vectorised adjoint on Deep in the fused kernel.  One: the deep and
 // frame block ranges share one kernel, whose register count (and so the
 // frame threads' occupancy) follows the larger branch; two rows per thread
 // took 146 VGPRs against 91 and made the launch 20 % slower.
@@ -981,38 +979,39 @@ __global__ __launch_bounds__(256) void k_rho_grad(
 // habc_adj_cell with the same operands in the same order (bit-identical).
 // The other runs habc_adj_cell on every other cell, one cell per thread.
 // Requires y1, y2, um and gpart (the fused loop's call).
-template <int S, int VX>
-__global__ __launch_bounds__(256) void k_habc_adj_deep(
-    const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
-    const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
+// Block w of the strip ranges of the adjoint (-> false if w is past them):
+// [0, f.per*8) the frame's cells (the four corners outside Deep's rows and
+// columns), one cell per thread for ngf shot groups (the per-cell set-up once
+// per ngf groups); the next tb.per*8 the top and bottom bands in lane vectors
+// for ngb groups; the next ts.per*8 the left and right columns over Deep's
+// rows, one cell per lane for ngb groups.
+template <int S>
+__device__ __forceinline__ bool habc_adj_strip_block(
+    int w, const float* __restrict__ vp, const float* __restrict__ bw,
+    const float* __restrict__ y1, const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
     float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
-    const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
-    TileMap tm, Frame f, int ngf, int G, TileMap tb, int ngb, TileMap ts) {
-  // blocks [0, f.per*8) run the frame's cells (the four corners outside
-  // Deep's rows and columns), one cell per thread for ngf shot groups (the
-  // per-cell set-up once per ngf groups); the next tb.per*8 the top and
-  // bottom bands in lane vectors for ngb groups; the next ts.per*8 the left
-  // and right columns over Deep's rows, one cell per lane for ngb groups; the
-  // rest Deep's lane vectors.  The longer work per cell goes first.
+    const float* __restrict__ wn, int recon, int B, const HabcCtx& hc,
+    const seis::TimeConsts& k, const Deep& dp, const Frame& f, int ngf, int G,
+    const TileMap& tb, int ngb, const TileMap& ts) {
   const int nf = f.per * kXcds;
   const int nbnd = tb.per * kXcds;
   const int nsd = ts.per * kXcds;
   const Grid& g = hc.g;
-  if ((int)blockIdx.x < nf) {
+  if (w < nf) {
     int zc, xc, gc;
-    if (frame_cell(f, (int)blockIdx.x, zc, xc, gc))
+    if (frame_cell(f, w, zc, xc, gc))
       habc_adj_cell<S>(zc, xc, gc * ngf, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon, B,
                        hc, k, min(ngf, G - gc * ngf));
-    return;
+    return true;
   }
-  if ((int)blockIdx.x < nf + nbnd) {
+  if (w < nf + nbnd) {
     int tx, tzi, gb;
-    if (!tile_of_w(tb, (int)blockIdx.x - nf, tx, tzi, gb)) return;
-    constexpr int VB = band_vx(VX);
+    if (!tile_of_w(tb, w - nf, tx, tzi, gb)) return true;
+    constexpr int VB = kBandVX;
     const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VB;
     const int r = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
     const int ztop = dp.z0;  // rows [0, dp.z0) then [dp.z1, nz)
-    if (x0 >= dp.x1 || r >= ztop + (g.nz - dp.z1)) return;
+    if (x0 >= dp.x1 || r >= ztop + (g.nz - dp.z1)) return true;
     const int grp = gb * ngb, ng = min(ngb, G - grp);
     if (r < ztop)
       habc_adj_band<S, VB, true>(r, x0, grp, ng, vp, bw, y1, y2, out, um, uc, gpart, src, wn,
@@ -1020,17 +1019,17 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
     else
       habc_adj_band<S, VB, false>(dp.z1 + (r - ztop), x0, grp, ng, vp, bw, y1, y2, out, um, uc,
                                   gpart, src, wn, recon, B, hc, k);
-    return;
+    return true;
   }
-  if ((int)blockIdx.x < nf + nbnd + nsd) {
+  if (w < nf + nbnd + nsd) {
     // side tiles: tx 0 the left column (x = lane), tx 1 the right one
     // (x = dp.x1 + lane); both are narrower than 64 cells
     int tx, tzi, gb;
-    if (!tile_of_w(ts, (int)blockIdx.x - nf - nbnd, tx, tzi, gb)) return;
+    if (!tile_of_w(ts, w - nf - nbnd, tx, tzi, gb)) return true;
     const int lane = (int)(threadIdx.x % kLanes);
     const int z = dp.z0 + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
     const int x = tx == 0 ? lane : dp.x1 + lane;
-    if (z >= dp.z1 || (tx == 0 ? x >= dp.x0 : x >= g.nx)) return;
+    if (z >= dp.z1 || (tx == 0 ? x >= dp.x0 : x >= g.nx)) return true;
     const int grp = gb * ngb, ng = min(ngb, G - grp);
     if (tx == 0)
       habc_adj_side<S, true>(z, x, grp, ng, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon,
@@ -1038,17 +1037,29 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
     else
       habc_adj_side<S, false>(z, x, grp, ng, vp, bw, y1, y2, out, um, uc, gpart, src, wn, recon,
                               B, hc, k);
-    return;
+    return true;
   }
-  int tx, tzi, grp;
-  if (!tile_of_w(tm, (int)blockIdx.x - nf - nbnd - nsd, tx, tzi, grp)) return;
+  return false;
+}
+
+// One tile of Deep (z in [z0, z1), x in [x0, x1), at least kW+3 cells from
+// every edge, x0 and x1 multiples of VX): there no strip term reaches, so a
+// lane runs a vectorised copy of the per-cell deep path of habc_adj_cell with
+// the same operands in the same order (bit-identical), for RZ rows.
+template <int S, int VX, int RZ>
+__device__ __forceinline__ void habc_adj_deep_tile(
+    int tx, int tzi, int grp, const float* __restrict__ vp, const float* __restrict__ bw,
+    const float* __restrict__ y1, const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int recon, int B, const HabcCtx& hc,
+    const seis::TimeConsts& k, const Deep& dp) {
+  const Grid& g = hc.g;
   const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
-  const int zb = dp.z0 + (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * kHabcRZ;
+  const int zb = dp.z0 + (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
   if (x0 >= dp.x1 || zb >= dp.z1) return;
   // every cell of Deep and its neighbours are interior, away from the strips;
-  // a thread owns rows zb .. zb+kHabcRZ-1 (those below dp.z1) and reads rows
-  // zb-1 .. zb+kHabcRZ once per shot for all of them
-  constexpr int RZ = kHabcRZ;
+  // a thread owns rows zb .. zb+RZ-1 (those below dp.z1) and reads rows
+  // zb-1 .. zb+RZ once per shot for all of them
   const int nr = min(RZ, dp.z1 - zb);
   const float A0 = 1.0f / k.idt2;
   const float B0 = k.idt2;
@@ -1143,6 +1154,30 @@ __global__ __launch_bounds__(256) void k_habc_adj_deep(
     vstore<VX>(gpart + (int64_t)grp * g.plane + i, gv);
   }
 }
+
+
+// The fused adjoint step in one launch: the strip ranges of
+// habc_adj_strip_block first (the longer work per cell), then Deep's lane
+// vectors (one row per thread: the kernel's register count, and so the strip
+// threads' occupancy, follows the larger branch).  Requires y1, y2, um and
+// gpart (the fused loop's call).
+template <int S, int VX>
+__global__ __launch_bounds__(256) void k_habc_adj_deep(
+    const float* __restrict__ vp, const float* __restrict__ bw, const float* __restrict__ y1,
+    const float* __restrict__ y2, float* __restrict__ out, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int recon, int B, HabcCtx hc, seis::TimeConsts k, Deep dp,
+    TileMap tm, Frame f, int ngf, int G, TileMap tb, int ngb, TileMap ts) {
+  if (habc_adj_strip_block<S>((int)blockIdx.x, vp, bw, y1, y2, out, um, uc, gpart, src, wn,
+                              recon, B, hc, k, dp, f, ngf, G, tb, ngb, ts))
+    return;
+  const int w = (int)blockIdx.x - (f.per + tb.per + ts.per) * kXcds;
+  int tx, tzi, grp;
+  if (!tile_of_w(tm, w, tx, tzi, grp)) return;
+  habc_adj_deep_tile<S, VX, kHabcRZ>(tx, tzi, grp, vp, bw, y1, y2, out, um, uc, gpart, src, wn,
+                                     recon, B, hc, k, dp);
+}
+
 
 // Per-step model gradient (all cells): sum over shots of
 // alpha*A0*2c*L(u)*gy + sum_s beta_s*gy*(dl/dc (u1+p0-u0-p1) + dm/dc (2u1-u0-u2)).
@@ -1657,7 +1692,7 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
   const int ngf = G < kFrameGroups ? G : kFrameGroups;  // groups per frame thread
   const Frame fr = make_frame(g, dp, (G + ngf - 1) / ngf, true);
   const int ngb = G < kBandGroups ? G : kBandGroups;  // groups per band thread
-  const TileMap tmb = band_tiles(g, dp, band_vx(VX), (G + ngb - 1) / ngb);
+  const TileMap tmb = band_tiles(g, dp, kBandVX, (G + ngb - 1) / ngb);
   const TileMap tms = side_tiles(g, dp, (G + ngb - 1) / ngb);
   SEIS_CHECK_ARG(tms.per == 0 || (dp.x0 <= kLanes && (int)nx - dp.x1 <= kLanes),
                  "side columns wider than a wave");
@@ -1680,8 +1715,8 @@ int seis_habc_backward(const float* vp, const float* b, const float* wavelet, in
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     SEIS_HABC_LAUNCH_V(k_habc_adj_deep, S, VX,
                        dim3((tmd.per + fr.per + tmb.per + tms.per) * kXcds), s, vp, b, L1, L2,
-                       L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k, dp,
-                       tmd, fr, ngf, G, tmb, ngb, tms);
+                       L0, X, Y, grad_part, src_zx, wavelet + (kk - 1), recon, (int)B, hc, k,
+                       dp, tmd, fr, ngf, G, tmb, ngb, tms);
     SEIS_PROF(SEIS_KERNEL_HABC_BWD, s);
     const int64_t work = R + (recon ? B * ring_len : 0);
     if (work > 0) {
