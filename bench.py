@@ -21,6 +21,7 @@ import time
 import numpy as np
 
 METRIC = "Gcell-updates/s fwd+adjoint, % HBM roofline; wall-clock to target misfit"
+HELD_OUT_SEED = 20240611  # generator stream of the preconditioner's held-out pairs
 HBM_PEAK = 8.0e12  # B/s, MI355X
 # algorithmic bytes per cell-update of each timed kernel (fields touched once
 # per step, damping/coefficients read once per cell)
@@ -67,6 +68,12 @@ def parse():
     p.add_argument("--pair-models", type=int, default=5000,
                    help="precond: generator models whose gradients make the training pairs")
     p.add_argument("--train-steps", type=int, default=5000, help="precond: AdamW steps")
+    p.add_argument("--pair-first", type=int, default=0,
+                   help="precond: index of the first generator model of this run's pairs")
+    p.add_argument("--val-models", type=int, default=250,
+                   help="precond: held-out models (a second seeded stream)")
+    p.add_argument("--init-net", default=None,
+                   help="precond: continue training this saved network (its .json is carried)")
     p.add_argument("--save-net", default=None,
                    help="precond: save the trained network (torch state dict + .json)")
     p.add_argument("--load-net", default=None,
@@ -584,29 +591,39 @@ def run_precond(args, dev):
         if os.path.exists(info):
             training.update(json.load(open(info)))
     else:
+        prior = {}
+        if args.init_net:  # continue training a saved network on further pairs
+            p.load_state_dict(torch.load(args.init_net, map_location="cpu", weights_only=True))
+            info = args.init_net + ".json"
+            prior = json.load(open(info)) if os.path.exists(info) else {}
         t0 = time.perf_counter()
-        pairs = learned.make_pairs(args.pair_models, dev, log=log)
+        trn = learned.make_pairs(args.pair_models, dev, first=args.pair_first, log=log)
         t_pairs = time.perf_counter() - t0
-        n = len(pairs)
-        n_val = max(1, n // 20)
-        val, trn = pairs[:n_val], pairs[n_val:]
+        # held-out pairs: the first models of a second seeded stream
+        val = learned.make_pairs(args.val_models, dev, seed=HELD_OUT_SEED)
         opt = precond.AdamW(p, lr=1e-3)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        hist = learned.train(net, opt, trn, args.train_steps, log=log, log_every=500, flip=True)
+        hist = learned.train(net, opt, trn, args.train_steps, log=log, log_every=500, flip=True,
+                             seed=args.pair_first)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t1
         z, gs, nn, nb = learned.heldout_errors(net, val)
-        training = {"pairs": n, "train_pairs": n - n_val, "held_out_pairs": n_val,
-                    "augmentation": "random x-mirror", "pairs_s": round(t_pairs, 1),
-                    "train_steps": args.train_steps, "train_s": round(t_train, 1),
-                    "loss_history": hist,
+        seen = prior.get("pairs", 0) + len(trn)
+        training = {"pairs": seen, "pairs_this_run": len(trn), "first_model": args.pair_first,
+                    "held_out_pairs": len(val), "augmentation": "random x-mirror",
+                    "pairs_s": round(prior.get("pairs_s", 0.0) + t_pairs, 1),
+                    "train_steps": prior.get("train_steps", 0) + args.train_steps,
+                    "train_s": round(prior.get("train_s", 0.0) + t_train, 1),
+                    "loss_history": prior.get("loss_history", []) + hist,
                     "held_out_error_over_zero": {"best_multiple_of_gradient": round(gs / z, 4),
                                                  "network": round(nn / z, 4),
                                                  "best_multiple_of_network": round(nb / z, 4)}}
+        if prior:
+            training["runs"] = prior.get("runs", 1) + 1
         log(f"[bench] pairs {n} ({t_pairs:.1f} s), training {t_train:.1f} s, held-out "
             f"{training['held_out_error_over_zero']}")
-        del pairs, val, trn
+        del val, trn
         if args.save_net:
             torch.save(p.state_dict(), args.save_net)
             json.dump(training, open(args.save_net + ".json", "w"))

@@ -101,9 +101,12 @@ def make_instance(seed, spec=GenSpec()):
                 src=src.astype(np.int32), rec_flat=rec_flat_index(rec, NZ, NX))
 
 
-def make_instances(seed=DEFAULT_SEED, spec=GenSpec()):
+def make_instances(seed=DEFAULT_SEED, spec=GenSpec(), first=0):
+    """Instances first .. first + spec.n_models - 1 of the seeded stream (the
+    i-th instance's seed depends on seed and i only)."""
     ss = np.random.SeedSequence(seed)
-    return [make_instance(int(s.generate_state(1)[0]), spec) for s in ss.spawn(spec.n_models)]
+    kids = ss.spawn(first + spec.n_models)[first:]
+    return [make_instance(int(s.generate_state(1)[0]), spec) for s in kids]
 
 
 def cell_updates(insts):

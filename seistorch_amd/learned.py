@@ -23,11 +23,12 @@ PAD = generator.PAD
 
 
 def make_pairs(n_models, device, seed=20230503, mute=5, chunk=256, vscale=500.0, spec=None,
-               log=None):
-    """[(grad, model, target)] per model, each (h, w) FP32 on the device."""
+               log=None, first=0):
+    """[(grad, model, target)] per model, each (h, w) FP32 on the device, for
+    the generator's instances first .. first + n_models - 1 of the seed."""
     spec = spec or generator.GenSpec()
     spec = generator.GenSpec(**{**spec.__dict__, "n_models": n_models})
-    host = generator.make_instances(seed=seed, spec=spec)
+    host = generator.make_instances(seed=seed, spec=spec, first=first)
     out = []
     t0 = time.perf_counter()
     for c0 in range(0, len(host), chunk):
