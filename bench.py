@@ -549,6 +549,36 @@ def run_rho(args, dev):
                      "bytes_per_cell_update": bytes_per_cell},
         "checksum": float(rec.double().abs().sum()),
     }
+    # the gradient (vp and rho): forward with ring and snapshots, then the
+    # boundary-saving adjoint (k_rho_adj, k_rho_inject, k_rho_grad per step)
+    G = torch.randn_like(rec)
+    vp = P["vp_init"].detach().clone().requires_grad_(True)
+    rr = rho.detach().clone().requires_grad_(True)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    r = ar.propagate(vp, rr, b, P["w"], P["src"], P["rec"], B, P["dt"], P["h"])
+    e1.record()
+    r.backward(G)
+    e2.record()
+    torch.cuda.synchronize()
+    f_s, a_s = e0.elapsed_time(e1) / 1e3, e1.elapsed_time(e2) / 1e3
+    # adjoint per cell-update: lambda^{k+1}, lambda^{k+2} read and lambda^k
+    # written (k_rho_adj), then lambda^k, u^{k-1}, u^k read and u^{k-2}
+    # written (k_rho_grad): 28 B; model and gradient partials shared by shots
+    adj_bytes = 28.0
+    out["gradient"] = {
+        "value": round(cells * 2 / (f_s + a_s) / 1e9, 3),
+        "unit": "Gcell-updates/s fwd+adjoint (vp and rho gradients)",
+        "forward_s": round(f_s, 3), "adjoint_s": round(a_s, 3),
+        "adjoint_step_us": round(a_s / nt * 1e6, 1),
+        "adjoint_roofline": {"bound": "hbm", "achieved": round(adj_bytes * B * nz * nx /
+                                                               (a_s / nt) / 1e9, 1),
+                             "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                             "frac": round(adj_bytes * B * nz * nx / (a_s / nt) / HBM_PEAK, 4),
+                             "bytes_per_cell_update": adj_bytes},
+        "grad_vp_abs_sum": float(vp.grad.double().abs().sum()),
+        "grad_rho_abs_sum": float(rr.grad.double().abs().sum()),
+    }
     if not args.no_cpu_baseline:
         from oracle import ref_torch
         steps = 40

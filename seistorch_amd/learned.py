@@ -22,7 +22,7 @@ from . import generator, precond
 PAD = generator.PAD
 
 
-def make_pairs(n_models, device, seed=20230503, mute=5, chunk=256, vscale=500.0, spec=None,
+def make_pairs(n_models, device, seed=20230503, mute=5, chunk=128, vscale=500.0, spec=None,
                log=None, first=0):
     """[(grad, model, target)] per model, each (h, w) FP32 on the device, for
     the generator's instances first .. first + n_models - 1 of the seed."""
@@ -44,6 +44,10 @@ def make_pairs(n_models, device, seed=20230503, mute=5, chunk=256, vscale=500.0,
             ti = (it["vp_true"][sl] - vi) / vscale
             out.append((gi, vi, ti))
         del part, obs, grads
+        # the batch's ring and field buffers differ in size from chunk to chunk:
+        # hand the cached blocks back so that the next chunk's large ring
+        # buffer is not refused for fragmentation
+        torch.cuda.empty_cache()
         if log:
             log(f"[pairs] {min(c0 + chunk, len(host))}/{len(host)} models, "
                 f"{time.perf_counter() - t0:.1f} s")
