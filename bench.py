@@ -651,7 +651,7 @@ def run_precond(args, dev):
                                                  "best_multiple_of_network": round(nb / z, 4)}}
         if prior:
             training["runs"] = prior.get("runs", 1) + 1
-        log(f"[bench] pairs {n} ({t_pairs:.1f} s), training {t_train:.1f} s, held-out "
+        log(f"[bench] pairs {len(trn)} ({t_pairs:.1f} s), training {t_train:.1f} s, held-out "
             f"{training['held_out_error_over_zero']}")
         del val, trn
         if args.save_net:
