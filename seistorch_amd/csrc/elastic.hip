@@ -339,12 +339,34 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
   int tx, tzi, grp;
   if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
   const int lane = (int)(threadIdx.x % kLanes);
+  const int trow = (int)(threadIdx.x / kLanes);
   const int x0 = (tx * kElOut + lane - 1) * VX;
-  const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
-  if (zb >= g.nz) return;  // uniform over the wave
+  const int zb0 = tzi * kRowsOfThreads * RZ;  // the block's first row
+  const int zb = zb0 + trow * RZ;
   const bool xin = x0 >= 0 && x0 < g.nx;  // nx is a multiple of VX
   const bool inner = lane > 0 && lane < kLanes - 1;
   const int64_t P = g.plane;
+  // the block's coefficient planes in LDS, rows zb0-1 .. zb0+4*RZ (local
+  // row q = z - zb0 + 1), one lane vector per lane: read from HBM once per
+  // block instead of once per shot.  Rows outside the grid are never read.
+  constexpr int kCR = kRowsOfThreads * RZ + 2;
+  __shared__ __align__(16) float lcf[kNumCoef][kCR][kLanes * VX];
+  for (int q = trow; q < kCR; q += kRowsOfThreads) {
+    const int z = zb0 - 1 + q;
+    if (!xin || z < 0 || z >= g.nz) continue;
+#pragma unroll
+    for (int c = 0; c < kNumCoef; ++c) {
+      float t[VX];
+      vload<VX>(cf + c * P + (int64_t)z * g.nx + x0, t);
+      vstore<VX>(&lcf[c][q][lane * VX], t);
+    }
+  }
+  __syncthreads();
+  if (zb >= g.nz) return;  // uniform over the wave, after the block's barrier
+  // coefficient c on local row j (z = zb - 1 + j)
+  auto cload = [&](int c, int j, float(&o)[VX]) {
+    vload<VX>(&lcf[c][trow * RZ + j][lane * VX], o);
+  };
   const float sgn = 1.0f;
   const float sdti = dti;
   // row j of the local arrays is z = zb - 1 + j
@@ -371,8 +393,8 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
         float a[VX], rc[VX];
         vload<VX>(Xi.f[0] + off + i, XV[j]);
         vload<VX>(Xi.f[1] + off + i, ZV[j]);
-        vload<VX>(cf + kAinv * P + i, a);
-        vload<VX>(cf + kRc * P + i, rc);
+        cload(kAinv, j, a);
+        cload(kRc, j, rc);
 #pragma unroll
         for (int e = 0; e < VX; ++e) {
           const float c = sgn * a[e] * rc[e];
@@ -400,7 +422,7 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
         float t2[VX], t3[VX], a[VX];
         vload<VX>(Xi.f[2] + off + i, t2);
         vload<VX>(Xi.f[3] + off + i, t3);
-        vload<VX>(cf + kAinv * P + i, a);
+        cload(kAinv, j, a);
 #pragma unroll
         for (int e = 0; e < VX; ++e) {
           const int x = x0 + e;
@@ -426,7 +448,7 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
         const bool zl = z >= 1, zr = z <= g.nz - 2;
         float t4[VX], a[VX];
         vload<VX>(Xi.f[4] + off + i, t4);
-        vload<VX>(cf + kAinv * P + i, a);
+        cload(kAinv, j, a);
 #pragma unroll
         for (int e = 0; e < VX; ++e) {
           const int x = x0 + e;
@@ -450,8 +472,8 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
       if (rok(j)) {
         const int i = idx(j);
         float l2m[VX], lam[VX];
-        vload<VX>(cf + kL2M * P + i, l2m);
-        vload<VX>(cf + kLam * P + i, lam);
+        cload(kL2M, j, l2m);
+        cload(kLam, j, lam);
 #pragma unroll
         for (int e = 0; e < VX; ++e) Q2[j][e] = sdti * (lam[e] * U1[j][e] + l2m[e] * U2[j][e]);
       } else {
@@ -459,7 +481,7 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
       }
       if (rok(j + 1)) {
         float mc[VX];
-        vload<VX>(cf + kMc * P + idx(j + 1), mc);
+        cload(kMc, j + 1, mc);
 #pragma unroll
         for (int e = 0; e < VX; ++e) Q3[j][e] = sgn * mc[e] * U3[j][e];
       } else {
@@ -473,8 +495,8 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
       if (rok(j)) {
         const int i = idx(j);
         float l2m[VX], lam[VX];
-        vload<VX>(cf + kL2M * P + i, l2m);
-        vload<VX>(cf + kLam * P + i, lam);
+        cload(kL2M, j, l2m);
+        cload(kLam, j, lam);
 #pragma unroll
         for (int e = 0; e < VX; ++e) Q1[e] = sdti * (l2m[e] * U1[j][e] + lam[e] * U2[j][e]);
       } else {
@@ -487,8 +509,8 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
       const int z = zb + r;
       const bool zl = z >= 1, zr = z <= g.nz - 2;
       float a[VX], bm[VX], o0[VX], o1[VX], o2[VX], o3[VX], o4[VX];
-      vload<VX>(cf + kAinv * P + i, a);
-      vload<VX>(cf + kBm * P + i, bm);
+      cload(kAinv, j, a);
+      cload(kBm, j, bm);
 #pragma unroll
       for (int e = 0; e < VX; ++e) {
         const int x = x0 + e;
@@ -546,7 +568,7 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
         const int i = idx(j);
         float rc[VX];
         vload<VX>(Fi.f[0] + off + i, VXN[j]);
-        vload<VX>(cf + kRc * P + i, rc);
+        cload(kRc, j, rc);
 #pragma unroll
         for (int e = 0; e < VX; ++e)
           if (seis::interior(z, x0 + e, g)) VXN[j][e] = VXN[j][e] - rc[e] * SX[j][e];
@@ -569,7 +591,7 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
         const int i = idx(j);
         float rc[VX];
         vload<VX>(Fi.f[1] + off + i, VZN[j - 1]);
-        vload<VX>(cf + kRc * P + i, rc);
+        cload(kRc, j, rc);
 #pragma unroll
         for (int e = 0; e < VX; ++e)
           if (seis::interior(z, x0 + e, g)) VZN[j - 1][e] = VZN[j - 1][e] - rc[e] * SZ[j - 1][e];
@@ -588,10 +610,10 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
       const int z = zb + r;
       const bool zl = z >= 1, zr = z <= g.nz - 2;
       float l2m[VX], lam[VX], mc[VX], a[VX];
-      vload<VX>(cf + kL2M * P + i, l2m);
-      vload<VX>(cf + kLam * P + i, lam);
-      vload<VX>(cf + kMc * P + i, mc);
-      vload<VX>(cf + kAinv * P + i, a);
+      cload(kL2M, j, l2m);
+      cload(kLam, j, lam);
+      cload(kMc, j, mc);
+      cload(kAinv, j, a);
       float txx[VX], tzz[VX], txz[VX];
 #pragma unroll
       for (int e = 0; e < VX; ++e) {
