@@ -213,12 +213,34 @@ __device__ __forceinline__ void el_step_body(const float* __restrict__ cf, float
   int tx, tzi, grp;
   if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
   const int lane = (int)(threadIdx.x % kLanes);
+  const int trow = (int)(threadIdx.x / kLanes);
   const int x0 = (tx * kElOut + lane - 1) * VX;
-  const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
-  if (zb >= g.nz) return;  // uniform over the wave
+  const int zb0 = tzi * kRowsOfThreads * RZ;  // the block's first row
+  const int zb = zb0 + trow * RZ;
   const bool xin = x0 >= 0 && x0 < g.nx;  // nx is a multiple of VX
   const bool inner = lane > 0 && lane < kLanes - 1;
   const int64_t P = g.plane;
+  // the block's coefficient planes in LDS, rows zb0-1 .. zb0+4*RZ (local
+  // row z - zb0 + 1): read from HBM once per block, not once per shot.
+  // Rows outside the grid are never read.
+  constexpr int kCR = kRowsOfThreads * RZ + 2;
+  __shared__ __align__(16) float lcf[kNumCoef][kCR][kLanes * VX];
+  for (int q = trow; q < kCR; q += kRowsOfThreads) {
+    const int z = zb0 - 1 + q;
+    if (!xin || z < 0 || z >= g.nz) continue;
+#pragma unroll
+    for (int c = 0; c < kNumCoef; ++c) {
+      float t[VX];
+      vload<VX>(cf + c * P + (int64_t)z * g.nx + x0, t);
+      vstore<VX>(&lcf[c][q][lane * VX], t);
+    }
+  }
+  __syncthreads();
+  if (zb >= g.nz) return;  // uniform over the wave, after the block's barrier
+  // coefficient c on row z
+  auto cload = [&](int c, int z, float(&o)[VX]) {
+    vload<VX>(&lcf[c][z - zb0 + 1][lane * VX], o);
+  };
   const int b0 = grp * S, nb = min(S, B - b0);
   for (int s = 0; s < nb; ++s) {
     const int64_t off = (int64_t)(b0 + s) * P;
@@ -249,11 +271,11 @@ __device__ __forceinline__ void el_step_body(const float* __restrict__ cf, float
       if (xin && zok) {
         const int i = z * g.nx + x0;
         float l2m[VX], lam[VX], mc[VX], a[VX], bm[VX];
-        vload<VX>(cf + kL2M * P + i, l2m);
-        vload<VX>(cf + kLam * P + i, lam);
-        vload<VX>(cf + kMc * P + i, mc);
-        vload<VX>(cf + kAinv * P + i, a);
-        vload<VX>(cf + kBm * P + i, bm);
+        cload(kL2M, z, l2m);
+        cload(kLam, z, lam);
+        cload(kMc, z, mc);
+        cload(kAinv, z, a);
+        cload(kBm, z, bm);
         vload<VX>(Fi.f[2] + off + i, TXX[q]);
         vload<VX>(Fi.f[3] + off + i, TZZ[q]);
         vload<VX>(Fi.f[4] + off + i, TXZ[q]);
@@ -285,9 +307,9 @@ __device__ __forceinline__ void el_step_body(const float* __restrict__ cf, float
       if (!inner || !xin || z >= g.nz) continue;
       const int i = z * g.nx + x0;
       float rc[VX], a[VX], bm[VX], vxo[VX], vzo[VX];
-      vload<VX>(cf + kRc * P + i, rc);
-      vload<VX>(cf + kAinv * P + i, a);
-      vload<VX>(cf + kBm * P + i, bm);
+      cload(kRc, z, rc);
+      cload(kAinv, z, a);
+      cload(kBm, z, bm);
       const bool zl = z > 0, zr = z < g.nz - 1;
 #pragma unroll
       for (int e = 0; e < VX; ++e) {
@@ -309,7 +331,12 @@ __device__ __forceinline__ void el_step_body(const float* __restrict__ cf, float
   }
 }
 
-constexpr int kElRZ = 2;  // rows per thread of the fused step
+// Rows per thread of the fused step.  One: with the coefficients in LDS a
+// block of one-row threads holds 36.9 KB (4-cell lanes) and keeps 4 waves per
+// SIMD; two rows per thread would need 61 KB and halve that.  495 -> 410 us
+// per step at S = 2 on the 700x1900, 32-shot grid against two rows without
+// the staging.
+constexpr int kElRZ = 1;
 
 template <int S, int VX>
 __global__ __launch_bounds__(256) void k_el_step_rz(const float* __restrict__ cf, float dti,

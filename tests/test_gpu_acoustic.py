@@ -199,8 +199,12 @@ def test_fused_adjoint_interior(S, K, nt):
     e_ref_or = rel_l2(gr.numpy()[I], go[I])
     print(f"adjoint S={S} K={K} nt={nt}: interior rel-L2 HIP vs oracle {e_or:.2e}, HIP vs "
           f"ref_torch AD {e_ref:.2e}, ref_torch AD vs oracle {e_ref_or:.2e}")
-    if nt <= 200:
+    if nt <= 200 or K != 0:
         assert e_ref < TOL
+    else:
+        # ring-only rebuild over 800 reversed steps: the FP32 rebuild drifts
+        # from the forward (the reference's boundary-saving path does too)
+        assert e_ref < 2 * TOL
     assert _oracle_ok(e_or, e_ref_or)
     # outside the undamped interior the boundary-saving gradient is defined as zero
     mask = np.ones_like(g, bool); mask[I] = False
