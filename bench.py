@@ -221,6 +221,44 @@ def cpu_baseline(P, steps):
                       f"{el:.1f} s"}
 
 
+def cpu_baseline_ad(P, workload):
+    """The reference's autograd path (boundary_saving off: autograd through
+    the loop of the FP32 torch restatement) on the host cores for the HABC
+    and elastic workloads: 1 shot on the full padded grid over the first
+    steps of the time axis (16 HABC steps, 24 elastic steps: 10-30 s),
+    forward + backward, scaled per cell-update."""
+    import torch
+    from oracle import ref_torch
+    from seistorch_amd import synthetic
+    threads = torch.get_num_threads()
+    src, rec = synthetic.surface_acquisition(P["nx_i"], P["B"], src_depth=2)
+    w = P["w"].cpu().numpy()
+    if workload == "habc":
+        steps, fn = 16, "oracle/ref_torch.habc_gradient_ad"
+        c, b = P["vp_init"].cpu().numpy(), P["b"].cpu().numpy()
+
+        def run(n):
+            G = np.ones((n, len(rec[0])), np.float32)
+            ref_torch.habc_gradient_ad(c, b, P["dt"], P["h"], w[:n], src[:1], rec[:1], G)
+    else:
+        steps, fn = 24, "oracle/ref_torch.elastic_gradient_ad"
+        models = [P[k].cpu().numpy() for k in ("vp_init", "vs_init", "rho_init", "d")]
+
+        def run(n):
+            G = np.ones((n, len(rec[0]), 2), np.float32)
+            ref_torch.elastic_gradient_ad(*models, P["dt"], P["h"], w[:n], src[:1], rec[:1],
+                                          None, rec_cotangent=G)
+    run(2)  # first-touch of the autograd buffers, untimed
+    t0 = time.perf_counter()
+    run(steps)
+    el = time.perf_counter() - t0
+    cells = P["nz"] * P["nx"] * steps
+    return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+            "cpu": cpu_model(), "kind": "port",
+            "sample": f"1 shot, {P['nz']}x{P['nx']} padded grid, {steps} of {P['nt']} steps, "
+                      f"fwd+adjoint by autograd ({fn}, torch {torch.__version__}), {el:.1f} s"}
+
+
 def copy_calibration(dev, reps=5):
     """The library's float4 streaming copy of 1 GiB (seis_stream_copy),
     timed with HIP events on its stream: its rate is the practical HBM
@@ -815,6 +853,9 @@ def main():
     if not args.no_cpu_baseline and args.workload == "acoustic":
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, args.cpu_steps)
+    elif not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_ad(P, args.workload)
     print(json.dumps(out), flush=True)
 
 
