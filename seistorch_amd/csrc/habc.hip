@@ -891,9 +891,11 @@ __global__ __launch_bounds__(256) void k_rho_inject(float* __restrict__ lam,
   if (t < R) atomicAdd(lam + rec[t], s[t]);
 }
 
-// After k_rho_adj and k_rho_inject (lam = lambda^k, the cotangent of u^k):
-// on the undamped interior, where alpha = 1 at the cell and its neighbours,
-// the model gradient of the step u^k = F(u^{k-1}, u^{k-2}) and the reverse-time
+
+// One reverse step of the variable-density adjoint in one launch.  With lam
+// = lambda^k, the cotangent of u^k (k_rho_adj and k_rho_inject done): on the
+// undamped interior, where alpha = 1 at the cell and its neighbours, the
+// model gradient of the step u^k = F(u^{k-1}, u^{k-2}) and the reverse-time
 // rebuild of u^{k-2} over uc (u^k on entry); on the ring, its restore.
 //   dJ/dvp  += 2 C rho vp D lam,  D = M(u^{k-1})
 //   dJ/drho += C vp^2 D lam - (1/(2 rho^2)) sum_j (u_j - u) (mu - mu_j),
@@ -901,9 +903,16 @@ __global__ __launch_bounds__(256) void k_rho_inject(float* __restrict__ lam,
 //   u^{k-2} = 2 u^{k-1} - (u^k - w - K D)   (w at the shot's source cell)
 // gpart: [2][G][plane] (vp, then rho partials per group of spt shots).
 // recon: 0 none, 1 rebuild + ring from ring_slot, 2 zeros (u^0).
-__global__ __launch_bounds__(256) void k_rho_grad(
+// When out is given, the same thread also does k_rho_adj's work for step
+// k-1 at its cell: out = lambda^{k-1} from lam and lam1 = lambda^{k+1}.  Both
+// parts read lambda^k at the cell and its four neighbours (complete since
+// the previous launch) and write disjoint buffers, so they share the face
+// averages of 1/rho and the lambda^k loads; each output keeps the operands
+// and order it has in k_rho_adj.
+__global__ __launch_bounds__(256) void k_rho_adj_grad(
     const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ buoy,
-    const float* __restrict__ lam, const float* __restrict__ um, float* __restrict__ uc,
+    const float* __restrict__ bw, const float* __restrict__ lam, const float* __restrict__ lam1,
+    float* __restrict__ out, const float* __restrict__ um, float* __restrict__ uc,
     float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
     const float* __restrict__ ring_slot, int64_t ring_len, int recon, int B, int spt, int G,
     HabcCtx hc, float C, TileMap tm) {
@@ -915,61 +924,110 @@ __global__ __launch_bounds__(256) void k_rho_grad(
   if (x >= g.nx || z >= g.nz) return;
   const int i = z * g.nx + x;
   const int b0 = grp * spt, nb = min(spt, B - b0);
-  if (!seis::interior(z, x, g)) {
+  const bool in = seis::interior(z, x, g);
+  if (!in) {  // off the interior: the ring restore of u^{k-2}
     const int64_t j = recon ? seis::ring_index(z, x, g) : -1;
     if (j >= 0)
       for (int s = 0; s < nb; ++s) {
         const int bb = b0 + s;
         uc[(int64_t)bb * g.plane + i] = recon == 1 ? ring_slot[bb * ring_len + j] : 0.0f;
       }
-    return;
+    if (!out) return;
   }
-  const float c = vp[i];
-  const float r = rho[i];
+  const bool doA = out != nullptr;
   const float bc = buoyancy(rho, buoy, z, x, g);
   const float bxp = (bc + buoyancy(rho, buoy, z, x + 1, g)) / 2.0f;
   const float bxn = (buoyancy(rho, buoy, z, x - 1, g) + bc) / 2.0f;
   const float bzp = (bc + buoyancy(rho, buoy, z + 1, x, g)) / 2.0f;
   const float bzn = (buoyancy(rho, buoy, z - 1, x, g) + bc) / 2.0f;
+  // k_rho_adj's set-up: K*alpha at the cell and its neighbours, one-way terms
+  const bool deep = !near_edge(z, x, g, kW + 3);
+  auto ka = [&](int zz, int xx) {
+    if (zz < 0 || zz >= g.nz || xx < 0 || xx >= g.nx) return 0.0f;
+    const int j = zz * g.nx + xx;
+    const float cn = vp[j];
+    const float al = deep ? 1.0f : habc_weights(zz, xx, hc, bw[j]).alpha;
+    return ((C * rho[j]) * (cn * cn)) * al;
+  };
+  float a0 = 0.0f, axp = 0.0f, axn = 0.0f, azp = 0.0f, azn = 0.0f, al0 = 1.0f;
+  int on[16];
+  float oc1[16], oc2[16];
+  int no = 0;
+  if (doA) {
+    a0 = ka(z, x); axp = ka(z, x + 1); axn = ka(z, x - 1); azp = ka(z + 1, x);
+    azn = ka(z - 1, x);
+    al0 = deep ? 1.0f : habc_weights(z, x, hc, bw[i]).alpha;
+    no = deep ? 0 : oneway_terms(z, x, i, vp, bw, hc, on, oc1, oc2);
+  }
+  // the gradient's set-up: K (no blend weight) at the cell and its neighbours
   auto kk = [&](int zz, int xx) {
     if (zz < 0 || zz >= g.nz || xx < 0 || xx >= g.nx) return 0.0f;
     const int j = zz * g.nx + xx;
     const float cn = vp[j];
     return (C * rho[j]) * (cn * cn);
   };
-  const float k0 = (C * r) * (c * c);
-  const float kxp = kk(z, x + 1), kxn = kk(z, x - 1), kzp = kk(z + 1, x), kzn = kk(z - 1, x);
+  const float c = vp[i];
+  const float r = rho[i];
+  float k0 = 0.0f, kxp = 0.0f, kxn = 0.0f, kzp = 0.0f, kzn = 0.0f;
+  if (in) {
+    k0 = (C * r) * (c * c);
+    kxp = kk(z, x + 1); kxn = kk(z, x - 1); kzp = kk(z + 1, x); kzn = kk(z - 1, x);
+  }
   const float w = wn ? *wn : 0.0f;
   float gv = 0.0f, gb = 0.0f;
   for (int s = 0; s < nb; ++s) {
     const int bb = b0 + s;
     const int64_t off = (int64_t)bb * g.plane;
-    const float* U = um + off;
     const float* Lm = lam + off;
-    const float u0 = U[i];
-    const float pxp = (x + 1 < g.nx ? U[i + 1] : 0.0f) - u0;
-    const float pxn = u0 - (x > 0 ? U[i - 1] : 0.0f);
-    const float pzp = (z + 1 < g.nz ? U[i + g.nx] : 0.0f) - u0;
-    const float pzn = u0 - (z > 0 ? U[i - g.nx] : 0.0f);
-    const float D = ((bxp * pxp - bxn * pxn) + bzp * pzp) - bzn * pzn;
     const float l0 = Lm[i];
-    gv += D * l0;
-    const float m0 = k0 * l0;
-    const float mxp = x + 1 < g.nx ? kxp * Lm[i + 1] : 0.0f;
-    const float mxn = x > 0 ? kxn * Lm[i - 1] : 0.0f;
-    const float mzp = z + 1 < g.nz ? kzp * Lm[i + g.nx] : 0.0f;
-    const float mzn = z > 0 ? kzn * Lm[i - g.nx] : 0.0f;
-    gb += ((pxp * (m0 - mxp) - pxn * (m0 - mxn)) + pzp * (m0 - mzp)) - pzn * (m0 - mzn);
-    if (recon == 1) {
-      float un = uc[off + i];
-      if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
-      uc[off + i] = 2.0f * u0 - (un - k0 * D);
-    } else if (recon == 2) {
-      uc[off + i] = 0.0f;
+    const float lxp = x + 1 < g.nx ? Lm[i + 1] : 0.0f;
+    const float lxn = x > 0 ? Lm[i - 1] : 0.0f;
+    const float lzp = z + 1 < g.nz ? Lm[i + g.nx] : 0.0f;
+    const float lzn = z > 0 ? Lm[i - g.nx] : 0.0f;
+    if (in) {
+      const float* U = um + off;
+      const float u0 = U[i];
+      const float pxp = (x + 1 < g.nx ? U[i + 1] : 0.0f) - u0;
+      const float pxn = u0 - (x > 0 ? U[i - 1] : 0.0f);
+      const float pzp = (z + 1 < g.nz ? U[i + g.nx] : 0.0f) - u0;
+      const float pzn = u0 - (z > 0 ? U[i - g.nx] : 0.0f);
+      const float D = ((bxp * pxp - bxn * pxn) + bzp * pzp) - bzn * pzn;
+      gv += D * l0;
+      const float m0 = k0 * l0;
+      const float mxp = x + 1 < g.nx ? kxp * lxp : 0.0f;
+      const float mxn = x > 0 ? kxn * lxn : 0.0f;
+      const float mzp = z + 1 < g.nz ? kzp * lzp : 0.0f;
+      const float mzn = z > 0 ? kzn * lzn : 0.0f;
+      gb += ((pxp * (m0 - mxp) - pxn * (m0 - mxn)) + pzp * (m0 - mzp)) - pzn * (m0 - mzn);
+      if (recon == 1) {
+        float un = uc[off + i];
+        if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
+        uc[off + i] = 2.0f * u0 - (un - k0 * D);
+      } else if (recon == 2) {
+        uc[off + i] = 0.0f;
+      }
+    }
+    if (doA) {
+      const float* Y = lam + off;
+      const float m0 = a0 * l0;
+      const float mxp = x + 1 < g.nx ? axp * lxp : 0.0f;
+      const float mxn = x > 0 ? axn * lxn : 0.0f;
+      const float mzp = z + 1 < g.nz ? azp * lzp : 0.0f;
+      const float mzn = z > 0 ? azn * lzn : 0.0f;
+      const float Mm = ((bxp * (mxp - m0) + bxn * (mxn - m0)) + bzp * (mzp - m0)) +
+                       bzn * (mzn - m0);
+      float acc = (2.0f * al0) * l0 + Mm;
+      for (int o = 0; o < no; ++o) acc += oc1[o] * Y[on[o]];
+      const float* Y2 = lam1 + off;
+      acc += -al0 * Y2[i];
+      for (int o = 0; o < no; ++o) acc += oc2[o] * Y2[on[o]];
+      out[off + i] = acc;
     }
   }
-  gpart[(int64_t)grp * g.plane + i] += ((2.0f * C) * r * c) * gv;
-  gpart[((int64_t)G + grp) * g.plane + i] += (C * (c * c)) * gv - (0.5f * bc * bc) * gb;
+  if (in) {
+    gpart[(int64_t)grp * g.plane + i] += ((2.0f * C) * r * c) * gv;
+    gpart[((int64_t)G + grp) * g.plane + i] += (C * (c * c)) * gv - (0.5f * bc * bc) * gb;
+  }
 }
 
 // The fused adjoint step in one launch of two block ranges.  One covers the
@@ -1631,19 +1689,22 @@ int seis_rho_habc_backward(const float* vp, const float* rho, const float* b,
   float* L1 = l_a;  // lambda^{k+1}
   float* L2 = l_b;  // lambda^{k+2}
   float* L0 = l_c;  // lambda^k
+  // lambda^nt (from zero cotangents of the later states), then per reverse
+  // step k one launch: the gradient of step k and lambda^{k-1}, written into
+  // the buffer of lambda^{k+2}
+  k_rho_adj<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, buoy, b, L1, L2, L0, (int)B, S, hc, C,
+                                                 tm);
+  SEIS_CHECK_LAUNCH();
   for (int64_t kk = nt; kk >= 1; --kk) {
     const int recon = kk >= 3 ? 1 : (kk == 2 ? 2 : 0);
-    k_rho_adj<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, buoy, b, L1, L2, L0, (int)B, S, hc,
-                                                   C, tm);
-    SEIS_CHECK_LAUNCH();
     if (R > 0) {
       k_rho_inject<<<nblk(R), 256, 0, s>>>(L0, rec_flat, rec_grad + (kk - 1) * R, R);
       SEIS_CHECK_LAUNCH();
     }
     const float* slot = recon == 1 ? ring + (kk - 3) * B * ring_len : nullptr;
-    k_rho_grad<<<dim3(tm.per * kXcds), 256, 0, s>>>(vp, rho, buoy, L0, X, Y, grad_part, src_zx,
-                                                    wavelet + (kk - 1), slot, ring_len, recon,
-                                                    (int)B, S, G, hc, C, tm);
+    k_rho_adj_grad<<<dim3(tm.per * kXcds), 256, 0, s>>>(
+        vp, rho, buoy, b, L0, L1, kk > 1 ? L2 : nullptr, X, Y, grad_part, src_zx,
+        wavelet + (kk - 1), slot, ring_len, recon, (int)B, S, G, hc, C, tm);
     SEIS_CHECK_LAUNCH();
     const int64_t m = kk - 2;
     if (snaps && m >= snap_every && m % snap_every == 0 && m <= nt - 2) {
