@@ -355,8 +355,9 @@ def run_multiscale(p, scales, epochs, lr=5.0, optimizer="adam", scale_decay=0.85
 def problem_from_config(cfg, device="cuda", shots=None, mute=0):
     """A Problem from the reference's configuration keys (template.yml;
     seistorch_dist.py:90-160): ``equation`` ("acoustic" with a PML boundary,
-    "acoustic_habc", or "elastic": an ElasticProblem with vp, vs and rho,
-    geom.invlist, source_type and receiver_type), geom.truePath / initPath ``vp`` (.npy (nz, nx)
+    "acoustic_habc", "elastic": an ElasticProblem with vp, vs and rho,
+    geom.invlist, source_type and receiver_type, or "acoustic_rho_habc": a
+    RhoProblem with vp and rho), geom.truePath / initPath ``vp`` (.npy (nz, nx)
     models of the unpadded grid), geom.sources / receivers (.pkl),
     geom.obsPath (.npy records, one (nt, nr, 1) array per shot), the Ricker
     wavelet from geom.fm / dt / nt / wavelet_delay / wavelet_inverse or a
@@ -379,7 +380,7 @@ def problem_from_config(cfg, device="cuda", shots=None, mute=0):
         raise NotImplementedError("multiples (free surface without top boundary) are not "
                                   "supported by problem_from_config")
     equation = cfg.get("equation", "acoustic")
-    if equation not in ("acoustic", "acoustic_habc", "elastic"):
+    if equation not in ("acoustic", "acoustic_habc", "elastic", "acoustic_rho_habc"):
         raise ValueError(f"unsupported equation {equation!r}")
     dt, h, nt = float(geom["dt"]), float(geom["h"]), int(geom["nt"])
 
@@ -406,7 +407,7 @@ def problem_from_config(cfg, device="cuda", shots=None, mute=0):
     rec_list = sio.read_pkl(geom["receivers"])
     shots = list(range(len(src_list))) if shots is None else list(shots)
     src, rec = sio.setup_acquisition(src_list, rec_list, width, False, shots)
-    if equation == "acoustic_habc":
+    if equation in ("acoustic_habc", "acoustic_rho_habc"):
         d = generate_habc_coefficients_2d((nz, nx), width)
     else:
         d = generate_pml_coefficients_2d((nz, nx), width)
@@ -429,6 +430,20 @@ def problem_from_config(cfg, device="cuda", shots=None, mute=0):
                            invlist=tuple(k for k in ("vp", "vs", "rho") if inv.get(k)),
                            filter_ord=filter_ord, mute=mute)
         nch = len(p.rec_type)
+    elif equation == "acoustic_rho_habc":
+        true, init = {}, {}
+        for k in ("vp", "rho"):
+            init[k] = model("initPath", k)
+            true[k] = model("truePath", k)
+            if init[k] is None:
+                raise ValueError(f"geom.initPath.{k} is required for acoustic_rho_habc")
+            if true[k] is None:
+                true[k] = init[k].clone()
+        inv = geom.get("invlist") or {"vp": True, "rho": True}
+        p = RhoProblem(true, init, *common,
+                       invlist=tuple(k for k in ("vp", "rho") if inv.get(k)),
+                       filter_ord=filter_ord, mute=mute)
+        nch = 1
     else:
         p = Problem(vp_true, vp_init, *common, filter_ord=filter_ord, equation=equation,
                     mute=mute)
@@ -452,8 +467,8 @@ def run_from_config(cfg, device="cuda", shots=None, optimizer="adam", grad_smoot
     {"vp": lr}), scale_decay, lr_decay; the command-line switches
     --grad-smooth (training.smooth), --grad-cut (geom.seabed, an .npy mask of
     the unpadded grid) and --source-illumination.
-    Returns (Problem, vp, History); for the elastic equation (ElasticProblem,
-    {name: padded model}, History)."""
+    Returns (Problem, vp, History); for the elastic and variable-density
+    equations (ElasticProblem / RhoProblem, {name: padded model}, History)."""
     from . import io as sio
     p = problem_from_config(cfg, device, shots, mute=mute)
     if p.obs is None:
@@ -649,6 +664,25 @@ class ElasticProblem:
     def rmse(self, m, name="vp"):
         return float(torch.sqrt(torch.mean((self.interior(m[name])
                                             - self.interior(self.models_true[name])) ** 2)))
+
+
+@dataclass
+class RhoProblem(ElasticProblem):
+    """Variable-density acoustic FWI (equation: acoustic_rho_habc,
+    eqconfigure.py:11, acoustic_rho_habc.py:31-121): models_true /
+    models_init hold padded (nz, nx) 'vp' and 'rho', d is the HABC weight
+    field, records (nt, R) of the pressure; invlist names the inverted
+    parameters (geom.invlist, default both).  The gradients come from the
+    boundary-saving adjoint of seis_rho_habc_backward; run_multiscale_elastic
+    drives the updates (one learning rate per parameter, ELASTIC_BOUNDS)."""
+    src_type: tuple = ("p",)
+    rec_type: tuple = ("p",)
+    invlist: tuple = ("vp", "rho")
+
+    def _propagate(self, m, illum=None):
+        from .equations2d import acoustic_rho_habc as ar
+        return ar.propagate(m["vp"], m["rho"], self.d, self.wavelet, self.src, self.rec, self.B,
+                            self.dt, self.h, illum=illum)
 
 
 def run_multiscale_elastic(p, scales, epochs, lr, optimizer="adam", scale_decay=0.85,

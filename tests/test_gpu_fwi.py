@@ -238,3 +238,56 @@ def test_run_from_config_elastic(tmp_path):
     # above the seabed (6 rows) vp and vs keep their starting values
     for k in ("vp", "vs"):
         assert torch.equal(p.interior(models[k])[:6], p.interior(p.models_init[k])[:6])
+
+
+def test_run_from_config_variable_density(tmp_path):
+    """equation: acoustic_rho_habc from files: vp and rho models, geometry,
+    observed records; both inverted through the variable-density adjoint
+    (RhoProblem), each with its own learning rate; the misfit falls and the
+    rows above the seabed keep their starting values."""
+    import numpy as np
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from seistorch_amd import fwi, synthetic
+    from seistorch_amd import io as sio
+    nz_i, nx_i, nt, dt, h = 60, 140, 450, 1e-3, 10.0
+    vp, _ = synthetic.anomaly_model(nz_i, nx_i, vmin=1800.0, vmax=3000.0, seed=9, smooth=6.0)
+    zz, xx = np.mgrid[0:nz_i, 0:nx_i]
+    rho = 1800.0 + 4.0 * zz + 300.0 * (((zz - 35) / 8.0) ** 2 + ((xx - 70) / 15.0) ** 2 < 1.0)
+    paths = {}
+    for k, a in (("vp", vp.astype(np.float64)), ("rho", rho.astype(np.float64))):
+        np.save(tmp_path / f"true_{k}.npy", a)
+        np.save(tmp_path / f"init_{k}.npy", gaussian_filter(a, 5.0, mode="nearest"))
+        paths[k] = (str(tmp_path / f"true_{k}.npy"), str(tmp_path / f"init_{k}.npy"))
+    sources = [[15 + 55 * i, 1] for i in range(3)]
+    receivers = [[list(range(nx_i)), [2] * nx_i] for _ in range(3)]
+    sio.write_pkl(tmp_path / "s.pkl", sources)
+    sio.write_pkl(tmp_path / "r.pkl", receivers)
+    np.save(tmp_path / "seabed.npy", (np.arange(nz_i)[:, None] >= 6) * np.ones((1, nx_i)))
+    cfg = {"equation": "acoustic_rho_habc",
+           "geom": {"truePath": {k: v[0] for k, v in paths.items()},
+                    "initPath": {k: v[1] for k, v in paths.items()},
+                    "sources": str(tmp_path / "s.pkl"), "receivers": str(tmp_path / "r.pkl"),
+                    "seabed": str(tmp_path / "seabed.npy"), "obsPath": str(tmp_path / "obs.npy"),
+                    "wavelet": None, "fm": 15.0, "dt": dt, "nt": nt, "h": h,
+                    "wavelet_delay": 80, "wavelet_inverse": False, "multiple": False,
+                    "boundary": {"type": "habc", "width": 50}, "multiscale": ["all"],
+                    "invlist": {"vp": True, "rho": True}},
+           "training": {"N_epochs": 4, "lr": {"vp": 10.0, "rho": 10.0},
+                        "smooth": {"counts": 1, "sigma": {"z": 1.0, "x": 1.0},
+                                   "radius": {"z": 2, "x": 2}}}}
+    p = fwi.problem_from_config({**cfg, "geom": {**cfg["geom"], "obsPath": None}})
+    assert isinstance(p, fwi.RhoProblem) and p.invlist == ("vp", "rho")
+    obs = p.records(p.models_true).cpu().numpy()
+    store = sio.SeisRecord(cfg)
+    store.setup("forward")
+    for i, shot in enumerate(sio.split_records(obs, [len(r[0]) for r in receivers])):
+        store[i] = shot
+    store.write()
+    p, models, hist = fwi.run_from_config(cfg, grad_smooth=True, grad_cut=True,
+                                          source_illumination=True, log=print)
+    print("variable-density FWI misfit", [f"{m:.3e}" for m in hist.misfit], flush=True)
+    assert hist.misfit[-1] < hist.misfit[0]
+    for k in ("vp", "rho"):
+        assert not torch.equal(models[k], p.models_init[k])
+        assert torch.equal(p.interior(models[k])[:6], p.interior(p.models_init[k])[:6])
