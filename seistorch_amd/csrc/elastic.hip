@@ -689,8 +689,11 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
 
 constexpr int kElAdjRZ = 1;  // rows per thread of the fused reverse step
 
+// At least 4 waves per SIMD: the 4-cell-lane instances fit 128 VGPRs with a
+// 12-byte spill (135 unconstrained, 3 waves): 952 -> 924 us per step at S = 8
+// on the 700x1900, 32-shot grid (981 -> 986 at S = 4).
 template <int S, int VX>
-__global__ __launch_bounds__(256) void k_el_adj_rz(const float* __restrict__ cf, float dti,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_el_adj_rz(const float* __restrict__ cf, float dti,
                                                    CFields Xi, Fields Xo, CFields Fi, Fields Fo,
                                                    float* __restrict__ gco, int recon, int B,
                                                    Grid g, TileMap tm) {
