@@ -909,19 +909,14 @@ __global__ __launch_bounds__(256) void k_rho_inject(float* __restrict__ lam,
 // the previous launch) and write disjoint buffers, so they share the face
 // averages of 1/rho and the lambda^k loads; each output keeps the operands
 // and order it has in k_rho_adj.
-__global__ __launch_bounds__(256) void k_rho_adj_grad(
-    const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ buoy,
-    const float* __restrict__ bw, const float* __restrict__ lam, const float* __restrict__ lam1,
-    float* __restrict__ out, const float* __restrict__ um, float* __restrict__ uc,
-    float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
-    const float* __restrict__ ring_slot, int64_t ring_len, int recon, int B, int spt, int G,
-    HabcCtx hc, float C, TileMap tm) {
-  int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;
+__device__ __forceinline__ void rho_adj_grad_cell(
+    int z, int x, int grp, const float* __restrict__ vp, const float* __restrict__ rho,
+    const float* __restrict__ buoy, const float* __restrict__ bw, const float* __restrict__ lam,
+    const float* __restrict__ lam1, float* __restrict__ out, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, const float* __restrict__ ring_slot, int64_t ring_len,
+    int recon, int B, int spt, int G, const HabcCtx& hc, float C) {
   const Grid& g = hc.g;
-  const int x = tx * kLanes + (int)(threadIdx.x % kLanes);
-  const int z = tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
-  if (x >= g.nx || z >= g.nz) return;
   const int i = z * g.nx + x;
   const int b0 = grp * spt, nb = min(spt, B - b0);
   const bool in = seis::interior(z, x, g);
@@ -1028,6 +1023,161 @@ __global__ __launch_bounds__(256) void k_rho_adj_grad(
     gpart[(int64_t)grp * g.plane + i] += ((2.0f * C) * r * c) * gv;
     gpart[((int64_t)G + grp) * g.plane + i] += (C * (c * c)) * gv - (0.5f * bc * bc) * gb;
   }
+}
+
+// Deep tile of the variable-density reverse step (cells at least kW+3 from
+// every edge: interior, no strip term, blend weight 1 at the cell and its
+// neighbours): a lane takes VX consecutive cells of one row, with the face
+// averages and K of its cells set up once and every field accessed as row
+// vectors; each cell's values use the operands and order of
+// rho_adj_grad_cell (bit-identical).
+template <int VX>
+__device__ __forceinline__ void rho_adj_grad_deep(
+    int z, int x0, int grp, const float* __restrict__ vp, const float* __restrict__ rho,
+    const float* __restrict__ buoy, const float* __restrict__ lam,
+    const float* __restrict__ lam1, float* __restrict__ out, const float* __restrict__ um,
+    float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
+    const float* __restrict__ wn, int recon, int B, int spt, int G, const HabcCtx& hc,
+    float C) {
+  const Grid& g = hc.g;
+  const int i0 = z * g.nx + x0;
+  const bool doA = out != nullptr;
+  float bcv[VX], bxp[VX], bxn[VX], bzp[VX], bzn[VX], k0[VX], kxp[VX], kxn[VX], kzp[VX],
+      kzn[VX], c[VX], r[VX];
+  {
+    // 1/rho on rows z-1, z, z+1 and at the vector's two ends; K = C rho vp^2
+    float bm[VX], b0[VX], bp[VX], vm[VX], v0[VX], vpp[VX], rm[VX], rpp[VX];
+    vload<VX>(buoy + i0 - g.nx, bm);
+    vload<VX>(buoy + i0, b0);
+    vload<VX>(buoy + i0 + g.nx, bp);
+    vload<VX>(vp + i0 - g.nx, vm);
+    vload<VX>(vp + i0, v0);
+    vload<VX>(vp + i0 + g.nx, vpp);
+    vload<VX>(rho + i0 - g.nx, rm);
+    vload<VX>(rho + i0, r);
+    vload<VX>(rho + i0 + g.nx, rpp);
+    const float bl = buoy[i0 - 1], br = buoy[i0 + VX];
+    const float vl = vp[i0 - 1], vr = vp[i0 + VX];
+    const float rl = rho[i0 - 1], rr = rho[i0 + VX];
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      c[e] = v0[e];
+      bcv[e] = b0[e];
+      const float bxr = e == VX - 1 ? br : b0[e + 1];
+      const float bxl = e == 0 ? bl : b0[e - 1];
+      bxp[e] = (b0[e] + bxr) / 2.0f;
+      bxn[e] = (bxl + b0[e]) / 2.0f;
+      bzp[e] = (b0[e] + bp[e]) / 2.0f;
+      bzn[e] = (bm[e] + b0[e]) / 2.0f;
+      k0[e] = (C * r[e]) * (c[e] * c[e]);
+      const float cxr = e == VX - 1 ? vr : v0[e + 1], rxr = e == VX - 1 ? rr : r[e + 1];
+      const float cxl = e == 0 ? vl : v0[e - 1], rxl = e == 0 ? rl : r[e - 1];
+      kxp[e] = (C * rxr) * (cxr * cxr);
+      kxn[e] = (C * rxl) * (cxl * cxl);
+      kzp[e] = (C * rpp[e]) * (vpp[e] * vpp[e]);
+      kzn[e] = (C * rm[e]) * (vm[e] * vm[e]);
+    }
+  }
+  const float w = wn ? *wn : 0.0f;
+  float gv[VX], gb[VX];
+  vzero<VX>(gv);
+  vzero<VX>(gb);
+  const int b0 = grp * spt, nb = min(spt, B - b0);
+  for (int s = 0; s < nb; ++s) {
+    const int bb = b0 + s;
+    const int64_t off = (int64_t)bb * g.plane;
+    const float* Lm = lam + off;
+    const float* U = um + off;
+    float l0[VX], lzm[VX], lzp[VX], u0[VX], uzm[VX], uzp[VX], un[VX];
+    vload<VX>(Lm + i0, l0);
+    vload<VX>(Lm + i0 - g.nx, lzm);
+    vload<VX>(Lm + i0 + g.nx, lzp);
+    vload<VX>(U + i0, u0);
+    vload<VX>(U + i0 - g.nx, uzm);
+    vload<VX>(U + i0 + g.nx, uzp);
+    const float ll = Lm[i0 - 1], lr = Lm[i0 + VX];
+    const float ul = U[i0 - 1], ur = U[i0 + VX];
+    if (recon == 1) vload<VX>(uc + off + i0, un);
+    const int sz = src[2 * bb], sx = src[2 * bb + 1];
+    float y2[VX], acc[VX];
+    if (doA) vload<VX>(lam1 + off + i0, y2);
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+      const float lxp = e == VX - 1 ? lr : l0[e + 1];
+      const float lxn = e == 0 ? ll : l0[e - 1];
+      const float uxp = e == VX - 1 ? ur : u0[e + 1];
+      const float uxn = e == 0 ? ul : u0[e - 1];
+      const float pxp = uxp - u0[e];
+      const float pxn = u0[e] - uxn;
+      const float pzp = uzp[e] - u0[e];
+      const float pzn = u0[e] - uzm[e];
+      const float D = ((bxp[e] * pxp - bxn[e] * pxn) + bzp[e] * pzp) - bzn[e] * pzn;
+      gv[e] += D * l0[e];
+      const float m0 = k0[e] * l0[e];
+      const float mxp = kxp[e] * lxp;
+      const float mxn = kxn[e] * lxn;
+      const float mzp = kzp[e] * lzp[e];
+      const float mzn = kzn[e] * lzm[e];
+      gb[e] += ((pxp * (m0 - mxp) - pxn * (m0 - mxn)) + pzp * (m0 - mzp)) - pzn * (m0 - mzn);
+      if (recon == 1) {
+        float v = un[e];
+        if (z == sz && x0 + e == sx) v -= w;
+        un[e] = 2.0f * u0[e] - (v - k0[e] * D);
+      } else if (recon == 2) {
+        un[e] = 0.0f;
+      }
+      if (doA) {
+        // k_rho_adj with blend weight 1: K*1 = K, 2*1, -1
+        const float Mm = ((bxp[e] * (mxp - m0) + bxn[e] * (mxn - m0)) + bzp[e] * (mzp - m0)) +
+                         bzn[e] * (mzn - m0);
+        float a = (2.0f * 1.0f) * l0[e] + Mm;
+        a += -1.0f * y2[e];
+        acc[e] = a;
+      }
+    }
+    if (recon != 0) vstore<VX>(uc + off + i0, un);
+    if (doA) vstore<VX>(out + off + i0, acc);
+  }
+  float* gp = gpart + (int64_t)grp * g.plane + i0;
+  float* gq = gpart + ((int64_t)G + grp) * g.plane + i0;
+  float pv[VX], pq[VX];
+  vload<VX>(gp, pv);
+  vload<VX>(gq, pq);
+#pragma unroll
+  for (int e = 0; e < VX; ++e) {
+    pv[e] += ((2.0f * C) * r[e] * c[e]) * gv[e];
+    pq[e] += (C * (c[e] * c[e])) * gv[e] - (0.5f * bcv[e] * bcv[e]) * gb[e];
+  }
+  vstore<VX>(gp, pv);
+  vstore<VX>(gq, pq);
+}
+
+// One reverse step of the variable-density adjoint in one launch of two
+// block ranges: the frame (the cells outside Deep) one cell per thread with
+// rho_adj_grad_cell, then Deep's lane vectors with rho_adj_grad_deep.
+template <int VX>
+__global__ __launch_bounds__(256) void k_rho_adj_grad(
+    const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ buoy,
+    const float* __restrict__ bw, const float* __restrict__ lam, const float* __restrict__ lam1,
+    float* __restrict__ out, const float* __restrict__ um, float* __restrict__ uc,
+    float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
+    const float* __restrict__ ring_slot, int64_t ring_len, int recon, int B, int spt, int G,
+    HabcCtx hc, float C, Deep dp, Frame f, TileMap tm) {
+  const int nf = f.per * kXcds;
+  if ((int)blockIdx.x < nf) {
+    int z, x, grp;
+    if (frame_cell(f, (int)blockIdx.x, z, x, grp))
+      rho_adj_grad_cell(z, x, grp, vp, rho, buoy, bw, lam, lam1, out, um, uc, gpart, src, wn,
+                        ring_slot, ring_len, recon, B, spt, G, hc, C);
+    return;
+  }
+  int tx, tzi, grp;
+  if (!tile_of_w(tm, (int)blockIdx.x - nf, tx, tzi, grp)) return;
+  const int x0 = dp.x0 + (tx * kLanes + (int)(threadIdx.x % kLanes)) * VX;
+  const int z = dp.z0 + tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes);
+  if (x0 >= dp.x1 || z >= dp.z1) return;
+  rho_adj_grad_deep<VX>(z, x0, grp, vp, rho, buoy, lam, lam1, out, um, uc, gpart, src, wn,
+                        recon, B, spt, G, hc, C);
 }
 
 // The fused adjoint step in one launch of two block ranges.  One covers the
@@ -1673,6 +1823,16 @@ int seis_rho_habc_backward(const float* vp, const float* rho, const float* b,
   const int S = shots_per_group;
   const int G = (int)((B + S - 1) / S);
   const TileMap tm = make_tiles(g, 1, 1, G);
+  // Deep (kW+3 cells from every edge, column bounds multiples of VX) runs
+  // lane vectors of VX cells, the frame around it one cell per thread
+  const int VX = habc_vx(nx, {vp, rho, buoy, u_a, u_b, l_a, l_b, l_c, grad_part});
+  constexpr int m = kW + 3;
+  Deep dp{m, (int)nz - m, (m + VX - 1) / VX * VX, ((int)nx - m) / VX * VX};
+  if (dp.x1 <= dp.x0 || dp.z1 <= dp.z0) dp = Deep{0, 0, 0, 0};
+  const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
+  TileMap tmd = make_tiles(gd, VX, 1, G);
+  if (gd.plane == 0) tmd.per = 0;
+  const Frame fr = make_frame(g, dp, G);
   const float C = (float)(dt * dt * std::pow(h, -2.0));
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -1702,9 +1862,24 @@ int seis_rho_habc_backward(const float* vp, const float* rho, const float* b,
       SEIS_CHECK_LAUNCH();
     }
     const float* slot = recon == 1 ? ring + (kk - 3) * B * ring_len : nullptr;
-    k_rho_adj_grad<<<dim3(tm.per * kXcds), 256, 0, s>>>(
-        vp, rho, buoy, b, L0, L1, kk > 1 ? L2 : nullptr, X, Y, grad_part, src_zx,
-        wavelet + (kk - 1), slot, ring_len, recon, (int)B, S, G, hc, C, tm);
+    const dim3 grid((fr.per + tmd.per) * kXcds);
+    float* next = kk > 1 ? L2 : nullptr;
+    switch (VX) {
+      case 4:
+        k_rho_adj_grad<4><<<grid, 256, 0, s>>>(vp, rho, buoy, b, L0, L1, next, X, Y, grad_part,
+                                                src_zx, wavelet + (kk - 1), slot, ring_len, recon,
+                                                (int)B, S, G, hc, C, dp, fr, tmd);
+        break;
+      case 2:
+        k_rho_adj_grad<2><<<grid, 256, 0, s>>>(vp, rho, buoy, b, L0, L1, next, X, Y, grad_part,
+                                                src_zx, wavelet + (kk - 1), slot, ring_len, recon,
+                                                (int)B, S, G, hc, C, dp, fr, tmd);
+        break;
+      default:
+        k_rho_adj_grad<1><<<grid, 256, 0, s>>>(vp, rho, buoy, b, L0, L1, next, X, Y, grad_part,
+                                                src_zx, wavelet + (kk - 1), slot, ring_len, recon,
+                                                (int)B, S, G, hc, C, dp, fr, tmd);
+    }
     SEIS_CHECK_LAUNCH();
     const int64_t m = kk - 2;
     if (snaps && m >= snap_every && m % snap_every == 0 && m <= nt - 2) {
