@@ -555,11 +555,11 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
         o3[e] = bm[e] * U2[j][e];
         o4[e] = bm[e] * U3[r][e];
       }
-      vstore<VX>(Xo.f[0] + off + i, o0);
-      vstore<VX>(Xo.f[1] + off + i, o1);
-      vstore<VX>(Xo.f[2] + off + i, o2);
-      vstore<VX>(Xo.f[3] + off + i, o3);
-      vstore<VX>(Xo.f[4] + off + i, o4);
+      vstore_nt<VX>(Xo.f[0] + off + i, o0);
+      vstore_nt<VX>(Xo.f[1] + off + i, o1);
+      vstore_nt<VX>(Xo.f[2] + off + i, o2);
+      vstore_nt<VX>(Xo.f[3] + off + i, o3);
+      vstore_nt<VX>(Xo.f[4] + off + i, o4);
     }
     // state n+1 stresses on rows zb-1 .. zb+RZ; the stage-V sums
     // sx = Dx+ txx + Dz+ txz (rows zb-1 .. zb+RZ-1) and sz = Dx- txz + Dz- tzz
@@ -663,35 +663,45 @@ __device__ __forceinline__ void el_adj_body(const float* __restrict__ cf, float 
           txz[e] = txz[e] - sgn * mc[e] * (vz_x + vx_z);
         }
       }
-      vstore<VX>(Fo.f[0] + off + i, VXN[j]);
-      vstore<VX>(Fo.f[1] + off + i, VZN[r]);
-      vstore<VX>(Fo.f[2] + off + i, txx);
-      vstore<VX>(Fo.f[3] + off + i, tzz);
-      vstore<VX>(Fo.f[4] + off + i, txz);
+      vstore_nt<VX>(Fo.f[0] + off + i, VXN[j]);
+      vstore_nt<VX>(Fo.f[1] + off + i, VZN[r]);
+      vstore_nt<VX>(Fo.f[2] + off + i, txx);
+      vstore_nt<VX>(Fo.f[3] + off + i, tzz);
+      vstore_nt<VX>(Fo.f[4] + off + i, txz);
     }
   }
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int j = r + 1;
     if (!inner || !rok(j)) continue;
-    const int z = zb + r;
+    // whole lane vectors: G0..G3 hold exact zeros outside the interior, and
+    // adding them leaves those cells of this group's planes as they are
+    float* Gp = gco + (int64_t)grp * 4 * P + idx(j);
+    float t0[VX], t1[VX], t2[VX], t3[VX];
+    vload<VX>(Gp, t0);
+    vload<VX>(Gp + P, t1);
+    vload<VX>(Gp + 2 * P, t2);
+    vload<VX>(Gp + 3 * P, t3);
 #pragma unroll
     for (int e = 0; e < VX; ++e) {
-      if (!seis::interior(z, x0 + e, g)) continue;
-      float* Gp = gco + (int64_t)grp * 4 * P + idx(j) + e;
-      Gp[0] += G0[r][e];
-      Gp[P] += G1[r][e];
-      Gp[2 * P] += G2[r][e];
-      Gp[3 * P] += G3[r][e];
+      t0[e] += G0[r][e];
+      t1[e] += G1[r][e];
+      t2[e] += G2[r][e];
+      t3[e] += G3[r][e];
     }
+    vstore<VX>(Gp, t0);
+    vstore<VX>(Gp + P, t1);
+    vstore<VX>(Gp + 2 * P, t2);
+    vstore<VX>(Gp + 3 * P, t3);
   }
 }
 
 constexpr int kElAdjRZ = 1;  // rows per thread of the fused reverse step
 
-// At least 4 waves per SIMD: the 4-cell-lane instances fit 128 VGPRs with a
-// 12-byte spill (135 unconstrained, 3 waves): 952 -> 924 us per step at S = 8
-// on the 700x1900, 32-shot grid (981 -> 986 at S = 4).
+// At least 4 waves per SIMD: the 4-cell-lane instances fit 128 VGPRs (135
+// unconstrained, 3 waves; with the gradient partials added per element they
+// spilled 12 bytes): 952 -> 924 us per step at S = 8 on the 700x1900, 32-shot
+// grid (981 -> 986 at S = 4).
 template <int S, int VX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_el_adj_rz(const float* __restrict__ cf, float dti,
                                                    CFields Xi, Fields Xo, CFields Fi, Fields Fo,

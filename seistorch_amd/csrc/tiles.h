@@ -186,6 +186,24 @@ __device__ __forceinline__ void vstore(float* __restrict__ p, const float (&o)[V
   }
 }
 
+// A vector store with the nontemporal hint (global_store ... nt): for fields
+// that this launch writes once and only a later launch reads, so that they do
+// not displace the halo rows still being read from L2.
+template <int VX>
+__device__ __forceinline__ void vstore_nt(float* __restrict__ p, const float (&o)[VX]) {
+  if constexpr (VX == 4) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = {o[0], o[1], o[2], o[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<f4*>(p));
+  } else if constexpr (VX == 2) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 v = {o[0], o[1]};
+    __builtin_nontemporal_store(v, reinterpret_cast<f2*>(p));
+  } else {
+    __builtin_nontemporal_store(o[0], p);
+  }
+}
+
 template <int VX>
 __device__ __forceinline__ void vzero(float (&o)[VX]) {
 #pragma unroll
