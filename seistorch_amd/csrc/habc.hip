@@ -161,6 +161,10 @@ constexpr int kBandVX = 1;
 // frame threads' occupancy) follows the larger branch; two rows per thread
 // took 146 VGPRs against 91 and made the launch 20 % slower.
 constexpr int kHabcRZ = 1;
+// Shot groups per frame thread of the variable-density adjoint (its per-cell
+// set-up, up to 16 neighbours' blend weights and one-way terms, runs once
+// for them all).
+constexpr int kRhoFrameGroups = 64;
 
 // The cells outside Deep as up to four rectangles (top, bottom, left, right),
 // tiled 64 x 4 each (one cell per thread) and numbered consecutively, so the
@@ -915,18 +919,17 @@ __device__ __forceinline__ void rho_adj_grad_cell(
     const float* __restrict__ lam1, float* __restrict__ out, const float* __restrict__ um,
     float* __restrict__ uc, float* __restrict__ gpart, const int32_t* __restrict__ src,
     const float* __restrict__ wn, const float* __restrict__ ring_slot, int64_t ring_len,
-    int recon, int B, int spt, int G, const HabcCtx& hc, float C) {
+    int recon, int B, int spt, int G, const HabcCtx& hc, float C, int ngrp) {
   const Grid& g = hc.g;
   const int i = z * g.nx + x;
-  const int b0 = grp * spt, nb = min(spt, B - b0);
+  // shot groups grp .. grp+ngrp-1 (those below G), the set-up below once for all
+  const int g_end = min(G, grp + ngrp);
   const bool in = seis::interior(z, x, g);
   if (!in) {  // off the interior: the ring restore of u^{k-2}
     const int64_t j = recon ? seis::ring_index(z, x, g) : -1;
     if (j >= 0)
-      for (int s = 0; s < nb; ++s) {
-        const int bb = b0 + s;
+      for (int bb = grp * spt; bb < min(B, g_end * spt); ++bb)
         uc[(int64_t)bb * g.plane + i] = recon == 1 ? ring_slot[bb * ring_len + j] : 0.0f;
-      }
     if (!out) return;
   }
   const bool doA = out != nullptr;
@@ -969,59 +972,62 @@ __device__ __forceinline__ void rho_adj_grad_cell(
     kxp = kk(z, x + 1); kxn = kk(z, x - 1); kzp = kk(z + 1, x); kzn = kk(z - 1, x);
   }
   const float w = wn ? *wn : 0.0f;
-  float gv = 0.0f, gb = 0.0f;
-  for (int s = 0; s < nb; ++s) {
-    const int bb = b0 + s;
-    const int64_t off = (int64_t)bb * g.plane;
-    const float* Lm = lam + off;
-    const float l0 = Lm[i];
-    const float lxp = x + 1 < g.nx ? Lm[i + 1] : 0.0f;
-    const float lxn = x > 0 ? Lm[i - 1] : 0.0f;
-    const float lzp = z + 1 < g.nz ? Lm[i + g.nx] : 0.0f;
-    const float lzn = z > 0 ? Lm[i - g.nx] : 0.0f;
-    if (in) {
-      const float* U = um + off;
-      const float u0 = U[i];
-      const float pxp = (x + 1 < g.nx ? U[i + 1] : 0.0f) - u0;
-      const float pxn = u0 - (x > 0 ? U[i - 1] : 0.0f);
-      const float pzp = (z + 1 < g.nz ? U[i + g.nx] : 0.0f) - u0;
-      const float pzn = u0 - (z > 0 ? U[i - g.nx] : 0.0f);
-      const float D = ((bxp * pxp - bxn * pxn) + bzp * pzp) - bzn * pzn;
-      gv += D * l0;
-      const float m0 = k0 * l0;
-      const float mxp = x + 1 < g.nx ? kxp * lxp : 0.0f;
-      const float mxn = x > 0 ? kxn * lxn : 0.0f;
-      const float mzp = z + 1 < g.nz ? kzp * lzp : 0.0f;
-      const float mzn = z > 0 ? kzn * lzn : 0.0f;
-      gb += ((pxp * (m0 - mxp) - pxn * (m0 - mxn)) + pzp * (m0 - mzp)) - pzn * (m0 - mzn);
-      if (recon == 1) {
-        float un = uc[off + i];
-        if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
-        uc[off + i] = 2.0f * u0 - (un - k0 * D);
-      } else if (recon == 2) {
-        uc[off + i] = 0.0f;
+  for (; grp < g_end; ++grp) {
+    const int b0 = grp * spt, nb = min(spt, B - b0);
+    float gv = 0.0f, gb = 0.0f;
+    for (int s = 0; s < nb; ++s) {
+      const int bb = b0 + s;
+      const int64_t off = (int64_t)bb * g.plane;
+      const float* Lm = lam + off;
+      const float l0 = Lm[i];
+      const float lxp = x + 1 < g.nx ? Lm[i + 1] : 0.0f;
+      const float lxn = x > 0 ? Lm[i - 1] : 0.0f;
+      const float lzp = z + 1 < g.nz ? Lm[i + g.nx] : 0.0f;
+      const float lzn = z > 0 ? Lm[i - g.nx] : 0.0f;
+      if (in) {
+        const float* U = um + off;
+        const float u0 = U[i];
+        const float pxp = (x + 1 < g.nx ? U[i + 1] : 0.0f) - u0;
+        const float pxn = u0 - (x > 0 ? U[i - 1] : 0.0f);
+        const float pzp = (z + 1 < g.nz ? U[i + g.nx] : 0.0f) - u0;
+        const float pzn = u0 - (z > 0 ? U[i - g.nx] : 0.0f);
+        const float D = ((bxp * pxp - bxn * pxn) + bzp * pzp) - bzn * pzn;
+        gv += D * l0;
+        const float m0 = k0 * l0;
+        const float mxp = x + 1 < g.nx ? kxp * lxp : 0.0f;
+        const float mxn = x > 0 ? kxn * lxn : 0.0f;
+        const float mzp = z + 1 < g.nz ? kzp * lzp : 0.0f;
+        const float mzn = z > 0 ? kzn * lzn : 0.0f;
+        gb += ((pxp * (m0 - mxp) - pxn * (m0 - mxn)) + pzp * (m0 - mzp)) - pzn * (m0 - mzn);
+        if (recon == 1) {
+          float un = uc[off + i];
+          if (z == src[2 * bb] && x == src[2 * bb + 1]) un -= w;
+          uc[off + i] = 2.0f * u0 - (un - k0 * D);
+        } else if (recon == 2) {
+          uc[off + i] = 0.0f;
+        }
+      }
+      if (doA) {
+        const float* Y = lam + off;
+        const float m0 = a0 * l0;
+        const float mxp = x + 1 < g.nx ? axp * lxp : 0.0f;
+        const float mxn = x > 0 ? axn * lxn : 0.0f;
+        const float mzp = z + 1 < g.nz ? azp * lzp : 0.0f;
+        const float mzn = z > 0 ? azn * lzn : 0.0f;
+        const float Mm = ((bxp * (mxp - m0) + bxn * (mxn - m0)) + bzp * (mzp - m0)) +
+                         bzn * (mzn - m0);
+        float acc = (2.0f * al0) * l0 + Mm;
+        for (int o = 0; o < no; ++o) acc += oc1[o] * Y[on[o]];
+        const float* Y2 = lam1 + off;
+        acc += -al0 * Y2[i];
+        for (int o = 0; o < no; ++o) acc += oc2[o] * Y2[on[o]];
+        out[off + i] = acc;
       }
     }
-    if (doA) {
-      const float* Y = lam + off;
-      const float m0 = a0 * l0;
-      const float mxp = x + 1 < g.nx ? axp * lxp : 0.0f;
-      const float mxn = x > 0 ? axn * lxn : 0.0f;
-      const float mzp = z + 1 < g.nz ? azp * lzp : 0.0f;
-      const float mzn = z > 0 ? azn * lzn : 0.0f;
-      const float Mm = ((bxp * (mxp - m0) + bxn * (mxn - m0)) + bzp * (mzp - m0)) +
-                       bzn * (mzn - m0);
-      float acc = (2.0f * al0) * l0 + Mm;
-      for (int o = 0; o < no; ++o) acc += oc1[o] * Y[on[o]];
-      const float* Y2 = lam1 + off;
-      acc += -al0 * Y2[i];
-      for (int o = 0; o < no; ++o) acc += oc2[o] * Y2[on[o]];
-      out[off + i] = acc;
+    if (in) {
+      gpart[(int64_t)grp * g.plane + i] += ((2.0f * C) * r * c) * gv;
+      gpart[((int64_t)G + grp) * g.plane + i] += (C * (c * c)) * gv - (0.5f * bc * bc) * gb;
     }
-  }
-  if (in) {
-    gpart[(int64_t)grp * g.plane + i] += ((2.0f * C) * r * c) * gv;
-    gpart[((int64_t)G + grp) * g.plane + i] += (C * (c * c)) * gv - (0.5f * bc * bc) * gb;
   }
 }
 
@@ -1153,8 +1159,9 @@ __device__ __forceinline__ void rho_adj_grad_deep(
 }
 
 // One reverse step of the variable-density adjoint in one launch of two
-// block ranges: the frame (the cells outside Deep) one cell per thread with
-// rho_adj_grad_cell, then Deep's lane vectors with rho_adj_grad_deep.
+// block ranges: the frame (the cells outside Deep) one cell per thread for ngf
+// shot groups with rho_adj_grad_cell (its per-cell set-up once per ngf
+// groups), then Deep's lane vectors with rho_adj_grad_deep.
 template <int VX>
 __global__ __launch_bounds__(256) void k_rho_adj_grad(
     const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ buoy,
@@ -1162,13 +1169,13 @@ __global__ __launch_bounds__(256) void k_rho_adj_grad(
     float* __restrict__ out, const float* __restrict__ um, float* __restrict__ uc,
     float* __restrict__ gpart, const int32_t* __restrict__ src, const float* __restrict__ wn,
     const float* __restrict__ ring_slot, int64_t ring_len, int recon, int B, int spt, int G,
-    HabcCtx hc, float C, Deep dp, Frame f, TileMap tm) {
+    HabcCtx hc, float C, Deep dp, Frame f, int ngf, TileMap tm) {
   const int nf = f.per * kXcds;
   if ((int)blockIdx.x < nf) {
-    int z, x, grp;
-    if (frame_cell(f, (int)blockIdx.x, z, x, grp))
-      rho_adj_grad_cell(z, x, grp, vp, rho, buoy, bw, lam, lam1, out, um, uc, gpart, src, wn,
-                        ring_slot, ring_len, recon, B, spt, G, hc, C);
+    int z, x, fg;
+    if (frame_cell(f, (int)blockIdx.x, z, x, fg))
+      rho_adj_grad_cell(z, x, fg * ngf, vp, rho, buoy, bw, lam, lam1, out, um, uc, gpart, src,
+                        wn, ring_slot, ring_len, recon, B, spt, G, hc, C, ngf);
     return;
   }
   int tx, tzi, grp;
@@ -1832,7 +1839,8 @@ int seis_rho_habc_backward(const float* vp, const float* rho, const float* b,
   const Grid gd{dp.z1 - dp.z0, dp.x1 - dp.x0, 0, (int64_t)(dp.z1 - dp.z0) * (dp.x1 - dp.x0)};
   TileMap tmd = make_tiles(gd, VX, 1, G);
   if (gd.plane == 0) tmd.per = 0;
-  const Frame fr = make_frame(g, dp, G);
+  const int ngf = std::min(G, kRhoFrameGroups);
+  const Frame fr = make_frame(g, dp, (G + ngf - 1) / ngf);
   const float C = (float)(dt * dt * std::pow(h, -2.0));
   hipStream_t s = (hipStream_t)stream;
   const int64_t field = B * g.plane;
@@ -1868,17 +1876,17 @@ int seis_rho_habc_backward(const float* vp, const float* rho, const float* b,
       case 4:
         k_rho_adj_grad<4><<<grid, 256, 0, s>>>(vp, rho, buoy, b, L0, L1, next, X, Y, grad_part,
                                                 src_zx, wavelet + (kk - 1), slot, ring_len, recon,
-                                                (int)B, S, G, hc, C, dp, fr, tmd);
+                                                (int)B, S, G, hc, C, dp, fr, ngf, tmd);
         break;
       case 2:
         k_rho_adj_grad<2><<<grid, 256, 0, s>>>(vp, rho, buoy, b, L0, L1, next, X, Y, grad_part,
                                                 src_zx, wavelet + (kk - 1), slot, ring_len, recon,
-                                                (int)B, S, G, hc, C, dp, fr, tmd);
+                                                (int)B, S, G, hc, C, dp, fr, ngf, tmd);
         break;
       default:
         k_rho_adj_grad<1><<<grid, 256, 0, s>>>(vp, rho, buoy, b, L0, L1, next, X, Y, grad_part,
                                                 src_zx, wavelet + (kk - 1), slot, ring_len, recon,
-                                                (int)B, S, G, hc, C, dp, fr, tmd);
+                                                (int)B, S, G, hc, C, dp, fr, ngf, tmd);
     }
     SEIS_CHECK_LAUNCH();
     const int64_t m = kk - 2;
