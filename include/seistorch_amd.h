@@ -16,7 +16,7 @@
 
 #include <stdint.h>
 
-#define SEIS_ABI_VERSION 14
+#define SEIS_ABI_VERSION 15
 
 #ifdef __cplusplus
 extern "C" {
@@ -184,19 +184,23 @@ int seis_acoustic_backward(const float* vp, const float* d, const float* wavelet
  *   rec_out, rec_grad    (nt, R)
  *   ring                 (nt-1, B, seis_ring_len(nz, nx, npml, 0))
  *   snaps                (seis_snapshot_count(nt, snap_every), 2, B, nz, nx)
- *   u_a, u_b, mu_a, mu_b (B, nz, nx)
+ *   u_a, u_b, mu_a, mu_b (B, nz, nx); u_c, u_d too, or both null
  *   grad_part            (ceil(B/shots_per_group), nz, nx)
  * table is device scratch of n x SEIS_BATCH_COLS int64 (written here).
+ * The forward advances two time steps per launch when u_c and u_d are given
+ * (four rotating field arenas; records, rings, snapshots and the final u_a,
+ * u_b equal the one-step loop's bit for bit), one step per launch when both
+ * are null.
  * vec_x must divide every nx.  use_graph != 0 records the whole time loop
  * into one hipGraph on an internal capture stream and launches it on
  * `stream` (waits for it to finish before returning); 0 launches the
  * kernels directly on `stream`. */
-#define SEIS_BATCH_COLS 22
+#define SEIS_BATCH_COLS 23
 int seis_acoustic_forward_batch(int64_t n, const int64_t* shape, const float* dt_h,
                                 const float* vp, const float* d, const float* wavelet,
                                 const int32_t* src_zx, const int64_t* rec_flat, float* rec_out,
                                 float* ring, float* snaps, int32_t snap_every, float* u_a,
-                                float* u_b, int64_t* table, int32_t npml,
+                                float* u_b, float* u_c, float* u_d, int64_t* table, int32_t npml,
                                 int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                                 int32_t use_graph, void* stream);
 

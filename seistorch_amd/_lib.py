@@ -39,8 +39,8 @@ SIGNATURES = {
                                               _p, _p, _i64, _i64, _i64, _f, _f, _i32, _i32, _i32,
                                               _i32, _i32, _p]),
     "seis_acoustic_forward_batch": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p,
-                                                   _p, _i32, _p, _p, _p, _i32, _i32, _i32, _i32,
-                                                   _i32, _p]),
+                                                   _p, _i32, _p, _p, _p, _p, _p, _i32, _i32, _i32,
+                                                   _i32, _i32, _p]),
     "seis_acoustic_backward_batch": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                                     _p, _i32, _p, _p, _p, _p, _p, _p, _p, _i32,
                                                     _i32, _i32, _i32, _i32, _p]),
@@ -102,8 +102,8 @@ SIGNATURES = {
     "seis_unet_fill": (ctypes.c_int, [_p, _i64, _f, _p]),
 }
 
-ABI_VERSION = 14
-BATCH_COLS = 22  # SEIS_BATCH_COLS: int64 columns per instance of the batch table
+ABI_VERSION = 15
+BATCH_COLS = 23  # SEIS_BATCH_COLS: int64 columns per instance of the batch table
 
 _lock = threading.Lock()
 _lib = None

@@ -226,16 +226,14 @@ __device__ __forceinline__ void fwd2_tile(
     const float* __restrict__ vp, const float* __restrict__ d, const float* __restrict__ ua,
     const float* __restrict__ ub, float* __restrict__ uc, float* __restrict__ ud,
     const int32_t* __restrict__ src, const float* __restrict__ wn, int B, const Grid& g,
-    const seis::TimeConsts& k, const TileMap& tm, float* __restrict__ ipart) {
-  int tx, tzi, grp;
-  if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
+    const seis::TimeConsts& k, int tx, int tzi, int grp, float* __restrict__ ipart, bool two) {
   const int lane = (int)(threadIdx.x % kLanes);
   const int x0 = (tx * kFwd2Out + lane - 1) * VX;  // lane 0 starts one vector left
   const int zb = (tzi * kRowsOfThreads + (int)(threadIdx.x / kLanes)) * RZ;
   if (zb >= g.nz) return;  // uniform over the wave
   const bool xin = x0 >= 0 && x0 < g.nx;  // nx is a multiple of VX
   const bool inner = lane > 0 && lane < kLanes - 1;
-  const float w0 = wn[0], w1 = wn[1];
+  const float w0 = wn[0], w1 = two ? wn[1] : 0.0f;  // wn[1] exists only for two steps
   // coefficients on the rows zb-1 .. zb+RZ (index q = z - zb + 1)
   float A[RZ + 2][VX], Bc[RZ + 2][VX], c2[RZ + 2][VX];
 #pragma unroll
@@ -323,6 +321,7 @@ __device__ __forceinline__ void fwd2_tile(
         }
       }
     }
+    if (!two) continue;  // uniform over the block (one instance)
     // step 2 on rows zb .. zb+RZ-1 (inner lanes)
 #pragma unroll
     for (int r = 0; r < RZ; ++r) {
@@ -368,7 +367,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd2(
     const float* __restrict__ ub, float* __restrict__ uc, float* __restrict__ ud,
     const int32_t* __restrict__ src, const float* __restrict__ wn, int B, Grid g,
     seis::TimeConsts k, TileMap tm) {
-  fwd2_tile<S, RZ, VX, false>(vp, d, ua, ub, uc, ud, src, wn, B, g, k, tm, nullptr);
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
+  fwd2_tile<S, RZ, VX, false>(vp, d, ua, ub, uc, ud, src, wn, B, g, k, tx, tzi, grp, nullptr,
+                              true);
 }
 
 // k_fwd2 that also accumulates the source illumination into ipart
@@ -378,7 +380,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd2_illum(
     const float* __restrict__ ub, float* __restrict__ uc, float* __restrict__ ud,
     const int32_t* __restrict__ src, const float* __restrict__ wn, int B, Grid g,
     seis::TimeConsts k, TileMap tm, float* __restrict__ ipart) {
-  fwd2_tile<S, RZ, VX, true>(vp, d, ua, ub, uc, ud, src, wn, B, g, k, tm, ipart);
+  int tx, tzi, grp;
+  if (!tile_of(tm, tx, tzi, grp)) return;  // uniform over the block
+  fwd2_tile<S, RZ, VX, true>(vp, d, ua, ub, uc, ud, src, wn, B, g, k, tx, tzi, grp, ipart, true);
 }
 
 // Receivers of the new field into one record row, and its ring into one slot.
@@ -899,6 +903,7 @@ enum BatchCol {
   kCRingLen,
   kCGpart,     // (groups, nz, nx) gradient planes
   kCIdt2, kCIdt, kCTwoIdt2, kCIh2,  // FP32 time constants as bit patterns
+  kCTile2,     // first global tile of the row in the two-step forward's tiling
   kBatchCols
 };
 static_assert(kBatchCols == SEIS_BATCH_COLS, "batch table width");
@@ -930,7 +935,22 @@ __device__ __forceinline__ seis::TimeConsts row_consts(const int64_t* r) {
 
 // u^m of an instance lives in field arena a iff m + nt is even (so u^nt ends
 // in a, as in the single-instance loop); mu^m lives in a iff m + nt + 1 is even.
-__device__ __forceinline__ bool in_a(int64_t m, int64_t nt) { return ((m + nt) & 1) == 0; }
+__host__ __device__ __forceinline__ bool in_a(int64_t m, int64_t nt) {
+  return ((m + nt) & 1) == 0;
+}
+
+// The two-step forward keeps four levels in four arenas (a, c, d, b): u^m of
+// an instance lives in arena (m - nt) mod 4 of that order, so u^nt ends in a
+// and u^{nt-1} in b, where the one-step loop leaves them.
+__host__ __device__ __forceinline__ float* level_arena(int64_t m, int64_t nt, float* fa,
+                                                       float* fb, float* fc, float* fd) {
+  switch ((int)(((m - nt) % 4 + 4) % 4)) {
+    case 0: return fa;
+    case 1: return fc;
+    case 2: return fd;
+    default: return fb;
+  }
+}
 
 // Row of global tile t (XCD-aware order over the active tiles) and its local
 // tile coordinates.
@@ -965,17 +985,47 @@ __global__ __launch_bounds__(kBlock) void k_fwd_batch(
                       row_grid(r, npml), row_consts(r));
 }
 
-// Receivers of u^{n+1} into records row n, and its ring into slot n.
+// Global steps n and n+1 in one launch (fwd2_tile): every active row (nt > n)
+// writes u^{n+1}, and u^{n+2} where nt > n+1; tiles of kFwd2Out lane vectors
+// in x, numbered by kCTile2.
+template <int S, int RZ, int VX>
+__global__ __launch_bounds__(kBlock) void k_fwd2_batch(
+    const int64_t* __restrict__ tab, int n_active, int64_t tiles, int per, int npml,
+    const float* __restrict__ vp, const float* __restrict__ d, float* __restrict__ fa,
+    float* __restrict__ fb, float* __restrict__ fc, float* __restrict__ fd,
+    const int32_t* __restrict__ src, const float* __restrict__ wav, int64_t n) {
+  const int w = blockIdx.x;
+  const int64_t t = (int64_t)(w % kXcds) * per + w / kXcds;
+  if (t >= tiles) return;
+  const int64_t* r = tab + (int64_t)find_row(tab, n_active, kCTile2, t) * kBatchCols;
+  const int64_t local = t - r[kCTile2];
+  const int64_t nx = r[kCNx], ntz = r[kCNtz], nt = r[kCNt];
+  const int64_t ntx2 = (nx + kFwd2Out * VX - 1) / (kFwd2Out * VX);
+  const int tx = (int)(local % ntx2);
+  const int tzi = (int)((local / ntx2) % ntz);
+  const int grp = (int)(local / (ntx2 * ntz));
+  const int64_t fo = r[kCField];
+  fwd2_tile<S, RZ, VX, false>(
+      vp + r[kCModel], d + r[kCModel], level_arena(n, nt, fa, fb, fc, fd) + fo,
+      level_arena(n - 1, nt, fa, fb, fc, fd) + fo, level_arena(n + 1, nt, fa, fb, fc, fd) + fo,
+      level_arena(n + 2, nt, fa, fb, fc, fd) + fo, src + 2 * r[kCSrc], wav + r[kCWav] + n,
+      (int)r[kCB], row_grid(r, npml), row_consts(r), tx, tzi, grp, nullptr, n + 2 <= nt);
+}
+
+// Receivers of u^{n+1} into records row n, and its ring into slot n (fc null:
+// the one-step loop's two arenas; else the two-step forward's four).
 __global__ __launch_bounds__(256) void k_record_batch(
     const int64_t* __restrict__ tab, int n_active, int64_t work, int npml,
-    const float* __restrict__ fa, const float* __restrict__ fb, const int64_t* __restrict__ rec,
-    float* __restrict__ out, float* __restrict__ ring, int64_t n) {
+    float* __restrict__ fa, float* __restrict__ fb, float* __restrict__ fc,
+    float* __restrict__ fd, const int64_t* __restrict__ rec, float* __restrict__ out,
+    float* __restrict__ ring, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= work) return;
   const int64_t* r = tab + (int64_t)find_row(tab, n_active, kCAux0, t) * kBatchCols;
   const int64_t local = t - r[kCAux0];
   const int64_t R = r[kCR], nt = r[kCNt];
-  const float* u = (in_a(n + 1, nt) ? fa : fb) + r[kCField];
+  const float* u = (fc ? level_arena(n + 1, nt, fa, fb, fc, fd)
+                       : (in_a(n + 1, nt) ? fa : fb)) + r[kCField];
   if (local < R) {
     out[r[kCOut] + n * R + local] = u[rec[r[kCRec] + local]];
     return;
@@ -1091,6 +1141,7 @@ SEIS_DEFINE_LAUNCH(launch_fwd2, k_fwd2)
 SEIS_DEFINE_LAUNCH(launch_fwd2_illum, k_fwd2_illum)
 SEIS_DEFINE_LAUNCH(launch_bwd2, k_bwd2)
 SEIS_DEFINE_LAUNCH(launch_fwd_batch, k_fwd_batch)
+SEIS_DEFINE_LAUNCH(launch_fwd2_batch, k_fwd2_batch)
 SEIS_DEFINE_LAUNCH(launch_bwd_batch, k_bwd_batch)
 
 inline dim3 tile_grid(const Grid& g, int zdim) {
@@ -1228,6 +1279,7 @@ struct BatchPlan {
   std::vector<int64_t> tab;        // sorted rows, kBatchCols each
   std::vector<int64_t> nt_sorted;  // nt of the sorted rows
   std::vector<int64_t> tile_end;   // tiles of the first k sorted rows, k = 0..n
+  std::vector<int64_t> tile2_end;  // the same in the two-step forward's tiling
   std::vector<int64_t> aux_end;    // receiver + ring work of the first k sorted rows
   std::vector<int64_t> snap_off;   // per sorted row: offset into the snapshot arena
   std::vector<int64_t> groups;     // per sorted row: shot groups
@@ -1293,19 +1345,22 @@ int make_plan(BatchPlan& p, int64_t n, const int64_t* shape, const float* dt_h, 
   });
   p.tab.assign((size_t)n * kBatchCols, 0);
   p.nt_sorted.resize(n); p.snap_off.resize(n); p.groups.resize(n);
-  p.tile_end.assign(n + 1, 0); p.aux_end.assign(n + 1, 0);
+  p.tile_end.assign(n + 1, 0); p.tile2_end.assign(n + 1, 0); p.aux_end.assign(n + 1, 0);
   for (int k = 0; k < (int)n; ++k) {
     int64_t* r = p.tab.data() + (size_t)k * kBatchCols;
     std::memcpy(r, rows.data() + (size_t)order[k] * kBatchCols, sizeof(int64_t) * kBatchCols);
     r[kCTile0] = p.tile_end[k];
+    r[kCTile2] = p.tile2_end[k];
     r[kCAux0] = p.aux_end[k];
     p.tile_end[k + 1] = p.tile_end[k] + r[kCNtx] * r[kCNtz] * groups[order[k]];
+    const int64_t ntx2 = (r[kCNx] + kFwd2Out * VX - 1) / (kFwd2Out * VX);
+    p.tile2_end[k + 1] = p.tile2_end[k] + ntx2 * r[kCNtz] * groups[order[k]];
     p.aux_end[k + 1] = p.aux_end[k] + r[kCR] + r[kCB] * r[kCRingLen];
     p.nt_sorted[k] = r[kCNt];
     p.snap_off[k] = snap_off[order[k]];
     p.groups[k] = groups[order[k]];
   }
-  SEIS_CHECK_ARG(p.tile_end[n] / kXcds < (int64_t)1 << 30, "too many tiles in one batch");
+  SEIS_CHECK_ARG(p.tile2_end[n] / kXcds < (int64_t)1 << 30, "too many tiles in one batch");
   p.nt_max = p.nt_sorted[0];
   p.fields = o_field;
   p.gparts = o_gpart;
@@ -1339,7 +1394,60 @@ struct BatchArgs {
   float *ua, *ub, *ma, *mb, *gpart, *grad;
   const int64_t* table;
   int npml, S, RZ, VX;
+  float *uc = nullptr, *ud = nullptr;  // two-step forward: the third and fourth arenas
 };
+
+// Two steps per launch (k_fwd2_batch) over four arenas; each instance's
+// levels rotate through them by level_arena, so u^nt and u^{nt-1} end in ua
+// and ub.  Records, rings and snapshots are taken per level as in the
+// one-step loop.
+int enqueue_forward_batch2(const BatchPlan& p, const BatchArgs& a, hipStream_t s, bool prof) {
+  for (float* f : {a.ua, a.ub, a.uc, a.ud})
+    if (int e = fill(f, p.fields, 0.0f, s)) return e;
+  auto record = [&](int64_t m) -> int {  // u^{m+1} of the rows with nt > m
+    const int na = active_rows(p, m);
+    const int64_t work = p.aux_end[na];
+    if (work > 0) {
+      k_record_batch<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(
+          a.table, na, work, a.npml, a.ua, a.ub, a.uc, a.ud, a.rec, a.out, a.ring, m);
+      SEIS_CHECK_LAUNCH();
+    }
+    return 0;
+  };
+  auto snapshot = [&](int64_t m) -> int {  // (u^m, u^{m+1}) at m = K, 2K, ...
+    if (!a.snaps || m < a.K || m % a.K != 0) return 0;
+    const int na = active_rows(p, m);
+    for (int k = 0; k < na; ++k) {
+      const int64_t* r = p.tab.data() + (size_t)k * kBatchCols;
+      const int64_t nt = r[kCNt];
+      if (m > nt - 2) continue;
+      const int64_t sz = r[kCB] * r[kCNz] * r[kCNx];
+      float* cur = level_arena(m, nt, a.ua, a.ub, a.uc, a.ud) + r[kCField];
+      float* nxt = level_arena(m + 1, nt, a.ua, a.ub, a.uc, a.ud) + r[kCField];
+      float* slot = a.snaps + p.snap_off[k] + (m / a.K - 1) * 2 * sz;
+      if (int e = copy2(cur, slot, nxt, slot + sz, sz, s)) return e;
+    }
+    return 0;
+  };
+  for (int64_t n = 0; n < p.nt_max; n += 2) {
+    const int na = active_rows(p, n);
+    const int64_t tiles = p.tile2_end[na];
+    const int64_t per = (tiles + kXcds - 1) / kXcds;
+    if (prof) SEIS_PROF(SEIS_KERNEL_BATCH_FWD, s);
+    if (int e = launch_fwd2_batch(a.S, a.RZ, a.VX, per * kXcds, s, a.table, na, tiles, (int)per,
+                                  a.npml, a.vp, a.d, a.ua, a.ub, a.uc, a.ud, a.src, a.wavelet,
+                                  n))
+      return e;
+    if (prof) SEIS_PROF(SEIS_KERNEL_BATCH_FWD, s);
+    if (int e = record(n)) return e;
+    if (int e = snapshot(n)) return e;
+    if (n + 1 < p.nt_max) {
+      if (int e = record(n + 1)) return e;
+      if (int e = snapshot(n + 1)) return e;
+    }
+  }
+  return 0;
+}
 
 int enqueue_forward_batch(const BatchPlan& p, const BatchArgs& a, hipStream_t s, bool prof) {
   if (int e = fill(a.ua, p.fields, 0.0f, s)) return e;
@@ -1356,7 +1464,7 @@ int enqueue_forward_batch(const BatchPlan& p, const BatchArgs& a, hipStream_t s,
     const int64_t work = p.aux_end[na];
     if (work > 0) {
       k_record_batch<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(
-          a.table, na, work, a.npml, a.ua, a.ub, a.rec, a.out, a.ring, n);
+          a.table, na, work, a.npml, a.ua, a.ub, nullptr, nullptr, a.rec, a.out, a.ring, n);
       SEIS_CHECK_LAUNCH();
     }
     if (a.snaps && n >= a.K && n % a.K == 0) {
@@ -1736,7 +1844,7 @@ int seis_acoustic_forward_batch(int64_t n, const int64_t* shape, const float* dt
                                 const float* vp, const float* d, const float* wavelet,
                                 const int32_t* src_zx, const int64_t* rec_flat, float* rec_out,
                                 float* ring, float* snaps, int32_t snap_every, float* u_a,
-                                float* u_b, int64_t* table, int32_t npml,
+                                float* u_b, float* u_c, float* u_d, int64_t* table, int32_t npml,
                                 int32_t shots_per_group, int32_t rows_per_thread, int32_t vec_x,
                                 int32_t use_graph, void* stream) {
   BatchPlan p;
@@ -1752,13 +1860,20 @@ int seis_acoustic_forward_batch(int64_t n, const int64_t* shape, const float* dt
                      aligned(u_b, 4 * vec_x),
                  "field and model pointers must be %d-byte aligned for vec_x %d", 4 * vec_x,
                  vec_x);
+  SEIS_CHECK_ARG(!u_c == !u_d, "u_c and u_d are given together (two steps per launch) or not");
+  if (u_c) {
+    SEIS_CHECK_ARG(aligned(u_c, 4 * vec_x) && aligned(u_d, 4 * vec_x),
+                   "u_c and u_d must be %d-byte aligned for vec_x %d", 4 * vec_x, vec_x);
+    SEIS_CHECK_ARG(u_c != u_a && u_c != u_b && u_d != u_a && u_d != u_b && u_c != u_d,
+                   "the four field buffers must be distinct");
+  }
   hipStream_t s = (hipStream_t)stream;
   if (int e = upload_plan(p, table, s)) return e;
   BatchArgs a{vp, d, wavelet, src_zx, rec_flat, rec_out, nullptr, ring, snaps, snap_every,
               u_a, u_b, nullptr, nullptr, nullptr, nullptr, table, npml, shots_per_group,
-              rows_per_thread, vec_x};
+              rows_per_thread, vec_x, u_c, u_d};
   return run_maybe_graph(use_graph != 0, s, [&](hipStream_t st, bool prof) {
-    return enqueue_forward_batch(p, a, st, prof);
+    return u_c ? enqueue_forward_batch2(p, a, st, prof) : enqueue_forward_batch(p, a, st, prof);
   });
 }
 

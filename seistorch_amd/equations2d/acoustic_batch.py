@@ -127,7 +127,7 @@ def _hptr(t):
 class _BatchLoop(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vp_arena, d_arena, wav_arena, src_arena, rec_arena, sizes, dts, launch,
-                snapshot_every, use_graph):
+                snapshot_every, use_graph, two_step=True):
         S, rz, vx = launch
         dev = vp_arena.device
         need_grad = ctx.needs_input_grad[0]
@@ -142,14 +142,18 @@ class _BatchLoop(torch.autograd.Function):
         rec = torch.empty(lay.total["out"], **f32)
         u_a = torch.empty(lay.total["field"], **f32)
         u_b = torch.empty_like(u_a)
+        # two steps per launch need the third and fourth field arenas
+        u_c, u_d = (torch.empty_like(u_a), torch.empty_like(u_a)) if two_step else (None, None)
         ring = torch.empty(lay.total["ring"], **f32) if need_grad and lay.total["ring"] else None
         snaps = torch.empty(lay.total["snap"], **f32) if K > 0 and lay.total["snap"] else None
         table = torch.empty(n * _lib.BATCH_COLS, device=dev, dtype=torch.int64)
         _lib.call("seis_acoustic_forward_batch", n, _hptr(shape), _hptr(dt_h), _lib.ptr(vp_arena),
                   _lib.ptr(d_arena), _lib.ptr(wav_arena), _lib.ptr(src_arena),
                   _lib.ptr(rec_arena), _lib.ptr(rec), _lib.ptr(ring), _lib.ptr(snaps),
-                  K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(table),
-                  NPML, S, rz, vx, int(use_graph), _lib.stream_of(vp_arena))
+                  K if snaps is not None else 0, _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(u_c),
+                  _lib.ptr(u_d), _lib.ptr(table), NPML, S, rz, vx, int(use_graph),
+                  _lib.stream_of(vp_arena))
+        del u_c, u_d
         if need_grad:
             ctx.save_for_backward(vp_arena, d_arena, wav_arena, src_arena, rec_arena)
             ctx.state = (ring, snaps, u_a, u_b, table)
@@ -178,7 +182,7 @@ class _BatchLoop(torch.autograd.Function):
                   _lib.ptr(u_a), _lib.ptr(u_b), _lib.ptr(mu_a), _lib.ptr(mu_b), _lib.ptr(gpart),
                   _lib.ptr(grad), _lib.ptr(table), NPML, S, rz, vx, int(use_graph),
                   _lib.stream_of(vp_arena))
-        return (grad,) + (None,) * 9
+        return (grad,) + (None,) * 10
 
 
 def _launch_params(sizes, shape):
@@ -190,13 +194,15 @@ def _launch_params(sizes, shape):
 
 
 def propagate_batch(insts, shots_per_group=None, rows_per_thread=None, vec_x=None,
-                    snapshot_every=None, use_graph=True):
+                    snapshot_every=None, use_graph=True, two_step=True):
     """Records of every instance, [(nt_q, R_q)], differentiable w.r.t. each vp.
 
     shots_per_group / rows_per_thread / vec_x: launch shape shared by the whole
     batch (None: tuned or default; vec_x must divide every nx).  snapshot_every:
     one spacing for all instances (None automatic, 0 ring only).  use_graph:
     record the time loop into one hipGraph (False: direct launches).
+    two_step: the forward advances two time steps per launch (two more field
+    arenas; records, rings and snapshots bit-identical to one step per launch).
     """
     insts = list(insts)
     if not insts:
@@ -216,4 +222,4 @@ def propagate_batch(insts, shots_per_group=None, rows_per_thread=None, vec_x=Non
         src_arena = torch.cat([it.src_zx.reshape(-1) for it in insts])
         rec_arena = torch.cat([it.rec_flat.reshape(-1) for it in insts])
     return list(_BatchLoop.apply(vp_arena, d_arena, wav_arena, src_arena, rec_arena, sizes,
-                                 dts, launch, snapshot_every, bool(use_graph)))
+                                 dts, launch, snapshot_every, bool(use_graph), bool(two_step)))

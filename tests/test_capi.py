@@ -58,7 +58,7 @@ def test_batch_and_unet_argument_checks():
     from seistorch_amd import _lib
     shape = (ctypes.c_int64 * 10)(200, 300, 4, 500, 300, 220, 302, 4, 400, 302)
     dth = (ctypes.c_float * 4)(1e-3, 10.0, 1e-3, 10.0)
-    args = [None] * 8 + [0, None, None, None, 49]  # vp .. snaps, snap_every, u_a, u_b, table, npml
+    args = [None] * 8 + [0, None, None, None, None, None, 49]  # vp .. snaps, snap_every, u_a .. u_d, table, npml
     with pytest.raises(RuntimeError, match="instance count"):
         _lib.call("seis_acoustic_forward_batch", 0, shape, dth, *args, 4, 2, 4, 0, None)
     with pytest.raises(RuntimeError, match="instance 1: vec_x 4 does not divide nx = 302"):

@@ -12,13 +12,14 @@ from tests.cases import acoustic_case, rel_l2, to_dev
 pytestmark = pytest.mark.gpu
 
 # (nz_i, nx_i, shots, nt, dt, h, fm): different grids, shot counts, time axes
-# (two share nt, one stops after 2 steps, one after 1), dt and h
+# (two share nt, one stops after 2 steps, one after 1, one odd), dt and h
 SPECS = [(60, 92, 3, 300, 1e-3, 10.0, 15.0),
          (40, 60, 2, 120, 8e-4, 8.0, 20.0),
          (70, 80, 5, 300, 1e-3, 10.0, 12.0),
          (30, 40, 1, 2, 1e-3, 10.0, 15.0),
          (50, 52, 4, 1, 1e-3, 10.0, 15.0),
-         (36, 100, 6, 180, 6e-4, 6.0, 25.0)]
+         (36, 100, 6, 180, 6e-4, 6.0, 25.0),
+         (44, 64, 3, 151, 1e-3, 10.0, 15.0)]
 GRAD_TOL = 1e-6
 
 
@@ -123,6 +124,22 @@ def test_graph_replays_match_direct_launches():
         for q in range(len(insts)):
             assert torch.equal(r0[q], r1[q]), (rep, q)
             assert rel_l2(g1[q].cpu().numpy(), g0[q].cpu().numpy()) <= GRAD_TOL, (rep, q)
+
+
+@pytest.mark.parametrize("S,rz,vx,K,graph", [(4, 2, 4, 50, False), (2, 1, 1, 0, True),
+                                             (8, 4, 2, 60, False)])
+def test_two_step_forward_matches_one_step(S, rz, vx, K, graph):
+    """The batched forward at two time steps per launch (four rotating field
+    arenas; instances whose last step falls in the first half of a pair stop
+    there) against one step per launch: records bit for bit, and the
+    gradients, which read the forward's final fields, rings and snapshots."""
+    insts, weights = _instances()
+    out = {two: _batch(insts, weights, shots_per_group=S, rows_per_thread=rz, vec_x=vx,
+                       snapshot_every=K, use_graph=graph, two_step=two) for two in (True, False)}
+    for q in range(len(insts)):
+        assert torch.equal(out[True][0][q], out[False][0][q]), f"instance {q}: records differ"
+        e = rel_l2(out[True][1][q].cpu().numpy(), out[False][1][q].cpu().numpy())
+        assert e <= GRAD_TOL, (q, e)
 
 
 def test_forward_only_batch_has_no_grad_state():
