@@ -29,11 +29,11 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 BYTES_PER_CELL = {"k_fwd": 16.0, "k_bwd": 36.0, "k_fwd2": 32.0, "k_bwd2": 72.0,
                   "k_habc_fwd": 16.0, "k_habc_adj": 36.0,
                   "k_el_fwd": 52.0, "k_el_bwd": 116.0,
-                  "k_fwd_batch": 16.0, "k_bwd_batch": 36.0}
+                  "k_fwd_batch": 16.0, "k_fwd2_batch": 32.0, "k_bwd_batch": 36.0}
 KERNEL_IDS = {"acoustic": (("k_fwd", 0), ("k_bwd", 1), ("k_fwd2", 8), ("k_bwd2", 9)),
               "habc": (("k_habc_fwd", 4), ("k_habc_adj", 5)),
               "elastic": (("k_el_fwd", 2), ("k_el_bwd", 3)),
-              "generator": (("k_fwd_batch", 6), ("k_bwd_batch", 7))}
+              "generator": (("k_fwd2_batch", 6), ("k_bwd_batch", 7))}  # slot 6: the batched forward, two steps per launch
 
 
 def parse():
