@@ -70,6 +70,13 @@ def test_batch_and_unet_argument_checks():
     bad_dt = (ctypes.c_float * 4)(1e-3, 10.0, 0.0, 10.0)
     with pytest.raises(RuntimeError, match="instance 1: dt and h must be positive"):
         _lib.call("seis_acoustic_forward_batch", 2, shape, bad_dt, *args, 4, 2, 2, 0, None)
+    # the two-step forward's extra arenas come as a pair (checked before any
+    # device call; the addresses are never dereferenced)
+    fake = 1 << 20
+    no_rec = (ctypes.c_int64 * 10)(200, 300, 4, 500, 0, 220, 302, 4, 400, 0)
+    half = [fake] * 6 + [None, None, 0, fake, fake, fake, None, fake, 49]
+    with pytest.raises(RuntimeError, match="u_c and u_d are given together"):
+        _lib.call("seis_acoustic_forward_batch", 2, no_rec, dth, *half, 4, 2, 2, 0, None)
     with pytest.raises(RuntimeError, match="multiple of 8"):
         _lib.call("seis_unet_conv3x3", None, 1, 16, 16, 12, None, None, 32, 1, None, None, None,
                   None)
